@@ -1,0 +1,108 @@
+/*
+ * pefdtd -- C ABI of the MI355X (gfx950) label-indexed anisotropic elastic
+ * staggered-grid FDTD solver.
+ *
+ * The upstream solver has no C ABI: its hot path is the Python function
+ * sim/core/fdtd.py::forward_fused_labels (lines 1618-1816) launching
+ * NVRTC-compiled CUDA kernels through cupy.RawKernel.  Each entry point below
+ * names the upstream code it replaces.  The Python package
+ * pulse_echo_azimuthal_simulation_amd binds them with ctypes and keeps
+ * forward_fused_labels' signature on top.
+ *
+ * Conventions
+ *  - Every pointer inside pe_shot is a DEVICE pointer (HBM), owned by the
+ *    caller.  The library allocates nothing.
+ *  - Grids are C order (nz, ny, nx), x fastest; flat index (iz*ny+iy)*nx+ix.
+ *  - Real arrays are float (precision PE_F32) or double (PE_F64).
+ *  - Functions return PE_OK (0) or a negative PE_E* code; pe_last_error()
+ *    then holds a message (thread-local).
+ *  - `stream` is a hipStream_t (NULL = the default stream).  All calls are
+ *    asynchronous on that stream.
+ */
+#ifndef PEFDTD_H
+#define PEFDTD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PE_ABI_VERSION 1
+
+enum { PE_OK = 0, PE_EINVAL = -1, PE_EHIP = -2, PE_ERANGE = -3 };
+enum { PE_F32 = 0, PE_F64 = 1 };
+enum { PE_RECORD_PRESSURE = 0, PE_RECORD_VX = 1, PE_RECORD_VY = 2, PE_RECORD_VZ = 3 };
+/* Kernel family for the two half-steps.
+ *  PE_KERNEL_DIRECT: one thread per cell, strains materialised in `g`
+ *                    (the upstream two-kernel shape; needs g).
+ *  PE_KERNEL_TILED:  z-marching LDS-tiled kernels, strains computed on the
+ *                    fly inside the stress kernel (no g). */
+enum { PE_KERNEL_DIRECT = 0, PE_KERNEL_TILED = 1 };
+
+/* One shot: medium, fields, source and receivers.  Replaces the device state
+ * forward_fused_labels builds at sim/core/fdtd.py:1678-1784. */
+typedef struct pe_shot {
+    int32_t nz, ny, nx;          /* grid; nz*ny*nx < 2^31 */
+    int32_t nc;                  /* staggered stencil half-length, 1..4 */
+    double coeffs[4];            /* c_0..c_{nc-1}, cast to the real type */
+    double dt;                   /* time step (s) */
+    double inv_h;                /* 1 / h */
+    double dt_h;                 /* dt / h */
+    int32_t n_mat;               /* rows of Ct / rho_t */
+    int32_t precision;           /* PE_F32 or PE_F64 */
+    uint64_t act_mask;           /* bit 6I+J: some material has C_IJ != 0 */
+    const int32_t *mats;         /* material index per cell = label + 1 */
+    const void *Ct;              /* n_mat x 36 stiffness rows (C_IJ at 6I+J) */
+    const void *rho_t;           /* n_mat densities */
+    const void *damp;            /* per-cell damping factor, or NULL (= 1) */
+    void *v[3];                  /* vx, vy, vz */
+    void *s[6];                  /* s1..s6 (Voigt order xx, yy, zz, yz, xz, xy) */
+    void *g;                     /* 6*ncell strain scratch (PE_KERNEL_DIRECT) */
+    int32_t n_src;               /* source footprint cells */
+    const int64_t *src_idx;      /* flat indices, unique */
+    const void *src_w;           /* weights */
+    int32_t n_rec;               /* receiver groups */
+    const int32_t *rec_ptr;      /* n_rec+1 offsets into rec_idx / rec_w */
+    const int64_t *rec_idx;      /* flat indices */
+    const void *rec_w;           /* weights */
+    int32_t record_mode;         /* PE_RECORD_* */
+    int32_t kernel;              /* PE_KERNEL_* */
+} pe_shot;
+
+/* ABI version (PE_ABI_VERSION) and the last error message of this thread. */
+int pe_abi_version(void);
+const char *pe_last_error(void);
+
+/* Validate a shot descriptor without touching device memory. */
+int pe_check_shot(const pe_shot *shot);
+
+/* Stress half-step: strains from v, moved strains, label-indexed stiffness,
+ * deferred damping s <- s*damp + dt*C:q.
+ * Replaces strains_zp + stress_lab_u and their slab loop,
+ * sim/core/fdtd.py:1102-1177, 594-689, 1787-1797. */
+int pe_stress(const pe_shot *shot, void *stream);
+
+/* Explosive source s1..s3[k] += (real)wavelet[n] * w_k.
+ * Replaces sim/core/fdtd.py:1798-1800.  wavelet: device double array. */
+int pe_source(const pe_shot *shot, const double *wavelet, int32_t n, void *stream);
+
+/* Velocity half-step with face densities from labels and damping.
+ * Replaces vel_zp, sim/core/fdtd.py:1179-1257, 1801-1802. */
+int pe_velocity(const pe_shot *shot, void *stream);
+
+/* Receiver groups into row n of traces (nt x n_rec, real, device).
+ * Replaces sim/core/fdtd.py:1803-1813. */
+int pe_record(const pe_shot *shot, void *traces, int32_t n, void *stream);
+
+/* Steps n0..n1-1 of the time loop: stress, source, velocity, record.
+ * Replaces the hot loop sim/core/fdtd.py:1786-1815 (without the host
+ * progress callback, which the caller runs between pe_run calls). */
+int pe_run(const pe_shot *shot, int32_t n0, int32_t n1, const double *wavelet,
+           void *traces, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PEFDTD_H */

@@ -1,0 +1,255 @@
+/*
+ * CPU ORACLE body (test infrastructure only), instantiated for float and
+ * double by fdtd_oracle.c.  See that file's header for the upstream lines each
+ * step restates and for the floating-point convention.
+ *
+ * Every function that does arithmetic brackets its loops with FP_ENTER /
+ * FP_EXIT, which for the float instantiation switch the calling thread and
+ * each OpenMP worker to flush-to-zero / denormals-are-zero and restore the
+ * previous mode on exit.
+ */
+#define CAT_(a, b) a##b
+#define CAT(a, b) CAT_(a, b)
+
+/* Strains g1..g6 at every cell, from v (sim/core/fdtd.py:1136-1164).
+ * Each g is sum_j c_j * (difference) accumulated j-ascending with per-term
+ * edge guards, then scaled by 1/h. */
+static void CAT(strains, SUF)(const orc_shot *S)
+{
+    const int nz = S->nz, ny = S->ny, nx = S->nx, nc = S->nc;
+    const int64_t sy = nx, sz = (int64_t)nx * ny, ncell = sz * nz;
+    const REAL *vx = (const REAL *)S->v[0], *vy = (const REAL *)S->v[1],
+               *vz = (const REAL *)S->v[2];
+    REAL *g = (REAL *)S->g;
+    REAL c[4];
+    for (int j = 0; j < 4; ++j) c[j] = (REAL)S->coeffs[j];
+    const REAL inv_h = (REAL)S->inv_h;
+#pragma omp parallel
+    {
+        FP_ENTER
+#pragma omp for collapse(2) schedule(static)
+        for (int iz = 0; iz < nz; ++iz)
+            for (int iy = 0; iy < ny; ++iy)
+                for (int ix = 0; ix < nx; ++ix) {
+                    const int64_t i = ((int64_t)iz * ny + iy) * nx + ix;
+                    REAL a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0, a6 = 0;
+                    for (int j = 0; j < nc; ++j) {
+                        const REAL cj = c[j];
+                        if (ix >= j + 1 && ix < nx - j)
+                            a1 = FMA(cj, vx[i + j] - vx[i - (j + 1)], a1);
+                        if (iy >= j + 1 && iy < ny - j)
+                            a2 = FMA(cj, vy[i + j * sy] - vy[i - (j + 1) * sy], a2);
+                        if (iz >= j + 1 && iz < nz - j)
+                            a3 = FMA(cj, vz[i + j * sz] - vz[i - (j + 1) * sz], a3);
+                        if (iy >= j && iy < ny - j - 1)
+                            a4 = FMA(cj, vz[i + (j + 1) * sy] - vz[i - j * sy], a4);
+                        if (iz >= j && iz < nz - j - 1)
+                            a4 = FMA(cj, vy[i + (j + 1) * sz] - vy[i - j * sz], a4);
+                        if (ix >= j && ix < nx - j - 1)
+                            a5 = FMA(cj, vz[i + (j + 1)] - vz[i - j], a5);
+                        if (iz >= j && iz < nz - j - 1)
+                            a5 = FMA(cj, vx[i + (j + 1) * sz] - vx[i - j * sz], a5);
+                        if (ix >= j && ix < nx - j - 1)
+                            a6 = FMA(cj, vy[i + (j + 1)] - vy[i - j], a6);
+                        if (iy >= j && iy < ny - j - 1)
+                            a6 = FMA(cj, vx[i + (j + 1) * sy] - vx[i - j * sy], a6);
+                    }
+                    g[0 * ncell + i] = a1 * inv_h;
+                    g[1 * ncell + i] = a2 * inv_h;
+                    g[2 * ncell + i] = a3 * inv_h;
+                    g[3 * ncell + i] = a4 * inv_h;
+                    g[4 * ncell + i] = a5 * inv_h;
+                    g[5 * ncell + i] = a6 * inv_h;
+                }
+        FP_EXIT
+    }
+}
+
+/* Strain J moved from its own position to position p
+ * (sim/core/fdtd.py:612-642): per axis in z, y, x order, d > 0 takes taps
+ * {0,+1} and needs i <= n-2, d < 0 takes taps {-1,0} and needs i >= 1; the
+ * taps are summed left to right in z-major order and scaled by 1/2 or 1/4.
+ * The tap offsets and guards of each (p, J) pair come from move_table(). */
+static inline REAL CAT(moved, SUF)(const REAL *gJ, const move_t *mv, const int idx3[3],
+                                   const int dims[3], int64_t i)
+{
+    if (mv->ntap == 1) return gJ[i];
+    for (int a = 0; a < 3; ++a) {
+        if (mv->guard[a] > 0 && !(idx3[a] <= dims[a] - 2)) return (REAL)0;
+        if (mv->guard[a] < 0 && !(idx3[a] >= 1)) return (REAL)0;
+    }
+    REAL acc = gJ[i + mv->off[0]];
+    for (int t = 1; t < mv->ntap; ++t) acc = acc + gJ[i + mv->off[t]];
+    return acc * (mv->ntap == 4 ? (REAL)0.25 : (REAL)0.5);
+}
+
+/* Stress half-step with deferred damping (sim/core/fdtd.py:643-657):
+ * s_I <- fma(dt, sum_{J active} C_IJ[m] * q_{pos(I), J}, s_I * damp). */
+static void CAT(stress, SUF)(const orc_shot *S)
+{
+    const int nz = S->nz, ny = S->ny, nx = S->nx;
+    const int64_t ncell = (int64_t)nz * ny * nx;
+    const REAL *g = (const REAL *)S->g;
+    const REAL *Ct = (const REAL *)S->Ct;
+    const REAL *damp = (const REAL *)S->damp;
+    const REAL dt = (REAL)S->dt;
+    const uint64_t act = S->act_mask;
+    const int dims[3] = {nz, ny, nx};
+    move_t mv[4][6];
+    move_table(mv, nz, ny, nx);
+    int any[6];
+    for (int I = 0; I < 6; ++I) {
+        any[I] = 0;
+        for (int J = 0; J < 6; ++J)
+            if (act >> (6 * I + J) & 1u) any[I] = 1;
+    }
+#pragma omp parallel
+    {
+        FP_ENTER
+#pragma omp for collapse(2) schedule(static)
+        for (int iz = 0; iz < nz; ++iz)
+            for (int iy = 0; iy < ny; ++iy)
+                for (int ix = 0; ix < nx; ++ix) {
+                    const int64_t i = ((int64_t)iz * ny + iy) * nx + ix;
+                    const int idx3[3] = {iz, iy, ix};
+                    REAL q[4][6];
+                    for (int p = 0; p < 4; ++p)
+                        for (int J = 0; J < 6; ++J)
+                            q[p][J] = CAT(moved, SUF)(g + J * ncell, &mv[p][J], idx3, dims, i);
+                    const REAL *Cm = Ct + (int64_t)S->mats[i] * 36;
+                    const REAL dmp = damp ? damp[i] : (REAL)1;
+                    for (int I = 0; I < 6; ++I) {
+                        const int p = VP[I];
+                        REAL acc = 0;
+                        for (int J = 0; J < 6; ++J)
+                            if (act >> (6 * I + J) & 1u) acc = FMA(Cm[6 * I + J], q[p][J], acc);
+                        REAL *sI = (REAL *)S->s[I];
+                        REAL sv = sI[i] * dmp;
+                        if (any[I]) sv = FMA(dt, acc, sv);
+                        sI[i] = sv;
+                    }
+                }
+        FP_EXIT
+    }
+}
+
+/* Explosive source (sim/core/fdtd.py:1798-1800): s1..s3[k] += w[n] * w_k. */
+static void CAT(source, SUF)(const orc_shot *S, REAL wn)
+{
+    const REAL *w = (const REAL *)S->src_w;
+    FP_ENTER
+    for (int I = 0; I < 3; ++I) {
+        REAL *sI = (REAL *)S->s[I];
+        for (int k = 0; k < S->n_src; ++k) sI[S->src_idx[k]] = sI[S->src_idx[k]] + wn * w[k];
+    }
+    FP_EXIT
+}
+
+/* Velocity half-step (sim/core/fdtd.py:1211-1244): j-interleaved sums in
+ * x, y, z tap order; face density 0.5 (rho[m] + rho[m(+1)]) or rho[m] on the
+ * last slice; v <- (v + dt/h * a / rho_face) * damp. */
+static void CAT(velocity, SUF)(const orc_shot *S)
+{
+    const int nz = S->nz, ny = S->ny, nx = S->nx, nc = S->nc;
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    const REAL *s1 = (const REAL *)S->s[0], *s2 = (const REAL *)S->s[1],
+               *s3 = (const REAL *)S->s[2], *s4 = (const REAL *)S->s[3],
+               *s5 = (const REAL *)S->s[4], *s6 = (const REAL *)S->s[5];
+    REAL *vx = (REAL *)S->v[0], *vy = (REAL *)S->v[1], *vz = (REAL *)S->v[2];
+    const REAL *rho = (const REAL *)S->rho_t;
+    const REAL *damp = (const REAL *)S->damp;
+    const int32_t *mats = S->mats;
+    REAL c[4];
+    for (int j = 0; j < 4; ++j) c[j] = (REAL)S->coeffs[j];
+    const REAL dt_h = (REAL)S->dt_h;
+    const REAL half = (REAL)0.5;
+#pragma omp parallel
+    {
+        FP_ENTER
+#pragma omp for collapse(2) schedule(static)
+        for (int iz = 0; iz < nz; ++iz)
+            for (int iy = 0; iy < ny; ++iy)
+                for (int ix = 0; ix < nx; ++ix) {
+                    const int64_t i = ((int64_t)iz * ny + iy) * nx + ix;
+                    REAL ax = 0, ay = 0, az = 0;
+                    for (int j = 0; j < nc; ++j) {
+                        const REAL cj = c[j];
+                        if (ix >= j && ix < nx - j - 1)
+                            ax = FMA(cj, s1[i + (j + 1)] - s1[i - j], ax);
+                        if (ix >= j + 1 && ix < nx - j) {
+                            ay = FMA(cj, s6[i + j] - s6[i - (j + 1)], ay);
+                            az = FMA(cj, s5[i + j] - s5[i - (j + 1)], az);
+                        }
+                        if (iy >= j && iy < ny - j - 1)
+                            ay = FMA(cj, s2[i + (j + 1) * sy] - s2[i - j * sy], ay);
+                        if (iy >= j + 1 && iy < ny - j) {
+                            ax = FMA(cj, s6[i + j * sy] - s6[i - (j + 1) * sy], ax);
+                            az = FMA(cj, s4[i + j * sy] - s4[i - (j + 1) * sy], az);
+                        }
+                        if (iz >= j && iz < nz - j - 1)
+                            az = FMA(cj, s3[i + (j + 1) * sz] - s3[i - j * sz], az);
+                        if (iz >= j + 1 && iz < nz - j) {
+                            ax = FMA(cj, s5[i + j * sz] - s5[i - (j + 1) * sz], ax);
+                            ay = FMA(cj, s4[i + j * sz] - s4[i - (j + 1) * sz], ay);
+                        }
+                    }
+                    const REAL r0 = rho[mats[i]];
+                    const REAL rvx = ix < nx - 1 ? half * (r0 + rho[mats[i + 1]]) : r0;
+                    const REAL rvy = iy < ny - 1 ? half * (r0 + rho[mats[i + sy]]) : r0;
+                    const REAL rvz = iz < nz - 1 ? half * (r0 + rho[mats[i + sz]]) : r0;
+                    const REAL dmp = damp ? damp[i] : (REAL)1;
+                    vx[i] = (vx[i] + dt_h * ax / rvx) * dmp;
+                    vy[i] = (vy[i] + dt_h * ay / rvy) * dmp;
+                    vz[i] = (vz[i] + dt_h * az / rvz) * dmp;
+                }
+        FP_EXIT
+    }
+}
+
+/* Receiver groups (sim/core/fdtd.py:1803-1813): weighted sum of
+ * -(s1+s2+s3)/3 (or one velocity component) over each group's cells,
+ * accumulated in footprint order. */
+static void CAT(record, SUF)(const orc_shot *S, REAL *row)
+{
+    const REAL *s1 = (const REAL *)S->s[0], *s2 = (const REAL *)S->s[1],
+               *s3 = (const REAL *)S->s[2];
+    const REAL *w = (const REAL *)S->rec_w;
+    FP_ENTER
+    for (int r = 0; r < S->n_rec; ++r) {
+        REAL acc = 0;
+        for (int k = S->rec_ptr[r]; k < S->rec_ptr[r + 1]; ++k) {
+            const int64_t q = S->rec_idx[k];
+            REAL val;
+            if (S->record_mode == 0)
+                val = -(s1[q] + s2[q] + s3[q]) / (REAL)3;
+            else
+                val = ((const REAL *)S->v[S->record_mode - 1])[q];
+            acc = acc + val * w[k];
+        }
+        row[r] = acc;
+    }
+    FP_EXIT
+}
+
+/* Steps n0 .. n1-1 of the hot loop (sim/core/fdtd.py:1786-1815):
+ * stress, source, velocity, record.  traces is (nt, n_rec), row n written. */
+void CAT(orc_run, SUF)(const orc_shot *S, int n0, int n1, const double *wavelet,
+                       REAL *traces)
+{
+    for (int n = n0; n < n1; ++n) {
+        CAT(strains, SUF)(S);
+        CAT(stress, SUF)(S);
+        CAT(source, SUF)(S, (REAL)wavelet[n]);
+        CAT(velocity, SUF)(S);
+        if (traces) CAT(record, SUF)(S, traces + (int64_t)n * S->n_rec);
+    }
+}
+
+/* Individual half-steps, exposed for finer-grained checks. */
+void CAT(orc_stress, SUF)(const orc_shot *S) { CAT(strains, SUF)(S); CAT(stress, SUF)(S); }
+void CAT(orc_velocity, SUF)(const orc_shot *S) { CAT(velocity, SUF)(S); }
+void CAT(orc_source, SUF)(const orc_shot *S, double wn) { CAT(source, SUF)(S, (REAL)wn); }
+void CAT(orc_record, SUF)(const orc_shot *S, REAL *row) { CAT(record, SUF)(S, row); }
+
+#undef CAT
+#undef CAT_

@@ -1,0 +1,172 @@
+// C ABI of the gfx950 FDTD solver: argument checking and kernel dispatch.
+// Declarations and the upstream code each entry point replaces are in
+// include/pefdtd.h.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "common.hpp"
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    return fail(PE_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+template <typename R>
+int stress_impl(const pe_shot *p, hipStream_t st)
+{
+    const pe::Shot<R> s = pe::make_shot<R>(p);
+    hipError_t e = p->kernel == PE_KERNEL_DIRECT ? pe::launch_stress_direct<R>(s, st)
+                                                 : pe::launch_stress_tiled<R>(s, st);
+    return e == hipSuccess ? PE_OK : hip_fail(e, "stress launch");
+}
+
+template <typename R>
+int velocity_impl(const pe_shot *p, hipStream_t st)
+{
+    const pe::Shot<R> s = pe::make_shot<R>(p);
+    hipError_t e = p->kernel == PE_KERNEL_DIRECT ? pe::launch_velocity_direct<R>(s, st)
+                                                 : pe::launch_velocity_tiled<R>(s, st);
+    return e == hipSuccess ? PE_OK : hip_fail(e, "velocity launch");
+}
+
+template <typename R>
+int source_record_impl(const pe_shot *p, const double *wavelet, int32_t n, bool src,
+                       void *traces, bool rec, hipStream_t st)
+{
+    const pe::Shot<R> s = pe::make_shot<R>(p);
+    hipError_t e = pe::launch_source_record<R>(s, p, wavelet, n, src, (R *)traces, rec, st);
+    return e == hipSuccess ? PE_OK : hip_fail(e, "source/record launch");
+}
+
+template <typename R>
+int run_impl(const pe_shot *p, int32_t n0, int32_t n1, const double *wavelet, void *traces,
+             hipStream_t st)
+{
+    const pe::Shot<R> s = pe::make_shot<R>(p);
+    const bool direct = p->kernel == PE_KERNEL_DIRECT;
+    // Pressure is read from s, which the velocity half-step does not touch,
+    // so source and record share one launch before the velocity update.
+    const bool rec_before = p->record_mode == PE_RECORD_PRESSURE;
+    for (int32_t n = n0; n < n1; ++n) {
+        hipError_t e = direct ? pe::launch_stress_direct<R>(s, st) : pe::launch_stress_tiled<R>(s, st);
+        if (e != hipSuccess) return hip_fail(e, "stress launch");
+        e = pe::launch_source_record<R>(s, p, wavelet, n, true, (R *)traces,
+                                        rec_before && traces != nullptr, st);
+        if (e != hipSuccess) return hip_fail(e, "source/record launch");
+        e = direct ? pe::launch_velocity_direct<R>(s, st) : pe::launch_velocity_tiled<R>(s, st);
+        if (e != hipSuccess) return hip_fail(e, "velocity launch");
+        if (!rec_before && traces != nullptr) {
+            e = pe::launch_source_record<R>(s, p, wavelet, n, false, (R *)traces, true, st);
+            if (e != hipSuccess) return hip_fail(e, "record launch");
+        }
+    }
+    return PE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pe_abi_version(void) { return PE_ABI_VERSION; }
+
+const char *pe_last_error(void) { return g_err; }
+
+int pe_check_shot(const pe_shot *p)
+{
+    if (!p) return fail(PE_EINVAL, "shot descriptor is NULL");
+    if (p->nz < 1 || p->ny < 1 || p->nx < 1)
+        return fail(PE_EINVAL, "grid (%d, %d, %d) must be positive", p->nz, p->ny, p->nx);
+    const int64_t ncell = (int64_t)p->nz * p->ny * p->nx;
+    if (ncell >= ((int64_t)1 << 31))
+        return fail(PE_ERANGE, "grid of %lld cells exceeds the 2^31-1 cell limit",
+                    (long long)ncell);
+    if (p->nc < 1 || p->nc > 4)
+        return fail(PE_EINVAL,
+                    "nc = %d stencil coefficients given: the kernels support order <= 8 "
+                    "(1..4 coefficients)", p->nc);
+    if (p->precision != PE_F32 && p->precision != PE_F64)
+        return fail(PE_EINVAL, "precision %d is neither PE_F32 nor PE_F64", p->precision);
+    if (p->kernel != PE_KERNEL_DIRECT && p->kernel != PE_KERNEL_TILED)
+        return fail(PE_EINVAL, "unknown kernel family %d", p->kernel);
+    if (p->n_mat < 1) return fail(PE_EINVAL, "n_mat = %d must be >= 1", p->n_mat);
+    if (!p->mats || !p->Ct || !p->rho_t) return fail(PE_EINVAL, "mats, Ct and rho_t are required");
+    for (int k = 0; k < 3; ++k)
+        if (!p->v[k]) return fail(PE_EINVAL, "velocity field %d is NULL", k);
+    for (int k = 0; k < 6; ++k)
+        if (!p->s[k]) return fail(PE_EINVAL, "stress field %d is NULL", k + 1);
+    if (p->kernel == PE_KERNEL_DIRECT && !p->g)
+        return fail(PE_EINVAL, "PE_KERNEL_DIRECT needs the 6*ncell strain scratch g");
+    if (p->n_src < 0 || (p->n_src > 0 && (!p->src_idx || !p->src_w)))
+        return fail(PE_EINVAL, "source footprint arrays missing");
+    if (p->n_rec < 0 || (p->n_rec > 0 && (!p->rec_ptr || !p->rec_idx || !p->rec_w)))
+        return fail(PE_EINVAL, "receiver arrays missing");
+    if (p->record_mode < PE_RECORD_PRESSURE || p->record_mode > PE_RECORD_VZ)
+        return fail(PE_EINVAL, "record mode %d unknown", p->record_mode);
+    return PE_OK;
+}
+
+int pe_stress(const pe_shot *p, void *stream)
+{
+    int rc = pe_check_shot(p);
+    if (rc) return rc;
+    return p->precision == PE_F32 ? stress_impl<float>(p, (hipStream_t)stream)
+                                  : stress_impl<double>(p, (hipStream_t)stream);
+}
+
+int pe_velocity(const pe_shot *p, void *stream)
+{
+    int rc = pe_check_shot(p);
+    if (rc) return rc;
+    return p->precision == PE_F32 ? velocity_impl<float>(p, (hipStream_t)stream)
+                                  : velocity_impl<double>(p, (hipStream_t)stream);
+}
+
+int pe_source(const pe_shot *p, const double *wavelet, int32_t n, void *stream)
+{
+    int rc = pe_check_shot(p);
+    if (rc) return rc;
+    if (!wavelet) return fail(PE_EINVAL, "wavelet is NULL");
+    if (n < 0) return fail(PE_EINVAL, "step %d is negative", n);
+    return p->precision == PE_F32
+               ? source_record_impl<float>(p, wavelet, n, true, nullptr, false, (hipStream_t)stream)
+               : source_record_impl<double>(p, wavelet, n, true, nullptr, false, (hipStream_t)stream);
+}
+
+int pe_record(const pe_shot *p, void *traces, int32_t n, void *stream)
+{
+    int rc = pe_check_shot(p);
+    if (rc) return rc;
+    if (!traces) return fail(PE_EINVAL, "traces is NULL");
+    if (n < 0) return fail(PE_EINVAL, "step %d is negative", n);
+    return p->precision == PE_F32
+               ? source_record_impl<float>(p, nullptr, n, false, traces, true, (hipStream_t)stream)
+               : source_record_impl<double>(p, nullptr, n, false, traces, true, (hipStream_t)stream);
+}
+
+int pe_run(const pe_shot *p, int32_t n0, int32_t n1, const double *wavelet, void *traces,
+           void *stream)
+{
+    int rc = pe_check_shot(p);
+    if (rc) return rc;
+    if (n0 < 0 || n1 < n0) return fail(PE_EINVAL, "bad step range [%d, %d)", n0, n1);
+    if (!wavelet && p->n_src > 0) return fail(PE_EINVAL, "wavelet is NULL");
+    return p->precision == PE_F32
+               ? run_impl<float>(p, n0, n1, wavelet, traces, (hipStream_t)stream)
+               : run_impl<double>(p, n0, n1, wavelet, traces, (hipStream_t)stream);
+}
+
+}  // extern "C"

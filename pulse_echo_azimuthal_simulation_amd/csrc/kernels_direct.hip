@@ -1,0 +1,306 @@
+// One-thread-per-cell kernels (PE_KERNEL_DIRECT) and the source/receiver
+// kernel shared by both kernel families.
+//
+// The direct family mirrors the upstream two-kernel shape (strains into a
+// scratch volume, then stress; sim/core/fdtd.py:1102-1177, 594-689) and the
+// velocity update (sim/core/fdtd.py:1179-1257).  It is the simplest correct
+// form of the step and serves as the cross-check for the tiled family.
+//
+// Arithmetic follows the upstream statement order exactly: every
+// "acc += c * x" is one fused multiply-add; the stress update is
+// fma(dt, acc, s * damp); the velocity update is (v + (dt_h * a) / rho_face)
+// * damp.  Built with -ffp-contract=off so nothing else is fused, and with
+// f32 subnormals flushed, which reproduces the upstream float32 archives.
+#include "common.hpp"
+
+namespace pe {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// Strain J moved onto position P at flat index i (sim/core/fdtd.py:612-642):
+// per axis (z, y, x) with d = pos[P] - pos[J]: d > 0 taps {0,+1} and needs
+// i <= n-2; d < 0 taps {-1,0} and needs i >= 1.  Taps are summed z-major,
+// left to right, then scaled by 1/2 or 1/4; 0 if any guard fails.
+template <int P, int J, typename R>
+__device__ __forceinline__ R moved_direct(const R *__restrict__ gJ, int i, int iz, int iy,
+                                          int ix, int nz, int ny, int nx, int sy, int sz)
+{
+    constexpr int src = kVP[J];
+    if constexpr (src == P) {
+        return gJ[i];
+    } else {
+        constexpr int dz = kPos[P][0] - kPos[src][0];
+        constexpr int dy = kPos[P][1] - kPos[src][1];
+        constexpr int dx = kPos[P][2] - kPos[src][2];
+        bool ok = true;
+        if constexpr (dz > 0) ok = ok && (iz <= nz - 2);
+        if constexpr (dz < 0) ok = ok && (iz >= 1);
+        if constexpr (dy > 0) ok = ok && (iy <= ny - 2);
+        if constexpr (dy < 0) ok = ok && (iy >= 1);
+        if constexpr (dx > 0) ok = ok && (ix <= nx - 2);
+        if constexpr (dx < 0) ok = ok && (ix >= 1);
+        if (!ok) return (R)0;
+        constexpr int z0 = dz > 0 ? 0 : (dz < 0 ? -1 : 0), nzt = dz != 0 ? 2 : 1;
+        constexpr int y0 = dy > 0 ? 0 : (dy < 0 ? -1 : 0), nyt = dy != 0 ? 2 : 1;
+        constexpr int x0 = dx > 0 ? 0 : (dx < 0 ? -1 : 0), nxt = dx != 0 ? 2 : 1;
+        R acc = (R)0;
+        bool first = true;
+#pragma unroll
+        for (int a = 0; a < nzt; ++a)
+#pragma unroll
+            for (int b = 0; b < nyt; ++b)
+#pragma unroll
+                for (int c = 0; c < nxt; ++c) {
+                    const R t = gJ[i + (z0 + a) * sz + (y0 + b) * sy + (x0 + c)];
+                    acc = first ? t : acc + t;
+                    first = false;
+                }
+        constexpr int ntap = nzt * nyt * nxt;
+        return acc * (ntap == 4 ? (R)0.25 : (R)0.5);
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_strain_direct(Shot<R> S)
+{
+    const int nz = S.nz, ny = S.ny, nx = S.nx;
+    const int sy = nx, sz = nx * ny;
+    const int ncell = sz * nz;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= ncell) return;
+    const int ix = i % nx, iy = (i / nx) % ny, iz = i / sz;
+    const R *__restrict__ vx = S.v[0];
+    const R *__restrict__ vy = S.v[1];
+    const R *__restrict__ vz = S.v[2];
+    R a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0, a6 = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (j >= S.nc) break;
+        const R cj = S.c[j];
+        if (ix >= j + 1 && ix < nx - j) a1 = fma_r(cj, vx[i + j] - vx[i - (j + 1)], a1);
+        if (iy >= j + 1 && iy < ny - j) a2 = fma_r(cj, vy[i + j * sy] - vy[i - (j + 1) * sy], a2);
+        if (iz >= j + 1 && iz < nz - j) a3 = fma_r(cj, vz[i + j * sz] - vz[i - (j + 1) * sz], a3);
+        if (iy >= j && iy < ny - j - 1) a4 = fma_r(cj, vz[i + (j + 1) * sy] - vz[i - j * sy], a4);
+        if (iz >= j && iz < nz - j - 1) a4 = fma_r(cj, vy[i + (j + 1) * sz] - vy[i - j * sz], a4);
+        if (ix >= j && ix < nx - j - 1) a5 = fma_r(cj, vz[i + (j + 1)] - vz[i - j], a5);
+        if (iz >= j && iz < nz - j - 1) a5 = fma_r(cj, vx[i + (j + 1) * sz] - vx[i - j * sz], a5);
+        if (ix >= j && ix < nx - j - 1) a6 = fma_r(cj, vy[i + (j + 1)] - vy[i - j], a6);
+        if (iy >= j && iy < ny - j - 1) a6 = fma_r(cj, vx[i + (j + 1) * sy] - vx[i - j * sy], a6);
+    }
+    R *__restrict__ g = S.g;
+    const size_t n = (size_t)ncell;
+    g[0 * n + i] = a1 * S.inv_h;
+    g[1 * n + i] = a2 * S.inv_h;
+    g[2 * n + i] = a3 * S.inv_h;
+    g[3 * n + i] = a4 * S.inv_h;
+    g[4 * n + i] = a5 * S.inv_h;
+    g[5 * n + i] = a6 * S.inv_h;
+}
+
+// Stress component I at position P = kVP[I]: acc = sum over active J of
+// C_IJ * q_{P,J} (J ascending), then s = fma(dt, acc, s * damp).
+template <int I, typename R>
+__device__ __forceinline__ void stress_component(const Shot<R> &S, const R *__restrict__ Cm,
+                                                 const R (&q)[6], R dmp, int i)
+{
+    R acc = 0;
+    bool any = false;
+#pragma unroll
+    for (int J = 0; J < 6; ++J) {
+        if ((S.act >> (6 * I + J)) & 1ull) {
+            acc = fma_r(Cm[6 * I + J], q[J], acc);
+            any = true;
+        }
+    }
+    R *__restrict__ sI = S.s[I];
+    R sv = sI[i] * dmp;
+    if (any) sv = fma_r(S.dt, acc, sv);
+    sI[i] = sv;
+}
+
+template <int P, typename R>
+__device__ __forceinline__ void gather_q(const Shot<R> &S, R (&q)[6], int i, int iz, int iy,
+                                         int ix)
+{
+    const int nz = S.nz, ny = S.ny, nx = S.nx, sy = nx, sz = nx * ny;
+    const size_t n = (size_t)nz * sz;
+    const R *g = S.g;
+    q[0] = moved_direct<P, 0>(g + 0 * n, i, iz, iy, ix, nz, ny, nx, sy, sz);
+    q[1] = moved_direct<P, 1>(g + 1 * n, i, iz, iy, ix, nz, ny, nx, sy, sz);
+    q[2] = moved_direct<P, 2>(g + 2 * n, i, iz, iy, ix, nz, ny, nx, sy, sz);
+    q[3] = moved_direct<P, 3>(g + 3 * n, i, iz, iy, ix, nz, ny, nx, sy, sz);
+    q[4] = moved_direct<P, 4>(g + 4 * n, i, iz, iy, ix, nz, ny, nx, sy, sz);
+    q[5] = moved_direct<P, 5>(g + 5 * n, i, iz, iy, ix, nz, ny, nx, sy, sz);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_stress_direct(Shot<R> S)
+{
+    const int nz = S.nz, ny = S.ny, nx = S.nx;
+    const int sz = nx * ny;
+    const int ncell = sz * nz;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= ncell) return;
+    const int ix = i % nx, iy = (i / nx) % ny, iz = i / sz;
+    const R *__restrict__ Cm = S.Ct + (size_t)S.mats[i] * 36;
+    const R dmp = S.damp ? S.damp[i] : (R)1;
+    R q[6];
+    gather_q<0>(S, q, i, iz, iy, ix);
+    stress_component<0>(S, Cm, q, dmp, i);
+    stress_component<1>(S, Cm, q, dmp, i);
+    stress_component<2>(S, Cm, q, dmp, i);
+    gather_q<1>(S, q, i, iz, iy, ix);
+    stress_component<3>(S, Cm, q, dmp, i);
+    gather_q<2>(S, q, i, iz, iy, ix);
+    stress_component<4>(S, Cm, q, dmp, i);
+    gather_q<3>(S, q, i, iz, iy, ix);
+    stress_component<5>(S, Cm, q, dmp, i);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_velocity_direct(Shot<R> S)
+{
+    const int nz = S.nz, ny = S.ny, nx = S.nx;
+    const int sy = nx, sz = nx * ny;
+    const int ncell = sz * nz;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= ncell) return;
+    const int ix = i % nx, iy = (i / nx) % ny, iz = i / sz;
+    const R *__restrict__ s1 = S.s[0];
+    const R *__restrict__ s2 = S.s[1];
+    const R *__restrict__ s3 = S.s[2];
+    const R *__restrict__ s4 = S.s[3];
+    const R *__restrict__ s5 = S.s[4];
+    const R *__restrict__ s6 = S.s[5];
+    R ax = 0, ay = 0, az = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (j >= S.nc) break;
+        const R cj = S.c[j];
+        if (ix >= j && ix < nx - j - 1) ax = fma_r(cj, s1[i + (j + 1)] - s1[i - j], ax);
+        if (ix >= j + 1 && ix < nx - j) {
+            ay = fma_r(cj, s6[i + j] - s6[i - (j + 1)], ay);
+            az = fma_r(cj, s5[i + j] - s5[i - (j + 1)], az);
+        }
+        if (iy >= j && iy < ny - j - 1) ay = fma_r(cj, s2[i + (j + 1) * sy] - s2[i - j * sy], ay);
+        if (iy >= j + 1 && iy < ny - j) {
+            ax = fma_r(cj, s6[i + j * sy] - s6[i - (j + 1) * sy], ax);
+            az = fma_r(cj, s4[i + j * sy] - s4[i - (j + 1) * sy], az);
+        }
+        if (iz >= j && iz < nz - j - 1) az = fma_r(cj, s3[i + (j + 1) * sz] - s3[i - j * sz], az);
+        if (iz >= j + 1 && iz < nz - j) {
+            ax = fma_r(cj, s5[i + j * sz] - s5[i - (j + 1) * sz], ax);
+            ay = fma_r(cj, s4[i + j * sz] - s4[i - (j + 1) * sz], ay);
+        }
+    }
+    const R *__restrict__ rho = S.rho;
+    const int32_t *__restrict__ mats = S.mats;
+    const R r0 = rho[mats[i]];
+    const R rvx = ix < nx - 1 ? (R)0.5 * (r0 + rho[mats[i + 1]]) : r0;
+    const R rvy = iy < ny - 1 ? (R)0.5 * (r0 + rho[mats[i + sy]]) : r0;
+    const R rvz = iz < nz - 1 ? (R)0.5 * (r0 + rho[mats[i + sz]]) : r0;
+    const R dmp = S.damp ? S.damp[i] : (R)1;
+    R *__restrict__ vx = S.v[0];
+    R *__restrict__ vy = S.v[1];
+    R *__restrict__ vz = S.v[2];
+    vx[i] = (vx[i] + S.dt_h * ax / rvx) * dmp;
+    vy[i] = (vy[i] + S.dt_h * ay / rvy) * dmp;
+    vz[i] = (vz[i] + S.dt_h * az / rvz) * dmp;
+}
+
+// Source injection and receiver recording in one single-workgroup launch.
+// Phase 1 adds (real)wavelet[n] * w_k to s1..s3 at the (unique) footprint
+// cells (sim/core/fdtd.py:1798-1800).  Phase 2, after a barrier, forms each
+// receiver group's weighted sum of -(s1+s2+s3)/3 or of one velocity
+// component (sim/core/fdtd.py:1803-1813): wave w takes groups w, w+nw, ...;
+// lane l sums cells l, l+64, ... in order; the 64 partials are combined by a
+// fixed butterfly, so the result is deterministic.
+constexpr int kSrBlock = 1024;
+
+template <typename R>
+__global__ __launch_bounds__(kSrBlock) void k_source_record(
+    Shot<R> S, int n_src, const int64_t *__restrict__ src_idx, const R *__restrict__ src_w,
+    const double *__restrict__ wavelet, int n, int do_source, int n_rec,
+    const int32_t *__restrict__ rec_ptr, const int64_t *__restrict__ rec_idx,
+    const R *__restrict__ rec_w, int mode, R *__restrict__ traces, int do_record)
+{
+    if (do_source) {
+        const R wn = (R)wavelet[n];
+        for (int k = threadIdx.x; k < n_src; k += kSrBlock) {
+            const int64_t q = src_idx[k];
+            const R add = wn * src_w[k];
+            S.s[0][q] = S.s[0][q] + add;
+            S.s[1][q] = S.s[1][q] + add;
+            S.s[2][q] = S.s[2][q] + add;
+        }
+        __syncthreads();
+    }
+    if (!do_record) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int nwave = kSrBlock / 64;
+    for (int r = wave; r < n_rec; r += nwave) {
+        R acc = 0;
+        for (int k = rec_ptr[r] + lane; k < rec_ptr[r + 1]; k += 64) {
+            const int64_t q = rec_idx[k];
+            R val;
+            if (mode == PE_RECORD_PRESSURE)
+                val = -(S.s[0][q] + S.s[1][q] + S.s[2][q]) / (R)3;
+            else
+                val = S.v[mode - 1][q];
+            acc = acc + val * rec_w[k];
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) acc = acc + __shfl_xor(acc, off, 64);
+        if (lane == 0) traces[(size_t)n * n_rec + r] = acc;
+    }
+}
+
+inline unsigned blocks_for(int ncell) { return (unsigned)((ncell + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+template <typename R>
+hipError_t launch_stress_direct(const Shot<R> &s, hipStream_t st)
+{
+    const int ncell = s.nz * s.ny * s.nx;
+    hipLaunchKernelGGL(k_strain_direct<R>, dim3(blocks_for(ncell)), dim3(kBlock), 0, st, s);
+    hipLaunchKernelGGL(k_stress_direct<R>, dim3(blocks_for(ncell)), dim3(kBlock), 0, st, s);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_velocity_direct(const Shot<R> &s, hipStream_t st)
+{
+    const int ncell = s.nz * s.ny * s.nx;
+    hipLaunchKernelGGL(k_velocity_direct<R>, dim3(blocks_for(ncell)), dim3(kBlock), 0, st, s);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_source_record(const Shot<R> &s, const pe_shot *p, const double *wavelet,
+                                int32_t n, bool do_source, R *traces, bool do_record,
+                                hipStream_t st)
+{
+    if (!(do_source && p->n_src > 0) && !(do_record && p->n_rec > 0)) return hipSuccess;
+    hipLaunchKernelGGL(k_source_record<R>, dim3(1), dim3(kSrBlock), 0, st, s, p->n_src,
+                       p->src_idx, (const R *)p->src_w, wavelet, n,
+                       (int)(do_source && p->n_src > 0), p->n_rec, p->rec_ptr, p->rec_idx,
+                       (const R *)p->rec_w, p->record_mode, traces,
+                       (int)(do_record && p->n_rec > 0));
+    return hipGetLastError();
+}
+
+template hipError_t launch_stress_direct<float>(const Shot<float> &, hipStream_t);
+template hipError_t launch_stress_direct<double>(const Shot<double> &, hipStream_t);
+template hipError_t launch_velocity_direct<float>(const Shot<float> &, hipStream_t);
+template hipError_t launch_velocity_direct<double>(const Shot<double> &, hipStream_t);
+template hipError_t launch_source_record<float>(const Shot<float> &, const pe_shot *,
+                                                const double *, int32_t, bool, float *, bool,
+                                                hipStream_t);
+template hipError_t launch_source_record<double>(const Shot<double> &, const pe_shot *,
+                                                 const double *, int32_t, bool, double *, bool,
+                                                 hipStream_t);
+
+}  // namespace pe

@@ -1,0 +1,157 @@
+"""GPU parity of the gfx950 kernels (through the C ABI) against the CPU
+oracle and the upstream float32 solver archives.
+
+Bars: float32 kernels reproduce the oracle's fields bit for bit (both are
+the same arithmetic with f32 subnormals flushed) and the upstream archives
+to rel-L2 <= 1e-5 (observed: bit-exact); float64 kernels reproduce the f64
+oracle bit for bit.  Multi-cell receiver groups are sums in a different
+order than the oracle's and are held to 1e-6 relative.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tests import helpers as H
+from tests.golden import recipes as R
+
+pytestmark = pytest.mark.gpu
+
+KERNELS = ["direct", "tiled"]
+
+
+@pytest.fixture(scope="module")
+def torch_gpu():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch
+
+
+def device_run(inp, kernel, precision="f32", record="pressure", nt=None):
+    from pulse_echo_azimuthal_simulation_amd.solver import DeviceShot
+    sh = DeviceShot(inp["mats"], inp["Ct"], inp["rho_t"], inp["damp"], inp["coeffs"],
+                    inp["h"], inp["dt"], inp["src"][0], inp["src"][1], *inp["rec"],
+                    record=record, precision=precision, kernel=kernel)
+    nt = inp["nt"] if nt is None else nt
+    wav = sh.wavelet(inp["wavelet"])
+    tr = sh.new_traces(nt)
+    sh.run(wav, 0, nt, tr)
+    v, s = sh.fields()
+    return tr.cpu().numpy(), v, s
+
+
+def oracle_run(inp, precision="f32", record="pressure", nt=None):
+    from oracle import oracle as O
+    sh = O.OracleShot(inp["mats"], inp["Ct"], inp["rho_t"], inp["damp"], inp["coeffs"],
+                      inp["h"], inp["dt"], inp["src"], inp["rec"], record, precision)
+    nt = inp["nt"] if nt is None else nt
+    tr = sh.run(inp["wavelet"], 0, nt)
+    return tr, sh.v, sh.s
+
+
+def _print_case(name):
+    print("\n[gpu case] %s" % name, flush=True)
+
+
+# ----------------------------------------------------------------- archives
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("med,ppw", [("iso", 4), ("ani", 4), ("iso", 6), ("ani", 6),
+                                     ("iso", 10), ("ani", 10)])
+def test_green_archive(torch_gpu, med, ppw, kernel):
+    _print_case("green %s ppw%d %s" % (med, ppw, kernel))
+    case = R.green_case(med, ppw)
+    inp = H.fixture_inputs(case)
+    tr, _, _ = device_run(inp, kernel)
+    E = case["expected"]
+    err = H.rel_l2(tr, E)
+    print("rel-L2 %.3e  bit-exact %.4f" % (err, np.mean(tr == E.astype(np.float32))))
+    assert err <= 1e-5
+    assert np.array_equal(tr, E.astype(np.float32))
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_lattice_archive(torch_gpu, kernel):
+    _print_case("lattice ppw6 %s" % kernel)
+    case = R.lattice_case(6)
+    inp = H.fixture_inputs(case)
+    tr, _, _ = device_run(inp, kernel)
+    E = case["expected"]
+    assert H.rel_l2(tr, E) <= 1e-5
+    assert np.array_equal(tr, E.astype(np.float32))
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("ppw", [6, 8, 10])
+@pytest.mark.parametrize("theta", [0, 30, 45, 60])
+@pytest.mark.parametrize("homog", [False, True])
+def test_twoway_archive(torch_gpu, ppw, theta, homog, kernel):
+    _print_case("twoway ppw%d theta%d %s %s" % (ppw, theta, "ho" if homog else "bi", kernel))
+    case = R.twoway_case(theta, ppw, homog)
+    inp = H.fixture_inputs(case)
+    tr, _, _ = device_run(inp, kernel)
+    E = case["expected"]
+    err = H.rel_l2(tr, E)
+    print("rel-L2 %.3e" % err)
+    assert err <= 1e-5
+    assert np.array_equal(tr, E.astype(np.float32))
+
+
+# ------------------------------------------------------- oracle, synthetic
+SHAPES = [(12, 13, 14), (7, 9, 70), (20, 6, 33), (5, 40, 9), (1, 8, 9), (16, 16, 64)]
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_random_medium_fields(torch_gpu, shape, precision, kernel):
+    _print_case("random %s %s %s" % (shape, precision, kernel))
+    inp = H.random_medium(shape, seed=sum(shape), sponge=2, fluid_pad=1)
+    tr_d, v_d, s_d = device_run(inp, kernel, precision)
+    tr_o, v_o, s_o = oracle_run(inp, precision)
+    for a, b in zip(v_d + s_d, v_o + s_o):
+        assert np.array_equal(a, b)
+    assert H.rel_l2(tr_d, tr_o) <= 1e-6
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("order", [2, 4, 6])
+def test_low_orders(torch_gpu, order, kernel):
+    _print_case("order %d %s" % (order, kernel))
+    inp = H.random_medium((10, 11, 37), order=order, seed=order)
+    _, v_d, s_d = device_run(inp, kernel)
+    _, v_o, s_o = oracle_run(inp)
+    for a, b in zip(v_d + s_d, v_o + s_o):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("record", ["vx", "vy", "vz"])
+def test_record_modes(torch_gpu, record):
+    _print_case("record %s" % record)
+    inp = H.random_medium((12, 14, 40), seed=3)
+    tr_d, _, _ = device_run(inp, "tiled", record=record)
+    tr_o, _, _ = oracle_run(inp, record=record)
+    assert np.abs(tr_o).max() > 0
+    assert H.rel_l2(tr_d, tr_o) <= 1e-6
+
+
+def test_f32_vs_f64_error(torch_gpu):
+    """States the FP32 kernels' distance from an FP64 run of the same kernels."""
+    _print_case("f32 vs f64 ani ppw4")
+    case = R.green_case("ani", 4)
+    inp = H.fixture_inputs(case)
+    t32, _, _ = device_run(inp, "tiled", "f32")
+    t64, _, _ = device_run(inp, "tiled", "f64")
+    err = H.rel_l2(t32, t64)
+    print("FP32 vs FP64 rel-L2 = %.3e" % err)
+    assert err < 1e-5
+
+
+def test_tiled_matches_direct_midsize(torch_gpu):
+    _print_case("tiled vs direct 48x96x96")
+    inp = H.random_medium((48, 96, 96), n_grain=40, seed=5, sponge=6, fluid_pad=4)
+    inp["nt"] = 30
+    _, v_a, s_a = device_run(inp, "tiled")
+    _, v_b, s_b = device_run(inp, "direct")
+    for a, b in zip(v_a + s_a, v_b + s_b):
+        assert np.array_equal(a, b)
