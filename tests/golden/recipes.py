@@ -192,6 +192,14 @@ def twoway_geometry(theta_deg, ppw):
     return dict(R=R, nrm=nrm, h=h, N=N, c=c, d=d, x0=x0, p_src=p_src)
 
 
+@functools.lru_cache(maxsize=4)
+def twoway_labels(theta, ppw):
+    """Naive-treatment bin labels of the tilted plane (twoway_ref.py:157-158)."""
+    g = twoway_geometry(theta, ppw)
+    fr = frac_field((g["N"],) * 3, g["h"], g["nrm"], g["x0"])
+    return np.rint(fr * (NBIN - 1)).astype(np.int32)
+
+
 def twoway_case(theta, ppw, homog, psi_a=0.0, psi_b=51.0):
     """Inputs of the bicrystal (homog=False) or homogeneous twoway run; the
     homogeneous run reuses the bicrystal's dt (rsg_twoway_run.py:95-124,
@@ -201,8 +209,7 @@ def twoway_case(theta, ppw, homog, psi_a=0.0, psi_b=51.0):
     ca = R @ np.array([np.cos(np.radians(psi_a)), 0.0, np.sin(np.radians(psi_a))])
     cb = R @ np.array([np.cos(np.radians(psi_b)), 0.0, np.sin(np.radians(psi_b))])
     CA, CB = grain_C(ca), grain_C(cb if not homog else ca)
-    fr = frac_field((N, N, N), h, nrm, g["x0"])
-    lab = np.rint(fr * (NBIN - 1)).astype(np.int32)
+    lab = twoway_labels(theta, ppw)
     Ct, rho_t = naive_tables(CA, CB)
     ref = load("rsg_twoway_ssg_ppw%d.npz" % ppw)
     key = "%d" % theta

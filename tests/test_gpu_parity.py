@@ -81,10 +81,12 @@ def test_lattice_archive(torch_gpu, kernel):
     assert np.array_equal(tr, E.astype(np.float32))
 
 
+TWOWAY = [(6, t) for t in (0, 30, 45, 60)] + [(8, t) for t in (0, 30, 45, 60)] + [(10, 0), (10, 45)]
+
+
 @pytest.mark.parametrize("kernel", KERNELS)
-@pytest.mark.parametrize("ppw", [6, 8, 10])
-@pytest.mark.parametrize("theta", [0, 30, 45, 60])
 @pytest.mark.parametrize("homog", [False, True])
+@pytest.mark.parametrize("ppw,theta", TWOWAY)
 def test_twoway_archive(torch_gpu, ppw, theta, homog, kernel):
     _print_case("twoway ppw%d theta%d %s %s" % (ppw, theta, "ho" if homog else "bi", kernel))
     case = R.twoway_case(theta, ppw, homog)

@@ -1,0 +1,188 @@
+"""Host-side setup helpers, pinned against the upstream logs, archives and
+closed forms."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from pulse_echo_azimuthal_simulation_amd import host, materials
+from tests.golden import recipes as R
+
+
+# ----------------------------------------------------------- coefficients
+def test_taylor_coeffs_textbook():
+    assert np.allclose(host.taylor_coeffs(2), [1.0])
+    assert np.allclose(host.taylor_coeffs(4), [9 / 8, -1 / 24], rtol=0, atol=1e-15)
+    c6 = host.taylor_coeffs(6)
+    assert np.allclose(c6, [75 / 64, -25 / 384, 3 / 640], rtol=0, atol=1e-15)
+
+
+def test_kh2_multistart_matches_upstream_log():
+    # sim/results/rsg_symbol.log:3-4 (order 8, kh_max 2.0, multistart)
+    c = R.coeffs_kh2()
+    assert np.allclose(c, [1.228845, -0.101788, 0.019095, -0.002708], rtol=0, atol=5e-7)
+    assert abs(np.abs(c).sum() - 1.352436) < 5e-7
+    # consistency: exact in the long-wave limit
+    assert abs((2 * np.arange(4) + 1) @ c - 1.0) < 1e-14
+
+
+@pytest.mark.parametrize("multistart,peak", [(False, 4.362e-4), (True, 5.989e-5)])
+def test_kh16_peak_errors_match_upstream(multistart, peak):
+    # peak phase errors quoted at sim/core/fdtd.py:102-106
+    c = host.optimised_coeffs(8, kh_max=1.6, multistart=multistart)
+    assert abs(host.stencil_peak_error(c, 1.6) - peak) / peak < 2e-3
+    assert abs((2 * np.arange(4) + 1) @ c - 1.0) < 1e-14
+
+
+def test_order2_is_taylor():
+    assert np.array_equal(host.optimised_coeffs(2), [1.0])
+
+
+# ------------------------------------------------------------------ dt
+@pytest.mark.parametrize("med,ppw", [("iso", 4), ("ani", 4), ("iso", 6), ("ani", 6),
+                                     ("iso", 10), ("ani", 10)])
+def test_safe_dt_reproduces_green_archives(med, ppw):
+    case = R.green_case(med, ppw)
+    dt = host.safe_dt_labels(case["lab"] + 1, case["Ct"], case["rho_t"], case["h"],
+                             R.coeffs_kh2(), safety=0.5)
+    assert abs(dt - case["dt_ref"]) / case["dt_ref"] < 1e-15
+    assert int(7e-6 / dt) == case["expected"].shape[0]
+
+
+@pytest.mark.parametrize("theta", [0, 30, 45, 60])
+def test_safe_dt_reproduces_twoway_archive(theta):
+    case = R.twoway_case(theta, 6, homog=False)
+    m, C, r = case["dt_inputs"]
+    dt = host.safe_dt_labels(m, C, r, case["h"], R.coeffs_kh2(), safety=0.5)
+    assert abs(dt - case["dt_ref"]) / case["dt_ref"] < 1e-15
+    assert int(7e-6 / dt) == case["expected"].shape[0]
+
+
+def test_cfl_limit_closed_form():
+    c = np.array([1.2, -0.1])
+    assert host.cfl_limit(c) == pytest.approx(1.0 / (np.sqrt(3) * 1.3), rel=1e-15)
+
+
+# ----------------------------------------------------------- sponge, wavelets
+def test_sponge_profile_closed_form():
+    shape, w, a = (30, 20, 50), 8, 0.09
+    d = host.sponge_profile(shape, w, a)
+    assert d.dtype == np.float32
+    # interior exactly 1
+    assert np.all(d[8:22, 8:12, 8:42] == 1.0)
+    # edge cell along z, interior in y/x: exp(-a * w) (t = 0)
+    assert d[0, 10, 25] == np.float32(np.exp(-a * 1.0 * 8))
+    # a short axis uses w = n // 2
+    d2 = host.sponge_profile((4, 20, 20), 8, a)
+    assert d2[1, 10, 10] == np.float32(np.exp(-a * (1 - 1 / 2) ** 2 * 2))
+    # mirror symmetry of each axis
+    assert np.array_equal(d, d[::-1, :, :]) and np.array_equal(d, d[:, :, ::-1])
+    # separable product, multiplied in z, y, x order
+    pz, py, px = host.sponge_axis_profiles(shape, w, a)
+    ref = np.ones(shape, np.float32)
+    ref *= pz[:, None, None]
+    ref *= py[None, :, None]
+    ref *= px[None, None, :]
+    assert np.array_equal(d, ref)
+
+
+def test_ricker_closed_form():
+    f0, dt = 2e6, 1e-8
+    w = host.ricker(f0, dt, 400)
+    n0 = int(round(1.2 / f0 / dt))
+    assert w[n0] == pytest.approx(1.0, abs=1e-12)
+    assert abs(w.sum()) * dt < 1e-9                        # zero mean
+    t = np.arange(400) * dt - 1.2 / f0
+    a = (np.pi * f0 * t) ** 2
+    assert np.allclose(w, (1 - 2 * a) * np.exp(-a), rtol=0, atol=1e-15)
+
+
+def test_hann_tone_burst():
+    f0, dt, nc = 1e6, 1e-8, 3
+    w = host.hann_tone_burst(500, dt, f0, nc)
+    T = nc / f0
+    t = np.arange(500) * dt
+    assert np.all(w[t > T + 1e-15] == 0)
+    k = 37
+    assert w[k] == pytest.approx(0.5 * (1 - np.cos(2 * np.pi * t[k] / T)) * np.sin(2 * np.pi * f0 * t[k]))
+    assert w[0] == 0.0
+
+
+def test_face_density_rim():
+    rho = np.full((3, 3, 4), 300.0)
+    rho[:, :, :2] = 917.0
+    fx = host.face_density(rho, 2)
+    assert fx[1, 1, 1] == pytest.approx(608.5)
+    assert fx[1, 1, 3] == 300.0                             # last slice keeps its value
+    assert fx[1, 1, 0] == 917.0
+
+
+# --------------------------------------------------------------- materials
+ICE6 = materials.ti_stiffness_6(**materials.ICE)
+
+
+def tensor_invariants(C6):
+    T = materials.voigt6_to_tensor(C6)
+    return np.einsum("iijj->", T), np.einsum("ijij->", T)
+
+
+def test_material_table_axis_z_is_ti():
+    Ct, rho = materials.material_tables(np.array([[0.0, 0.0, 1.0]]))
+    assert Ct.shape == (2, 36) and rho.dtype == np.float32
+    assert np.array_equal(Ct[1].reshape(6, 6), ICE6.astype(np.float32))
+    K = 300.0 * 1500.0 ** 2
+    fl = Ct[0].reshape(6, 6)
+    assert np.all(fl[:3, :3] == np.float32(K)) and np.all(fl[3:, :] == 0) and np.all(fl[:, 3:] == 0)
+    assert rho[0] == 300 and rho[1] == 917
+
+
+def test_axis_x_is_axis_permutation():
+    # c-axis along x: C'11 = C33, C'22 = C11, C'33 = C11, C'12 = C'13(old) ...
+    C = materials.ti_stiffness_3d([1.0, 0.0, 0.0])
+    assert C[0, 0] == pytest.approx(15.01e9) and C[1, 1] == pytest.approx(13.93e9)
+    assert C[2, 2] == pytest.approx(13.93e9) and C[1, 2] == pytest.approx(7.08e9)
+    assert C[0, 1] == pytest.approx(5.77e9) and C[0, 2] == pytest.approx(5.77e9)
+    assert C[3, 3] == pytest.approx((13.93e9 - 7.08e9) / 2) and C[4, 4] == pytest.approx(3.01e9)
+    assert np.count_nonzero(np.abs(C) > 1) == 12
+
+
+def test_rotation_against_bond_matrix():
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        a = rng.normal(size=3)
+        a /= np.linalg.norm(a)
+        Rm = materials.rotation_z_to(a)
+        assert np.allclose(Rm @ [0, 0, 1], a, atol=1e-14)
+        assert np.allclose(Rm @ Rm.T, np.eye(3), atol=1e-14)
+        C1 = materials.rotate_voigt_3d(ICE6, Rm)
+        C2 = materials.rotate_voigt_bond(ICE6, Rm)
+        assert np.allclose(C1, C2, rtol=0, atol=1e-6 * 15e9)
+        assert np.allclose(C1, C1.T, rtol=0, atol=1e-3)
+        assert np.allclose(tensor_invariants(C1), tensor_invariants(ICE6), rtol=1e-12)
+        assert np.all(np.linalg.eigvalsh(C1) > 0)
+
+
+def test_iso_medium_vp_is_ice_axial_qp():
+    ref = R.load("rsg_green_ssg_iso_ppw4.npz")
+    assert float(ref["vp"]) == pytest.approx(np.sqrt(15.01e9 / 917.0), rel=1e-13)
+
+
+def test_active_mask():
+    Ct, _ = materials.material_tables(np.array([[0.0, 0.0, 1.0]]))
+    m = materials.active_mask(Ct)
+    for I in range(6):
+        for J in range(6):
+            assert bool(m >> (6 * I + J) & 1) == bool(np.any(Ct[:, 6 * I + J] != 0))
+
+
+def test_green_receiver_radii_match_archive():
+    for med, ppw in (("iso", 4), ("ani", 6), ("iso", 10)):
+        case = R.green_case(med, ppw)
+        assert np.array_equal(case["rad"], case["ref"]["rad"])
+        assert len(case["groups"]) == case["expected"].shape[1]
+
+
+def test_lattice_receivers_match_archive():
+    case = R.lattice_case(6)
+    assert np.array_equal(np.array([(q[0] - case["c"], q[1] - case["c"], q[2] - case["c"])
+                                    for q in case["idx"]]), case["ref"]["off"])
