@@ -1,0 +1,212 @@
+"""Benchmark: forward FDTD throughput of the gfx950 kernels on one MI355X.
+
+Workload (default): one single-shot production run -- a 400 x 400 x 200
+FP32 grid (32M cells) holding a 100 mm x 35 mm ice disk of 3000 Voronoi
+grains with a Watson girdle fabric (kappa = -3), a 2.25 MHz Ricker from a
+6 mm piston on the top face, 20-cell sponge absorber, 12000 steps.
+
+A "step" is one FDTD time step over the whole grid (stress half-step,
+source, velocity half-step, receiver record), i.e. nz*ny*nx cell-updates.
+W untimed warm-up steps run first; the fields are then reset and K steps
+are timed from n = 0 with inputs resident in HBM.  The timed run records a
+hipEvent pair around every half-step kernel on the launch stream, which
+gives each kernel's average duration and so its achieved HBM bandwidth
+against algorithmic bytes (stress 68 B/cell, velocity 56 B/cell, step
+124 B/cell).
+
+Prints ONE JSON line.  Only one GPU is used: --gpus above 1 exits.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FDTD Gcell-updates/s (fwd, fwd+adj), %HBM peak; 72-azimuth sweep seconds"
+HBM_PEAK_GBS = 8000.0
+BYTES_STRESS = {"f32": 68, "f64": 132}     # v + s read, s written, label, damp
+BYTES_VELOCITY = {"f32": 56, "f64": 108}   # s + v read, v written, label, damp
+BYTES_STEP = {"f32": 124, "f64": 240}
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return "unknown CPU"
+
+
+def cpu_baseline(cfg, nz_sample=20, warm=2, timed=3):
+    """Upstream array-path loop (NumPy, float64, one thread) on a z-slab of
+    the same specimen: cell-updates per second."""
+    import numpy as np
+
+    from oracle import numpy_forward as NF
+    lab = cfg["lab"]
+    nz = lab.shape[0]
+    z0 = nz // 2 - nz_sample // 2
+    sub = lab[z0:z0 + nz_sample]
+    Ct, rho_t = cfg["mat_tables"]
+    ny, nx = sub.shape[1:]
+    c = (nz_sample // 2, ny // 2, nx // 2)
+    idx = np.array([np.ravel_multi_index(c, sub.shape)])
+    shot = NF.ArrayPathShot(sub + 1, Ct, rho_t, cfg["coeffs"], cfg["h"], cfg["dt"],
+                            (idx, np.array([1.0])),
+                            (np.array([0, 1]), idx, np.array([1.0])))
+    wav = cfg["wavelet"]
+    for n in range(warm):
+        shot.step(float(wav[n]))
+    t0 = time.perf_counter()
+    for n in range(warm, warm + timed):
+        shot.step(float(wav[n]))
+    dt = time.perf_counter() - t0
+    cells = sub.size
+    return dict(value=cells * timed / dt / 1e9, unit="Gcell-updates/s", cores=1, kind="port",
+                sample="%dx%dx%d z-slab of the benchmark volume (%.1fM cells), %d timed steps "
+                       "after %d warm-up, float64 NumPy restatement of the upstream array-path "
+                       "loop, 1 thread, %s; %.2f s/step"
+                       % (nz_sample, ny, nx, cells / 1e6, timed, warm, cpu_model(), dt / timed))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=12000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--kernel", default="tiled", choices=["tiled", "direct"])
+    ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--config", default="c1", choices=["c1", "c0"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=float, default=0.0,
+                    help="also run a plain device copy of this many GiB (PMC calibration)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only (this build has no multi-GPU path); "
+              "got --gpus %d" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from pulse_echo_azimuthal_simulation_amd import solver, specimen
+
+    t_setup = time.perf_counter()
+    nt = max(args.steps, args.warmup, 1)
+    if args.config == "c1":
+        cfg = specimen.config_c1(nt=nt)
+        workload = ("single-shot forward, 400x400x200 FP32 grid (32M cells), 3000-grain Voronoi "
+                    "ice disk 100x35 mm, Watson girdle kappa=-3, 2.25 MHz Ricker, 6 mm top "
+                    "piston (pulse-echo), 20-cell sponge + couplant damping mask, %d steps" % nt)
+    else:
+        cfg = specimen.config_c0(nt=nt)
+        workload = ("plain forward, 96x96x48 disk raster (+23-cell pads), 200 grains, Watson "
+                    "kappa=5, 1 MHz 3-cycle Hann, 6 mm top piston, 20-cell sponge, %d steps" % nt)
+    prep = solver.prepare_shot(cfg["lab"], cfg["axes"], cfg["h"], cfg["dt"], cfg["src_pts"],
+                               cfg["rec_groups"], coeffs=cfg["coeffs"],
+                               sponge_width=cfg["sponge_width"], damp_mask=cfg["damp_mask"],
+                               mat_tables=cfg["mat_tables"])
+    shot = solver.DeviceShot(prep["mats"], prep["Ct"], prep["rho_t"], prep["damp"],
+                             prep["coeffs"], cfg["h"], cfg["dt"], prep["src_idx"], prep["src_w"],
+                             prep["rec_ptr"], prep["rec_idx"], prep["rec_w"],
+                             precision=args.precision, kernel=args.kernel)
+    wav = shot.wavelet(cfg["wavelet"])
+    traces = shot.new_traces(nt)
+    nz, ny, nx = shot.shape
+    ncell = nz * ny * nx
+    log("setup %.1f s: grid %s, %d grains, dt %.4g s, %d footprint cells"
+        % (time.perf_counter() - t_setup, shot.shape, len(cfg["axes"]), cfg["dt"],
+           len(cfg["src_pts"])))
+
+    # warm-up, then reset the fields so the timed run is a whole shot from n = 0
+    if args.warmup > 0:
+        shot.run(wav, 0, args.warmup, traces)
+    torch.cuda.synchronize()
+    for a in shot.v + shot.s:
+        a.zero_()
+    traces.zero_()
+    torch.cuda.synchronize()
+    log("warm-up %d steps done; timing %d steps" % (args.warmup, args.steps))
+
+    t0 = time.perf_counter()
+    ms_stress, ms_vel, ms_sr = shot.run_timed(wav, 0, args.steps, traces)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    tr = traces.cpu().numpy()
+    finite = bool(np.isfinite(tr).all())
+    value = ncell * args.steps / wall / 1e9
+    log("timed %.3f s: %.2f Gcell-updates/s; stress %.3f ms, velocity %.3f ms, source/record "
+        "%.3f ms per step; traces finite=%s, peak |p| %.3e"
+        % (wall, value, ms_stress / args.steps, ms_vel / args.steps, ms_sr / args.steps,
+           finite, float(np.abs(tr[:args.steps]).max())))
+
+    p = args.precision
+    t_s = ms_stress / args.steps / 1e3
+    t_v = ms_vel / args.steps / 1e3
+    bw_s = BYTES_STRESS[p] * ncell / t_s / 1e9
+    bw_v = BYTES_VELOCITY[p] * ncell / t_v / 1e9
+    dominant = "stress" if t_s >= t_v else "velocity"
+    achieved = bw_s if dominant == "stress" else bw_v
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_%s_%s_%s.json" % (args.config, p, args.kernel))
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = json.load(fh).get(dominant)
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": p,
+        "data": "synthetic",
+        "config": {"workload": workload, "grid_zyx": [nz, ny, nx], "cells": ncell,
+                   "grains": int(len(cfg["axes"])), "boundary": "sponge20+fluid-mask",
+                   "kernel": args.kernel, "mode": "forward"},
+        "hbm_frac_step": round(value * BYTES_STEP[p] / HBM_PEAK_GBS, 4),
+        "kernels_ms_per_step": {"stress": round(t_s * 1e3, 5), "velocity": round(t_v * 1e3, 5),
+                                "source_record": round(ms_sr / args.steps, 5)},
+        "kernels_gbs": {"stress": round(bw_s, 1), "velocity": round(bw_v, 1)},
+        "roofline": {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "traces_finite": finite,
+    }
+    if args.calibrate_copy > 0:
+        n = int(args.calibrate_copy * 2 ** 30 / 4)
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = n * 4
+    if not args.no_cpu_baseline:
+        log("CPU baseline (NumPy array path, 1 thread) ...")
+        out["cpu_baseline"] = cpu_baseline(cfg)
+        log("CPU baseline %.4f Gcell-updates/s" % out["cpu_baseline"]["value"])
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -101,6 +101,14 @@ int pe_record(const pe_shot *shot, void *traces, int32_t n, void *stream);
 int pe_run(const pe_shot *shot, int32_t n0, int32_t n1, const double *wavelet,
            void *traces, void *stream);
 
+/* pe_run with a hipEvent pair on `stream` around every half-step kernel:
+ * ms_kernels[0] = summed stress-kernel time, [1] = velocity-kernel time,
+ * [2] = source/record time, in milliseconds.  Blocks until the run ends.
+ * No upstream counterpart (the upstream solver has no kernel timing); it
+ * serves benchmarks that report per-kernel HBM roofline fractions. */
+int pe_run_timed(const pe_shot *shot, int32_t n0, int32_t n1, const double *wavelet,
+                 void *traces, void *stream, double *ms_kernels);
+
 #ifdef __cplusplus
 }
 #endif

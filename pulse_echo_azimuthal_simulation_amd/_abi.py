@@ -18,7 +18,7 @@ KERNELS = {"direct": PE_KERNEL_DIRECT, "tiled": PE_KERNEL_TILED}
 ABI_VERSION = 1
 
 EXPORTS = ("pe_abi_version", "pe_last_error", "pe_check_shot", "pe_stress", "pe_source",
-           "pe_velocity", "pe_record", "pe_run")
+           "pe_velocity", "pe_record", "pe_run", "pe_run_timed")
 
 
 class PeShot(C.Structure):
@@ -69,7 +69,10 @@ def load():
     lib.pe_source.argtypes = [P, C.c_void_p, C.c_int32, C.c_void_p]
     lib.pe_record.argtypes = [P, C.c_void_p, C.c_int32, C.c_void_p]
     lib.pe_run.argtypes = [P, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
-    for name in ("pe_check_shot", "pe_stress", "pe_velocity", "pe_source", "pe_record", "pe_run"):
+    lib.pe_run_timed.argtypes = [P, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.POINTER(C.c_double)]
+    for name in ("pe_check_shot", "pe_stress", "pe_velocity", "pe_source", "pe_record", "pe_run",
+                 "pe_run_timed"):
         getattr(lib, name).restype = C.c_int
     ver = lib.pe_abi_version()
     if ver != ABI_VERSION:

@@ -77,6 +77,58 @@ int run_impl(const pe_shot *p, int32_t n0, int32_t n1, const double *wavelet, vo
     return PE_OK;
 }
 
+// Same sequence as run_impl with a hipEvent pair around every half-step
+// kernel; per-kernel time is summed over the run.  Events are recorded in
+// batches of kBatch steps, each batch read back after it completes.
+template <typename R>
+int run_timed_impl(const pe_shot *p, int32_t n0, int32_t n1, const double *wavelet,
+                   void *traces, hipStream_t st, double *ms)
+{
+    constexpr int kBatch = 128;
+    const pe::Shot<R> s = pe::make_shot<R>(p);
+    const bool direct = p->kernel == PE_KERNEL_DIRECT;
+    const bool rec_before = p->record_mode == PE_RECORD_PRESSURE;
+    hipEvent_t ev[kBatch][4];
+    for (int b = 0; b < kBatch; ++b)
+        for (int k = 0; k < 4; ++k)
+            if (hipEventCreate(&ev[b][k]) != hipSuccess) return fail(PE_EHIP, "hipEventCreate");
+    ms[0] = ms[1] = ms[2] = 0.0;
+    int rc = PE_OK;
+    for (int32_t b0 = n0; b0 < n1 && rc == PE_OK; b0 += kBatch) {
+        const int nb = (n1 - b0) < kBatch ? (n1 - b0) : kBatch;
+        for (int b = 0; b < nb && rc == PE_OK; ++b) {
+            const int32_t n = b0 + b;
+            (void)hipEventRecord(ev[b][0], st);
+            hipError_t e = direct ? pe::launch_stress_direct<R>(s, st) : pe::launch_stress_tiled<R>(s, st);
+            (void)hipEventRecord(ev[b][1], st);
+            if (e == hipSuccess)
+                e = pe::launch_source_record<R>(s, p, wavelet, n, true, (R *)traces,
+                                                rec_before && traces != nullptr, st);
+            (void)hipEventRecord(ev[b][2], st);
+            if (e == hipSuccess)
+                e = direct ? pe::launch_velocity_direct<R>(s, st) : pe::launch_velocity_tiled<R>(s, st);
+            (void)hipEventRecord(ev[b][3], st);
+            if (e == hipSuccess && !rec_before && traces != nullptr)
+                e = pe::launch_source_record<R>(s, p, wavelet, n, false, (R *)traces, true, st);
+            if (e != hipSuccess) rc = hip_fail(e, "timed step launch");
+        }
+        hipError_t e = hipEventSynchronize(ev[nb - 1][3]);
+        if (e != hipSuccess && rc == PE_OK) rc = hip_fail(e, "hipEventSynchronize");
+        for (int b = 0; b < nb && rc == PE_OK; ++b) {
+            float t01 = 0, t12 = 0, t23 = 0;
+            (void)hipEventElapsedTime(&t01, ev[b][0], ev[b][1]);
+            (void)hipEventElapsedTime(&t12, ev[b][1], ev[b][2]);
+            (void)hipEventElapsedTime(&t23, ev[b][2], ev[b][3]);
+            ms[0] += t01;
+            ms[1] += t23;
+            ms[2] += t12;
+        }
+    }
+    for (int b = 0; b < kBatch; ++b)
+        for (int k = 0; k < 4; ++k) (void)hipEventDestroy(ev[b][k]);
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -167,6 +219,19 @@ int pe_run(const pe_shot *p, int32_t n0, int32_t n1, const double *wavelet, void
     return p->precision == PE_F32
                ? run_impl<float>(p, n0, n1, wavelet, traces, (hipStream_t)stream)
                : run_impl<double>(p, n0, n1, wavelet, traces, (hipStream_t)stream);
+}
+
+int pe_run_timed(const pe_shot *p, int32_t n0, int32_t n1, const double *wavelet,
+                 void *traces, void *stream, double *ms_kernels)
+{
+    int rc = pe_check_shot(p);
+    if (rc) return rc;
+    if (n0 < 0 || n1 <= n0) return fail(PE_EINVAL, "bad step range [%d, %d)", n0, n1);
+    if (!wavelet && p->n_src > 0) return fail(PE_EINVAL, "wavelet is NULL");
+    if (!ms_kernels) return fail(PE_EINVAL, "ms_kernels is NULL");
+    return p->precision == PE_F32
+               ? run_timed_impl<float>(p, n0, n1, wavelet, traces, (hipStream_t)stream, ms_kernels)
+               : run_timed_impl<double>(p, n0, n1, wavelet, traces, (hipStream_t)stream, ms_kernels);
 }
 
 }  // extern "C"

@@ -131,6 +131,16 @@ class DeviceShot:
                                    None if traces_dev is None else traces_dev.data_ptr(),
                                    self._stream()), "pe_run")
 
+    def run_timed(self, wavelet_dev, n0, n1, traces_dev=None):
+        """Advance steps n0..n1-1 with per-kernel hipEvent timing; blocks.
+        Returns milliseconds summed over the run: (stress, velocity, source/record)."""
+        ms = (C.c_double * 3)()
+        _abi.check(self.lib.pe_run_timed(C.byref(self.desc), int(n0), int(n1),
+                                         wavelet_dev.data_ptr(),
+                                         None if traces_dev is None else traces_dev.data_ptr(),
+                                         self._stream(), ms), "pe_run_timed")
+        return float(ms[0]), float(ms[1]), float(ms[2])
+
     def new_traces(self, nt):
         return self.torch.zeros((int(nt), self.n_rec), dtype=self.real, device=self.device)
 
