@@ -68,6 +68,8 @@ typedef struct pe_shot {
     const void *rec_w;           /* weights */
     int32_t record_mode;         /* PE_RECORD_* */
     int32_t kernel;              /* PE_KERNEL_* */
+    int32_t tile[2];             /* PE_KERNEL_TILED tile variant: [0] stress, [1] velocity */
+    int32_t zchunk[2];           /* z planes per workgroup, 0 = automatic */
 } pe_shot;
 
 /* ABI version (PE_ABI_VERSION) and the last error message of this thread. */
@@ -100,6 +102,13 @@ int pe_record(const pe_shot *shot, void *traces, int32_t n, void *stream);
  * progress callback, which the caller runs between pe_run calls). */
 int pe_run(const pe_shot *shot, int32_t n0, int32_t n1, const double *wavelet,
            void *traces, void *stream);
+
+/* Tile variants of the tiled kernels, for tuning.  half: 0 stress, 1
+ * velocity.  Returns the number of variants at that precision (>= 1) and,
+ * when info3 is not NULL, writes (tile x, tile y, LDS row padding) of
+ * `variant`; PE_ERANGE when `variant` does not exist.  No upstream
+ * counterpart (the upstream kernels have fixed launch shapes). */
+int pe_tile_info(int32_t precision, int32_t half, int32_t variant, int32_t *info3);
 
 /* pe_run with a hipEvent pair on `stream` around every half-step kernel:
  * ms_kernels[0] = summed stress-kernel time, [1] = velocity-kernel time,

@@ -18,7 +18,7 @@ KERNELS = {"direct": PE_KERNEL_DIRECT, "tiled": PE_KERNEL_TILED}
 ABI_VERSION = 1
 
 EXPORTS = ("pe_abi_version", "pe_last_error", "pe_check_shot", "pe_stress", "pe_source",
-           "pe_velocity", "pe_record", "pe_run", "pe_run_timed")
+           "pe_velocity", "pe_record", "pe_run", "pe_run_timed", "pe_tile_info")
 
 
 class PeShot(C.Structure):
@@ -39,6 +39,8 @@ class PeShot(C.Structure):
         ("rec_w", C.c_void_p),
         ("record_mode", C.c_int32),
         ("kernel", C.c_int32),
+        ("tile", C.c_int32 * 2),
+        ("zchunk", C.c_int32 * 2),
     ]
 
 
@@ -71,6 +73,8 @@ def load():
     lib.pe_run.argtypes = [P, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.pe_run_timed.argtypes = [P, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.POINTER(C.c_double)]
+    lib.pe_tile_info.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
+    lib.pe_tile_info.restype = C.c_int
     for name in ("pe_check_shot", "pe_stress", "pe_velocity", "pe_source", "pe_record", "pe_run",
                  "pe_run_timed"):
         getattr(lib, name).restype = C.c_int
@@ -79,6 +83,19 @@ def load():
         raise ImportError("libpefdtd.so ABI version %d, expected %d: rebuild it" % (ver, ABI_VERSION))
     _lib = lib
     return lib
+
+
+def tile_variants(precision: str, half: int):
+    """[(tx, ty, pad), ...] of the tiled kernels (half 0 = stress, 1 = velocity)."""
+    lib = load()
+    p = PE_F32 if precision == "f32" else PE_F64
+    info = (C.c_int32 * 3)()
+    n = lib.pe_tile_info(p, half, 0, info)
+    out = []
+    for v in range(n):
+        lib.pe_tile_info(p, half, v, info)
+        out.append(tuple(int(x) for x in info))
+    return out
 
 
 def check(rc: int, what: str):

@@ -1,0 +1,67 @@
+"""Launch-shape autotuner for the tiled kernels.
+
+Each half-step kernel (stress, velocity) comes in several tile shapes
+(pe_tile_info) and can split z into chunks of any length.  The best choice
+depends on the grid (partial edge tiles, how many workgroups the chip sees)
+and on the precision, so it is measured: every (tile, z-chunk) candidate is
+timed over a few launches with HIP events on the launch stream, and the
+fastest is kept per (grid shape, precision) for the life of the process.
+
+Timing runs on the shot's own fields while they are still zero and before
+any source is applied: both half-steps map zero fields to zero fields, so
+tuning leaves the shot's state untouched.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+from . import _abi
+
+_CACHE: dict = {}
+ZCHUNKS = (0, 25, 50, 100)
+
+
+def _time_half(shot, half, reps):
+    torch = shot.torch
+    fn = shot.stress if half == 0 else shot.velocity
+    fn()                                     # first launch: code object load
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    b.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def tune(shot, reps=5, zchunks=ZCHUNKS, verbose=False):
+    """Pick tile variants and z-chunk lengths for ``shot`` (a DeviceShot with
+    zero fields); sets them in its descriptor and returns
+    {half: (variant, zchunk, ms)} plus the full timing table."""
+    key = (shot.shape, shot.desc.precision)
+    if key in _CACHE:
+        best = _CACHE[key]
+        for h in (0, 1):
+            shot.desc.tile[h], shot.desc.zchunk[h] = best[h][0], best[h][1]
+        return best, None
+    prec = "f32" if shot.desc.precision == _abi.PE_F32 else "f64"
+    table = {}
+    best = {}
+    for h in (0, 1):
+        variants = _abi.tile_variants(prec, h)
+        for v in range(len(variants)):
+            for zc in zchunks:
+                shot.desc.tile[h] = v
+                shot.desc.zchunk[h] = zc
+                _abi.check(shot.lib.pe_check_shot(C.byref(shot.desc)), "pe_check_shot")
+                ms = _time_half(shot, h, reps)
+                table[(h, v, zc)] = ms
+                if verbose:
+                    print("tune half %d tile %s zchunk %d: %.4f ms" % (h, variants[v], zc, ms),
+                          flush=True)
+                if h not in best or ms < best[h][2]:
+                    best[h] = (v, zc, ms)
+        shot.desc.tile[h], shot.desc.zchunk[h] = best[h][0], best[h][1]
+    _CACHE[key] = best
+    return best, table

@@ -30,7 +30,7 @@ int stress_impl(const pe_shot *p, hipStream_t st)
 {
     const pe::Shot<R> s = pe::make_shot<R>(p);
     hipError_t e = p->kernel == PE_KERNEL_DIRECT ? pe::launch_stress_direct<R>(s, st)
-                                                 : pe::launch_stress_tiled<R>(s, st);
+                                                 : pe::launch_stress_tiled<R>(s, p->tile[0], p->zchunk[0], st);
     return e == hipSuccess ? PE_OK : hip_fail(e, "stress launch");
 }
 
@@ -39,7 +39,7 @@ int velocity_impl(const pe_shot *p, hipStream_t st)
 {
     const pe::Shot<R> s = pe::make_shot<R>(p);
     hipError_t e = p->kernel == PE_KERNEL_DIRECT ? pe::launch_velocity_direct<R>(s, st)
-                                                 : pe::launch_velocity_tiled<R>(s, st);
+                                                 : pe::launch_velocity_tiled<R>(s, p->tile[1], p->zchunk[1], st);
     return e == hipSuccess ? PE_OK : hip_fail(e, "velocity launch");
 }
 
@@ -62,12 +62,12 @@ int run_impl(const pe_shot *p, int32_t n0, int32_t n1, const double *wavelet, vo
     // so source and record share one launch before the velocity update.
     const bool rec_before = p->record_mode == PE_RECORD_PRESSURE;
     for (int32_t n = n0; n < n1; ++n) {
-        hipError_t e = direct ? pe::launch_stress_direct<R>(s, st) : pe::launch_stress_tiled<R>(s, st);
+        hipError_t e = direct ? pe::launch_stress_direct<R>(s, st) : pe::launch_stress_tiled<R>(s, p->tile[0], p->zchunk[0], st);
         if (e != hipSuccess) return hip_fail(e, "stress launch");
         e = pe::launch_source_record<R>(s, p, wavelet, n, true, (R *)traces,
                                         rec_before && traces != nullptr, st);
         if (e != hipSuccess) return hip_fail(e, "source/record launch");
-        e = direct ? pe::launch_velocity_direct<R>(s, st) : pe::launch_velocity_tiled<R>(s, st);
+        e = direct ? pe::launch_velocity_direct<R>(s, st) : pe::launch_velocity_tiled<R>(s, p->tile[1], p->zchunk[1], st);
         if (e != hipSuccess) return hip_fail(e, "velocity launch");
         if (!rec_before && traces != nullptr) {
             e = pe::launch_source_record<R>(s, p, wavelet, n, false, (R *)traces, true, st);
@@ -99,14 +99,14 @@ int run_timed_impl(const pe_shot *p, int32_t n0, int32_t n1, const double *wavel
         for (int b = 0; b < nb && rc == PE_OK; ++b) {
             const int32_t n = b0 + b;
             (void)hipEventRecord(ev[b][0], st);
-            hipError_t e = direct ? pe::launch_stress_direct<R>(s, st) : pe::launch_stress_tiled<R>(s, st);
+            hipError_t e = direct ? pe::launch_stress_direct<R>(s, st) : pe::launch_stress_tiled<R>(s, p->tile[0], p->zchunk[0], st);
             (void)hipEventRecord(ev[b][1], st);
             if (e == hipSuccess)
                 e = pe::launch_source_record<R>(s, p, wavelet, n, true, (R *)traces,
                                                 rec_before && traces != nullptr, st);
             (void)hipEventRecord(ev[b][2], st);
             if (e == hipSuccess)
-                e = direct ? pe::launch_velocity_direct<R>(s, st) : pe::launch_velocity_tiled<R>(s, st);
+                e = direct ? pe::launch_velocity_direct<R>(s, st) : pe::launch_velocity_tiled<R>(s, p->tile[1], p->zchunk[1], st);
             (void)hipEventRecord(ev[b][3], st);
             if (e == hipSuccess && !rec_before && traces != nullptr)
                 e = pe::launch_source_record<R>(s, p, wavelet, n, false, (R *)traces, true, st);
@@ -166,6 +166,14 @@ int pe_check_shot(const pe_shot *p)
         return fail(PE_EINVAL, "source footprint arrays missing");
     if (p->n_rec < 0 || (p->n_rec > 0 && (!p->rec_ptr || !p->rec_idx || !p->rec_w)))
         return fail(PE_EINVAL, "receiver arrays missing");
+    for (int h = 0; h < 2; ++h) {
+        int tx, ty, pad;
+        const int n = pe::tile_variants(p->precision, h, 0, &tx, &ty, &pad);
+        if (p->tile[h] < 0 || p->tile[h] >= n)
+            return fail(PE_EINVAL, "tile variant %d of half-step %d out of range [0, %d)",
+                        p->tile[h], h, n);
+        if (p->zchunk[h] < 0) return fail(PE_EINVAL, "zchunk must be >= 0");
+    }
     if (p->record_mode < PE_RECORD_PRESSURE || p->record_mode > PE_RECORD_VZ)
         return fail(PE_EINVAL, "record mode %d unknown", p->record_mode);
     return PE_OK;
@@ -219,6 +227,22 @@ int pe_run(const pe_shot *p, int32_t n0, int32_t n1, const double *wavelet, void
     return p->precision == PE_F32
                ? run_impl<float>(p, n0, n1, wavelet, traces, (hipStream_t)stream)
                : run_impl<double>(p, n0, n1, wavelet, traces, (hipStream_t)stream);
+}
+
+int pe_tile_info(int32_t precision, int32_t half, int32_t variant, int32_t *info3)
+{
+    if (precision != PE_F32 && precision != PE_F64)
+        return fail(PE_EINVAL, "precision %d is neither PE_F32 nor PE_F64", precision);
+    if (half != 0 && half != 1) return fail(PE_EINVAL, "half must be 0 (stress) or 1 (velocity)");
+    int tx = 0, ty = 0, pad = 0;
+    const int n = pe::tile_variants(precision, half, variant, &tx, &ty, &pad);
+    if (variant < 0 || variant >= n) return fail(PE_ERANGE, "variant %d >= %d", variant, n);
+    if (info3) {
+        info3[0] = tx;
+        info3[1] = ty;
+        info3[2] = pad;
+    }
+    return n;
 }
 
 int pe_run_timed(const pe_shot *p, int32_t n0, int32_t n1, const double *wavelet,

@@ -59,11 +59,30 @@ inline Shot<R> make_shot(const pe_shot *p)
 __device__ __forceinline__ float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma_r(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
+// Workgroup barrier for LDS hand-offs only.  __syncthreads() also fences
+// global memory, which makes every wave wait for its outstanding global
+// loads (the prefetches of the next plane) at each barrier; the kernels
+// exchange data between threads through LDS alone, so an LDS-scoped
+// release/acquire around s_barrier is sufficient.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Launchers (defined in the kernel translation units).
 template <typename R> hipError_t launch_stress_direct(const Shot<R> &s, hipStream_t st);
 template <typename R> hipError_t launch_velocity_direct(const Shot<R> &s, hipStream_t st);
-template <typename R> hipError_t launch_stress_tiled(const Shot<R> &s, hipStream_t st);
-template <typename R> hipError_t launch_velocity_tiled(const Shot<R> &s, hipStream_t st);
+// Tiled launchers: `variant` picks a tile shape (see tile_variants),
+// `zchunk` the z planes per workgroup (0 = automatic).
+template <typename R>
+hipError_t launch_stress_tiled(const Shot<R> &s, int variant, int zchunk, hipStream_t st);
+template <typename R>
+hipError_t launch_velocity_tiled(const Shot<R> &s, int variant, int zchunk, hipStream_t st);
+// Number of tile variants of a half-step (0 stress, 1 velocity) at a
+// precision; fills (tx, ty, pad) of `variant` when it exists.
+int tile_variants(int precision, int half, int variant, int *tx, int *ty, int *pad);
 template <typename R>
 hipError_t launch_source_record(const Shot<R> &s, const pe_shot *p, const double *wavelet,
                                 int32_t n, bool do_source, R *traces, bool do_record,

@@ -42,18 +42,21 @@ __device__ __forceinline__ R ld(const R *p, int i, bool ok)
 }
 
 // ------------------------------------------------------------------ stress
-template <typename R, int TX, int TY>
+template <typename R, int TX, int TY, int PAD>
 struct StressTile {
     static constexpr int GX = TX + 2, GY = TY + 2, NG = GX * GY;   // strain columns
     static constexpr int NT = (NG + 63) / 64 * 64;                 // threads
-    static constexpr int VX = GX + 8, VY = GY + 8, NV = VX * VY;   // v plane tile
-    static constexpr int NFRAME = 8 * (VX + GY);                   // v tile outside G
+    // v plane tile; PAD extra words per row shift consecutive tile rows
+    // onto disjoint LDS banks
+    static constexpr int WX = GX + 8, VX = WX + PAD, VY = GY + 8, NV = VX * VY;
+    static constexpr int NFRAME = 8 * (WX + GY);                   // v tile outside G
     static constexpr size_t LDS = sizeof(R) * (3 * NV + 6 * 3 * NG);
 };
 
 // Moved strain q_{P,J} read from the LDS strain ring (see moved_direct in
 // kernels_direct.hip for the rule).  sl[d+1] is the ring slot of plane z+d.
-template <int P, int J, typename R, int GX, int NG>
+// G = false drops the edge guards (the caller guarantees they all hold).
+template <bool G, int P, int J, typename R, int GX, int NG>
 __device__ __forceinline__ R moved_lds(const R *__restrict__ gt, const int (&sl)[3], int c,
                                        int iz, int iy, int ix, int nz, int ny, int nx)
 {
@@ -66,12 +69,14 @@ __device__ __forceinline__ R moved_lds(const R *__restrict__ gt, const int (&sl)
         constexpr int dy = kPos[P][1] - kPos[src][1];
         constexpr int dx = kPos[P][2] - kPos[src][2];
         bool ok = true;
-        if constexpr (dz > 0) ok = ok && (iz <= nz - 2);
-        if constexpr (dz < 0) ok = ok && (iz >= 1);
-        if constexpr (dy > 0) ok = ok && (iy <= ny - 2);
-        if constexpr (dy < 0) ok = ok && (iy >= 1);
-        if constexpr (dx > 0) ok = ok && (ix <= nx - 2);
-        if constexpr (dx < 0) ok = ok && (ix >= 1);
+        if constexpr (G) {
+            if constexpr (dz > 0) ok = ok && (iz <= nz - 2);
+            if constexpr (dz < 0) ok = ok && (iz >= 1);
+            if constexpr (dy > 0) ok = ok && (iy <= ny - 2);
+            if constexpr (dy < 0) ok = ok && (iy >= 1);
+            if constexpr (dx > 0) ok = ok && (ix <= nx - 2);
+            if constexpr (dx < 0) ok = ok && (ix >= 1);
+        }
         constexpr int z0 = dz < 0 ? -1 : 0, nzt = dz != 0 ? 2 : 1;
         constexpr int y0 = dy < 0 ? -1 : 0, nyt = dy != 0 ? 2 : 1;
         constexpr int x0 = dx < 0 ? -1 : 0, nxt = dx != 0 ? 2 : 1;
@@ -86,20 +91,21 @@ __device__ __forceinline__ R moved_lds(const R *__restrict__ gt, const int (&sl)
                     acc = (a == 0 && b == 0 && e == 0) ? t : acc + t;
                 }
         constexpr int ntap = nzt * nyt * nxt;
-        return ok ? acc * (ntap == 4 ? (R)0.25 : (R)0.5) : (R)0;
+        const R q = acc * (ntap == 4 ? (R)0.25 : (R)0.5);
+        if constexpr (G) return ok ? q : (R)0;
+        return q;
     }
 }
 
-template <int I, typename R>
-__device__ __forceinline__ void stress_out(const Shot<R> &S, const R *__restrict__ Cm,
-                                           const R (&q)[6], R dmp, R &sv)
+template <int I, typename R, typename Row>
+__device__ __forceinline__ void stress_out(const Shot<R> &S, Row cm, const R (&q)[6], R dmp, R &sv)
 {
     R acc = 0;
     bool any = false;
 #pragma unroll
     for (int J = 0; J < 6; ++J) {
         if ((S.act >> (6 * I + J)) & 1ull) {
-            acc = fma_r(Cm[6 * I + J], q[J], acc);
+            acc = fma_r(cm[6 * I + J], q[J], acc);
             any = true;
         }
     }
@@ -107,25 +113,85 @@ __device__ __forceinline__ void stress_out(const Shot<R> &S, const R *__restrict
     if (any) sv = fma_r(S.dt, acc, sv);
 }
 
-template <int P, typename R, int GX, int NG>
+template <bool G, int P, typename R, int GX, int NG>
 __device__ __forceinline__ void gather_lds(const R *__restrict__ gt, const int (&sl)[3], int c,
                                            int iz, int iy, int ix, int nz, int ny, int nx,
                                            R (&q)[6])
 {
-    q[0] = moved_lds<P, 0, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx);
-    q[1] = moved_lds<P, 1, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx);
-    q[2] = moved_lds<P, 2, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx);
-    q[3] = moved_lds<P, 3, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx);
-    q[4] = moved_lds<P, 4, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx);
-    q[5] = moved_lds<P, 5, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx);
+    q[0] = moved_lds<G, P, 0, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx);
+    q[1] = moved_lds<G, P, 1, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx);
+    q[2] = moved_lds<G, P, 2, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx);
+    q[3] = moved_lds<G, P, 3, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx);
+    q[4] = moved_lds<G, P, 4, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx);
+    q[5] = moved_lds<G, P, 5, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx);
 }
 
-template <typename R, int TX, int TY>
-__global__ __launch_bounds__((StressTile<R, TX, TY>::NT)) void k_stress_tiled(Shot<R> S, int tiles_x,
-                                                                           int tiles_y, int zc_len)
+// Strains g1..g6 of plane zc at one column (strains_zp statement order,
+// sim/core/fdtd.py:1136-1164): x / y taps from the LDS plane tile (pointers
+// at the column), z taps from the register rings (r?[k] = plane zc-4+k).
+// G = false drops the per-term edge guards.
+template <bool G, typename R, int VX>
+__device__ __forceinline__ void strains_col(const Shot<R> &S, const R *tx_, const R *ty_,
+                                            const R *tz_, const R (&rx)[9], const R (&ry)[9],
+                                            const R (&rz)[9], int ix, int iy, int zc, R (&a)[6])
 {
-    using T = StressTile<R, TX, TY>;
+    const int nz = S.nz, ny = S.ny, nx = S.nx;
+    R a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0, a6 = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (j >= S.nc) break;
+        const R cj = S.c[j];
+        if (!G || (ix >= j + 1 && ix < nx - j)) a1 = fma_r(cj, tx_[j] - tx_[-(j + 1)], a1);
+        if (!G || (iy >= j + 1 && iy < ny - j)) a2 = fma_r(cj, ty_[j * VX] - ty_[-(j + 1) * VX], a2);
+        if (!G || (zc >= j + 1 && zc < nz - j)) a3 = fma_r(cj, rz[4 + j] - rz[3 - j], a3);
+        if (!G || (iy >= j && iy < ny - j - 1)) a4 = fma_r(cj, tz_[(j + 1) * VX] - tz_[-j * VX], a4);
+        if (!G || (zc >= j && zc < nz - j - 1)) a4 = fma_r(cj, ry[5 + j] - ry[4 - j], a4);
+        if (!G || (ix >= j && ix < nx - j - 1)) a5 = fma_r(cj, tz_[j + 1] - tz_[-j], a5);
+        if (!G || (zc >= j && zc < nz - j - 1)) a5 = fma_r(cj, rx[5 + j] - rx[4 - j], a5);
+        if (!G || (ix >= j && ix < nx - j - 1)) a6 = fma_r(cj, ty_[j + 1] - ty_[-j], a6);
+        if (!G || (iy >= j && iy < ny - j - 1)) a6 = fma_r(cj, tx_[(j + 1) * VX] - tx_[-j * VX], a6);
+    }
+    a[0] = a1 * S.inv_h;
+    a[1] = a2 * S.inv_h;
+    a[2] = a3 * S.inv_h;
+    a[3] = a4 * S.inv_h;
+    a[4] = a5 * S.inv_h;
+    a[5] = a6 * S.inv_h;
+}
+
+// Stiffness rows are read in place.  A wave whose active lanes share one
+// material (the common case: grains span thousands of cells) reads its row
+// through a wave-uniform pointer in the constant address space, which the
+// compiler serves with scalar loads.
+template <typename R>
+using ConstRow = const __attribute__((address_space(4))) R *;
+
+template <bool G, typename R, int GX, int NG, typename Row>
+__device__ __forceinline__ void stress_cell(const Shot<R> &S, const R *__restrict__ gt,
+                                            const int (&sl)[3], int c, int iz, int iy, int ix,
+                                            Row cm, R dmp, R (&sv)[6])
+{
+    const int nz = S.nz, ny = S.ny, nx = S.nx;
+    R q[6];
+    gather_lds<G, 0, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx, q);
+    stress_out<0>(S, cm, q, dmp, sv[0]);
+    stress_out<1>(S, cm, q, dmp, sv[1]);
+    stress_out<2>(S, cm, q, dmp, sv[2]);
+    gather_lds<G, 1, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx, q);
+    stress_out<3>(S, cm, q, dmp, sv[3]);
+    gather_lds<G, 2, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx, q);
+    stress_out<4>(S, cm, q, dmp, sv[4]);
+    gather_lds<G, 3, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx, q);
+    stress_out<5>(S, cm, q, dmp, sv[5]);
+}
+
+template <typename R, int TX, int TY, int PAD>
+__global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT)) void k_stress_tiled(
+    Shot<R> S, int tiles_x, int tiles_y, int zc_len)
+{
+    using T = StressTile<R, TX, TY, PAD>;
     constexpr int GX = T::GX, GY = T::GY, NG = T::NG, VX = T::VX, VY = T::VY, NV = T::NV;
+    constexpr int NT = T::NT, NF = (T::NFRAME + NT - 1) / NT;
     extern __shared__ unsigned char lds_raw[];
     R *vt = reinterpret_cast<R *>(lds_raw);       // [3][VY][VX]
     R *gt = vt + 3 * NV;                           // [6][3][GY][GX]
@@ -137,6 +203,8 @@ __global__ __launch_bounds__((StressTile<R, TX, TY>::NT)) void k_stress_tiled(Sh
     const int tx = t % tiles_x, ty = (t / tiles_x) % tiles_y, chunk = t / (tiles_x * tiles_y);
     const int x0 = tx * TX, y0 = ty * TY;
     const int z0 = chunk * zc_len, z1 = min(nz, z0 + zc_len);
+    // every strain column at least 4 cells (every output 1 cell) from the x/y edges
+    const bool inner_xy = x0 >= 5 && x0 + TX <= nx - 5 && y0 >= 5 && y0 + TY <= ny - 5;
 
     const int tid = threadIdx.x;
     const bool gact = tid < NG;
@@ -151,8 +219,36 @@ __global__ __launch_bounds__((StressTile<R, TX, TY>::NT)) void k_stress_tiled(Sh
     const R *__restrict__ vy = S.v[1];
     const R *__restrict__ vz = S.v[2];
 
-    // z rings: r?[k] = v at plane zc - 4 + k
-    R rx[9], ry[9], rz[9];
+    // this thread's entries of the v-tile frame (the cells outside the
+    // strain columns): LDS slot, x-y offset and whether the column exists
+    int f_lds[NF], f_xy[NF];
+    bool f_ok[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+        const int e = tid + k * NT;
+        int r = 0, c = 0;
+        constexpr int WX = T::WX;
+        if (e < 4 * WX) {
+            r = e / WX;
+            c = e % WX;
+        } else if (e < 8 * WX) {
+            r = VY - 4 + (e - 4 * WX) / WX;
+            c = (e - 4 * WX) % WX;
+        } else if (e < 8 * WX + 4 * GY) {
+            r = 4 + (e - 8 * WX) / 4;
+            c = (e - 8 * WX) % 4;
+        } else {
+            r = 4 + (e - 8 * WX - 4 * GY) / 4;
+            c = GX + 4 + (e - 8 * WX - 4 * GY) % 4;
+        }
+        const int gy_ = y0 - 5 + r, gx_ = x0 - 5 + c;
+        f_lds[k] = e < T::NFRAME ? r * VX + c : -1;
+        f_ok[k] = e < T::NFRAME && gy_ >= 0 && gy_ < ny && gx_ >= 0 && gx_ < nx;
+        f_xy[k] = f_ok[k] ? gy_ * sy + gx_ : 0;
+    }
+
+    // z rings: r?[k] = v at plane zc - 4 + k; frame registers: plane zc
+    R rx[9], ry[9], rz[9], fv[NF][3];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const int z = z0 - 5 + k;
@@ -161,44 +257,49 @@ __global__ __launch_bounds__((StressTile<R, TX, TY>::NT)) void k_stress_tiled(Sh
         ry[k] = ld(vy, col + z * sz, ok);
         rz[k] = ld(vz, col + z * sz, ok);
     }
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+        const bool ok = f_ok[k] && z0 - 1 >= 0;
+        const int gi = (z0 - 1) * sz + f_xy[k];
+        fv[k][0] = ld(vx, gi, ok);
+        fv[k][1] = ld(vy, gi, ok);
+        fv[k][2] = ld(vz, gi, ok);
+    }
 
     for (int zc = z0 - 1; zc <= z1; ++zc) {
         const bool plane_in = zc >= 0 && zc < nz;
-        // (1) plane zc of v into LDS: own column from the ring, frame from HBM/L2
+        // (1) plane zc of v into LDS: own column from the ring, frame from
+        //     the registers loaded one plane ahead
         if (gact) {
             vt[0 * NV + vc] = rx[4];
             vt[1 * NV + vc] = ry[4];
             vt[2 * NV + vc] = rz[4];
         }
-        for (int e = tid; e < T::NFRAME; e += T::NT) {
-            int r, c;
-            if (e < 4 * VX) {
-                r = e / VX;
-                c = e % VX;
-            } else if (e < 8 * VX) {
-                r = VY - 4 + (e - 4 * VX) / VX;
-                c = (e - 4 * VX) % VX;
-            } else if (e < 8 * VX + 4 * GY) {
-                r = 4 + (e - 8 * VX) / 4;
-                c = (e - 8 * VX) % 4;
-            } else {
-                r = 4 + (e - 8 * VX - 4 * GY) / 4;
-                c = VX - 4 + (e - 8 * VX - 4 * GY) % 4;
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+            if (f_lds[k] >= 0) {
+                vt[0 * NV + f_lds[k]] = fv[k][0];
+                vt[1 * NV + f_lds[k]] = fv[k][1];
+                vt[2 * NV + f_lds[k]] = fv[k][2];
             }
-            const int gy_ = y0 - 5 + r, gx_ = x0 - 5 + c;
-            const bool ok = plane_in && gy_ >= 0 && gy_ < ny && gx_ >= 0 && gx_ < nx;
-            const int gi = zc * sz + gy_ * sy + gx_;
-            vt[0 * NV + r * VX + c] = ld(vx, gi, ok);
-            vt[1 * NV + r * VX + c] = ld(vy, gi, ok);
-            vt[2 * NV + r * VX + c] = ld(vz, gi, ok);
         }
-        // prefetch of the ring's next plane, consumed after the stress update
+        // prefetch: frame of plane zc+1, ring plane zc+5, output plane operands
+        {
+            const int zf = zc + 1;
+#pragma unroll
+            for (int k = 0; k < NF; ++k) {
+                const bool ok = f_ok[k] && zf < nz && zf >= 0;
+                const int gi = zf * sz + f_xy[k];
+                fv[k][0] = ld(vx, gi, ok);
+                fv[k][1] = ld(vy, gi, ok);
+                fv[k][2] = ld(vz, gi, ok);
+            }
+        }
         const int zn = zc + 5;
         const bool okn = col_in && zn >= 0 && zn < nz;
         const R nxv = ld(vx, col + zn * sz, okn);
         const R nyv = ld(vy, col + zn * sz, okn);
         const R nzv = ld(vz, col + zn * sz, okn);
-        // loads of the output plane's stress, label and damping
         const int zo = zc - 1;
         const bool out = core && zo >= z0 && zo < z1;
         const int io = zo * sz + col;
@@ -211,65 +312,43 @@ __global__ __launch_bounds__((StressTile<R, TX, TY>::NT)) void k_stress_tiled(Sh
             m = S.mats[io];
             if (S.damp) dmp = S.damp[io];
         }
-        __syncthreads();
+        lds_barrier();
 
         // (2) strains of plane zc at every column of the tile and its ring
         const int slot = (zc + 3) % 3;
         if (gact) {
-            R a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0, a6 = 0;
-            if (col_in && plane_in) {
-                const R *tx_ = vt + 0 * NV + vc;
-                const R *ty_ = vt + 1 * NV + vc;
-                const R *tz_ = vt + 2 * NV + vc;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    if (j >= S.nc) break;
-                    const R cj = S.c[j];
-                    if (ix >= j + 1 && ix < nx - j) a1 = fma_r(cj, tx_[j] - tx_[-(j + 1)], a1);
-                    if (iy >= j + 1 && iy < ny - j)
-                        a2 = fma_r(cj, ty_[j * VX] - ty_[-(j + 1) * VX], a2);
-                    if (zc >= j + 1 && zc < nz - j) a3 = fma_r(cj, rz[4 + j] - rz[3 - j], a3);
-                    if (iy >= j && iy < ny - j - 1)
-                        a4 = fma_r(cj, tz_[(j + 1) * VX] - tz_[-j * VX], a4);
-                    if (zc >= j && zc < nz - j - 1) a4 = fma_r(cj, ry[5 + j] - ry[4 - j], a4);
-                    if (ix >= j && ix < nx - j - 1) a5 = fma_r(cj, tz_[j + 1] - tz_[-j], a5);
-                    if (zc >= j && zc < nz - j - 1) a5 = fma_r(cj, rx[5 + j] - rx[4 - j], a5);
-                    if (ix >= j && ix < nx - j - 1) a6 = fma_r(cj, ty_[j + 1] - ty_[-j], a6);
-                    if (iy >= j && iy < ny - j - 1)
-                        a6 = fma_r(cj, tx_[(j + 1) * VX] - tx_[-j * VX], a6);
-                }
-                a1 = a1 * S.inv_h;
-                a2 = a2 * S.inv_h;
-                a3 = a3 * S.inv_h;
-                a4 = a4 * S.inv_h;
-                a5 = a5 * S.inv_h;
-                a6 = a6 * S.inv_h;
-            }
+            R a[6] = {0, 0, 0, 0, 0, 0};
+            const bool fast = inner_xy && zc >= 4 && zc <= nz - 5;
+            if (fast)
+                strains_col<false, R, VX>(S, vt + vc, vt + NV + vc, vt + 2 * NV + vc, rx, ry, rz,
+                                          ix, iy, zc, a);
+            else if (col_in && plane_in)
+                strains_col<true, R, VX>(S, vt + vc, vt + NV + vc, vt + 2 * NV + vc, rx, ry, rz,
+                                         ix, iy, zc, a);
             R *gs = gt + slot * NG + tid;
-            gs[0 * 3 * NG] = a1;
-            gs[1 * 3 * NG] = a2;
-            gs[2 * 3 * NG] = a3;
-            gs[3 * 3 * NG] = a4;
-            gs[4 * 3 * NG] = a5;
-            gs[5 * 3 * NG] = a6;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) gs[k * 3 * NG] = a[k];
         }
-        __syncthreads();
+        lds_barrier();
 
         // (3) stress of plane zc - 1 from the strain planes zc-2, zc-1, zc
         if (out) {
             const int sl[3] = {(zc + 1) % 3, (zc + 2) % 3, slot};
-            const R *__restrict__ Cm = S.Ct + (size_t)m * 36;
-            R q[6];
-            gather_lds<0, R, GX, NG>(gt, sl, tid, zo, iy, ix, nz, ny, nx, q);
-            stress_out<0>(S, Cm, q, dmp, sv[0]);
-            stress_out<1>(S, Cm, q, dmp, sv[1]);
-            stress_out<2>(S, Cm, q, dmp, sv[2]);
-            gather_lds<1, R, GX, NG>(gt, sl, tid, zo, iy, ix, nz, ny, nx, q);
-            stress_out<3>(S, Cm, q, dmp, sv[3]);
-            gather_lds<2, R, GX, NG>(gt, sl, tid, zo, iy, ix, nz, ny, nx, q);
-            stress_out<4>(S, Cm, q, dmp, sv[4]);
-            gather_lds<3, R, GX, NG>(gt, sl, tid, zo, iy, ix, nz, ny, nx, q);
-            stress_out<5>(S, Cm, q, dmp, sv[5]);
+            const bool fast = inner_xy && zo >= 1 && zo <= nz - 2;
+            const int mu = __builtin_amdgcn_readfirstlane(m);
+            if (__all(m == mu)) {
+                ConstRow<R> cm = (ConstRow<R>)(S.Ct + (size_t)mu * 36);
+                if (fast)
+                    stress_cell<false, R, GX, NG>(S, gt, sl, tid, zo, iy, ix, cm, dmp, sv);
+                else
+                    stress_cell<true, R, GX, NG>(S, gt, sl, tid, zo, iy, ix, cm, dmp, sv);
+            } else {
+                const R *cm = S.Ct + (size_t)m * 36;
+                if (fast)
+                    stress_cell<false, R, GX, NG>(S, gt, sl, tid, zo, iy, ix, cm, dmp, sv);
+                else
+                    stress_cell<true, R, GX, NG>(S, gt, sl, tid, zo, iy, ix, cm, dmp, sv);
+            }
 #pragma unroll
             for (int k = 0; k < 6; ++k) S.s[k][io] = sv[k];
         }
@@ -290,15 +369,49 @@ template <typename R, int TX, int TY>
 struct VelTile {
     static constexpr int NT = TX * TY;
     static constexpr int VX = TX + 8, VY = TY + 8, NV = VX * VY;   // s plane tiles
+    static constexpr int NFRAME = 8 * TY + 8 * TX;                 // x frame, then y frame
     static constexpr size_t LDS = sizeof(R) * 5 * NV;
 };
+
+// Velocity of one cell (vel_zp statement order, sim/core/fdtd.py:1211-1244).
+template <bool G, typename R, int VX>
+__device__ __forceinline__ void accel_col(const Shot<R> &S, const R *p1, const R *p2, const R *p4,
+                                          const R *p5, const R *p6, const R (&r3)[9],
+                                          const R (&r4)[9], const R (&r5)[9], int ix, int iy,
+                                          int z, R &ax, R &ay, R &az)
+{
+    const int nz = S.nz, ny = S.ny, nx = S.nx;
+    ax = 0;
+    ay = 0;
+    az = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (j >= S.nc) break;
+        const R cj = S.c[j];
+        if (!G || (ix >= j && ix < nx - j - 1)) ax = fma_r(cj, p1[j + 1] - p1[-j], ax);
+        if (!G || (ix >= j + 1 && ix < nx - j)) {
+            ay = fma_r(cj, p6[j] - p6[-(j + 1)], ay);
+            az = fma_r(cj, p5[j] - p5[-(j + 1)], az);
+        }
+        if (!G || (iy >= j && iy < ny - j - 1)) ay = fma_r(cj, p2[(j + 1) * VX] - p2[-j * VX], ay);
+        if (!G || (iy >= j + 1 && iy < ny - j)) {
+            ax = fma_r(cj, p6[j * VX] - p6[-(j + 1) * VX], ax);
+            az = fma_r(cj, p4[j * VX] - p4[-(j + 1) * VX], az);
+        }
+        if (!G || (z >= j && z < nz - j - 1)) az = fma_r(cj, r3[5 + j] - r3[4 - j], az);
+        if (!G || (z >= j + 1 && z < nz - j)) {
+            ax = fma_r(cj, r5[4 + j] - r5[3 - j], ax);
+            ay = fma_r(cj, r4[4 + j] - r4[3 - j], ay);
+        }
+    }
+}
 
 template <typename R, int TX, int TY>
 __global__ __launch_bounds__(TX *TY) void k_velocity_tiled(Shot<R> S, int tiles_x, int tiles_y,
                                                           int zc_len)
 {
     using T = VelTile<R, TX, TY>;
-    constexpr int VX = T::VX, NV = T::NV;
+    constexpr int VX = T::VX, NV = T::NV, NT = T::NT, NF = (T::NFRAME + NT - 1) / NT;
     extern __shared__ unsigned char lds_raw[];
     R *st = reinterpret_cast<R *>(lds_raw);   // [5][VY][VX]: s1, s2, s4, s5, s6
     R *t1 = st + 0 * NV;
@@ -314,6 +427,7 @@ __global__ __launch_bounds__(TX *TY) void k_velocity_tiled(Shot<R> S, int tiles_
     const int tx = t % tiles_x, ty = (t / tiles_x) % tiles_y, chunk = t / (tiles_x * tiles_y);
     const int x0 = tx * TX, y0 = ty * TY;
     const int z0 = chunk * zc_len, z1 = min(nz, z0 + zc_len);
+    const bool inner_xy = x0 >= 4 && x0 + TX <= nx - 4 && y0 >= 4 && y0 + TY <= ny - 4;
 
     const int tid = threadIdx.x;
     const int lx = tid % TX, ly = tid / TX;
@@ -329,8 +443,37 @@ __global__ __launch_bounds__(TX *TY) void k_velocity_tiled(Shot<R> S, int tiles_
     const R *__restrict__ s5 = S.s[4];
     const R *__restrict__ s6 = S.s[5];
 
-    // z rings: r?[k] = s? at plane z - 4 + k
-    R r3[9], r4[9], r5[9];
+    // this thread's frame entries: x frame (s1, s5, s6 beside the tile
+    // rows) or y frame (s2, s4, s6 above / below the tile columns)
+    int f_lds[NF], f_xy[NF];
+    bool f_ok[NF], f_x[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+        const int e = tid + k * NT;
+        int li = -1, gx_ = 0, gy_ = 0;
+        const bool xf = e < 8 * TY;
+        if (xf) {
+            const int r = e / 8, q = e % 8;
+            const int c = q < 4 ? q : TX + q;
+            gx_ = x0 - 4 + c;
+            gy_ = y0 + r;
+            li = (r + 4) * VX + c;
+        } else if (e < T::NFRAME) {
+            const int q = (e - 8 * TY) / TX, c = (e - 8 * TY) % TX;
+            const int r = q < 4 ? q : TY + q;
+            gx_ = x0 + c;
+            gy_ = y0 - 4 + r;
+            li = r * VX + c + 4;
+        }
+        f_lds[k] = li;
+        f_x[k] = xf;
+        f_ok[k] = li >= 0 && gx_ >= 0 && gx_ < nx && gy_ >= 0 && gy_ < ny;
+        f_xy[k] = f_ok[k] ? gy_ * sy + gx_ : 0;
+    }
+
+    // z rings r?[k] = s? at plane z - 4 + k; registers for plane z: own
+    // column (s1, s2, s6, v, labels, damp) and frame
+    R r3[9], r4[9], r5[9], fv[NF][3];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const int z = z0 - 4 + k;
@@ -339,99 +482,82 @@ __global__ __launch_bounds__(TX *TY) void k_velocity_tiled(Shot<R> S, int tiles_
         r4[k] = ld(s4, col + z * sz, ok);
         r5[k] = ld(s5, col + z * sz, ok);
     }
-    int mnext = col_in && z0 < nz ? S.mats[z0 * sz + col] : 0;
+    auto load_frame = [&](int z) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+            const bool ok = f_ok[k] && z < nz;
+            const int gi = z * sz + f_xy[k];
+            fv[k][0] = ld(f_x[k] ? s1 : s2, gi, ok);
+            fv[k][1] = ld(f_x[k] ? s5 : s4, gi, ok);
+            fv[k][2] = ld(s6, gi, ok);
+        }
+    };
+    R c1, c2, c6, cvx, cvy, cvz, cdmp;
+    int cm0, cmx, cmy, cmz;
+    auto load_col = [&](int z) {
+        const bool ok = col_in && z < nz;
+        const int i = z * sz + col;
+        c1 = ld(s1, i, ok);
+        c2 = ld(s2, i, ok);
+        c6 = ld(s6, i, ok);
+        cvx = ld(S.v[0], i, ok);
+        cvy = ld(S.v[1], i, ok);
+        cvz = ld(S.v[2], i, ok);
+        cdmp = ok && S.damp ? S.damp[i] : (R)1;
+        cm0 = ok ? S.mats[i] : 0;
+        cmx = ok && ix < nx - 1 ? S.mats[i + 1] : 0;
+        cmy = ok && iy < ny - 1 ? S.mats[i + sy] : 0;
+        cmz = ok && z < nz - 1 ? S.mats[i + sz] : 0;
+    };
+    load_frame(z0);
+    load_col(z0);
 
     for (int z = z0; z < z1; ++z) {
         const int i = z * sz + col;
         // (1) plane z of s1, s2, s4, s5, s6 into LDS
-        if (col_in) {
-            t1[vc] = s1[i];
-            t2[vc] = s2[i];
-            t6[vc] = s6[i];
-        } else {
-            t1[vc] = (R)0;
-            t2[vc] = (R)0;
-            t6[vc] = (R)0;
-        }
+        t1[vc] = c1;
+        t2[vc] = c2;
+        t6[vc] = c6;
         t4[vc] = r4[4];
         t5[vc] = r5[4];
-        // x frame (4 columns each side of the tile rows): s1, s5, s6
-        for (int e = tid; e < 8 * TY; e += T::NT) {
-            const int r = e / 8, k = e % 8;
-            const int c = k < 4 ? k : TX + k;             // 0..3 and TX+4..TX+7
-            const int gx_ = x0 - 4 + c, gy_ = y0 + r;
-            const bool ok = gx_ >= 0 && gx_ < nx && gy_ < ny;
-            const int gi = z * sz + gy_ * sy + gx_;
-            const int li = (r + 4) * VX + c;
-            t1[li] = ld(s1, gi, ok);
-            t5[li] = ld(s5, gi, ok);
-            t6[li] = ld(s6, gi, ok);
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+            if (f_lds[k] >= 0) {
+                (f_x[k] ? t1 : t2)[f_lds[k]] = fv[k][0];
+                (f_x[k] ? t5 : t4)[f_lds[k]] = fv[k][1];
+                t6[f_lds[k]] = fv[k][2];
+            }
         }
-        // y frame (4 rows above and below the tile columns): s2, s4, s6
-        for (int e = tid; e < 8 * TX; e += T::NT) {
-            const int k = e / TX, c = e % TX;
-            const int r = k < 4 ? k : TY + k;             // 0..3 and TY+4..TY+7
-            const int gx_ = x0 + c, gy_ = y0 - 4 + r;
-            const bool ok = gx_ < nx && gy_ >= 0 && gy_ < ny;
-            const int gi = z * sz + gy_ * sy + gx_;
-            const int li = r * VX + c + 4;
-            t2[li] = ld(s2, gi, ok);
-            t4[li] = ld(s4, gi, ok);
-            t6[li] = ld(s6, gi, ok);
-        }
-        // per-cell operands of plane z
-        const int m0 = mnext;
-        const int zn = z + 1;
-        mnext = col_in && zn < nz ? S.mats[zn * sz + col] : 0;
-        const int mxp = col_in && ix < nx - 1 ? S.mats[i + 1] : 0;
-        const int myp = col_in && iy < ny - 1 ? S.mats[i + sy] : 0;
-        const R dmp = col_in && S.damp ? S.damp[i] : (R)1;
-        R vxo = (R)0, vyo = (R)0, vzo = (R)0;
-        if (col_in) {
-            vxo = S.v[0][i];
-            vyo = S.v[1][i];
-            vzo = S.v[2][i];
-        }
+        const R vxo = cvx, vyo = cvy, vzo = cvz, dmp = cdmp;
+        const int m0 = cm0, mxp = cmx, myp = cmy, mzp = cmz;
+        // prefetch plane z+1 and the ring's plane z+5
+        load_frame(z + 1);
+        load_col(z + 1);
         const int z5 = z + 5;
         const bool ok5 = col_in && z5 < nz;
         const R n3 = ld(s3, col + z5 * sz, ok5);
         const R n4 = ld(s4, col + z5 * sz, ok5);
         const R n5 = ld(s5, col + z5 * sz, ok5);
-        __syncthreads();
+        lds_barrier();
 
         if (col_in) {
-            R ax = 0, ay = 0, az = 0;
-            const R *p1 = t1 + vc, *p2 = t2 + vc, *p4 = t4 + vc, *p5 = t5 + vc, *p6 = t6 + vc;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                if (j >= S.nc) break;
-                const R cj = S.c[j];
-                if (ix >= j && ix < nx - j - 1) ax = fma_r(cj, p1[j + 1] - p1[-j], ax);
-                if (ix >= j + 1 && ix < nx - j) {
-                    ay = fma_r(cj, p6[j] - p6[-(j + 1)], ay);
-                    az = fma_r(cj, p5[j] - p5[-(j + 1)], az);
-                }
-                if (iy >= j && iy < ny - j - 1) ay = fma_r(cj, p2[(j + 1) * VX] - p2[-j * VX], ay);
-                if (iy >= j + 1 && iy < ny - j) {
-                    ax = fma_r(cj, p6[j * VX] - p6[-(j + 1) * VX], ax);
-                    az = fma_r(cj, p4[j * VX] - p4[-(j + 1) * VX], az);
-                }
-                if (z >= j && z < nz - j - 1) az = fma_r(cj, r3[5 + j] - r3[4 - j], az);
-                if (z >= j + 1 && z < nz - j) {
-                    ax = fma_r(cj, r5[4 + j] - r5[3 - j], ax);
-                    ay = fma_r(cj, r4[4 + j] - r4[3 - j], ay);
-                }
-            }
+            R ax, ay, az;
+            if (inner_xy && z >= 4 && z <= nz - 5)
+                accel_col<false, R, VX>(S, t1 + vc, t2 + vc, t4 + vc, t5 + vc, t6 + vc, r3, r4, r5,
+                                        ix, iy, z, ax, ay, az);
+            else
+                accel_col<true, R, VX>(S, t1 + vc, t2 + vc, t4 + vc, t5 + vc, t6 + vc, r3, r4, r5,
+                                       ix, iy, z, ax, ay, az);
             const R *__restrict__ rho = S.rho;
             const R r0 = rho[m0];
             const R rvx = ix < nx - 1 ? (R)0.5 * (r0 + rho[mxp]) : r0;
             const R rvy = iy < ny - 1 ? (R)0.5 * (r0 + rho[myp]) : r0;
-            const R rvz = z < nz - 1 ? (R)0.5 * (r0 + rho[mnext]) : r0;
+            const R rvz = z < nz - 1 ? (R)0.5 * (r0 + rho[mzp]) : r0;
             S.v[0][i] = (vxo + S.dt_h * ax / rvx) * dmp;
             S.v[1][i] = (vyo + S.dt_h * ay / rvy) * dmp;
             S.v[2][i] = (vzo + S.dt_h * az / rvz) * dmp;
         }
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             r3[k] = r3[k + 1];
@@ -445,18 +571,12 @@ __global__ __launch_bounds__(TX *TY) void k_velocity_tiled(Shot<R> S, int tiles_
 }
 
 // ------------------------------------------------------------- launchers
-template <typename R> struct Tiles;
-template <> struct Tiles<float> {
-    static constexpr int STX = 32, STY = 16, VTX = 64, VTY = 4;
-};
-template <> struct Tiles<double> {
-    static constexpr int STX = 32, STY = 6, VTX = 64, VTY = 4;
-};
-
-// z planes per workgroup: enough workgroups to cover the chip about four
-// times over, but no chunk shorter than 16 planes (ring refill cost).
-inline int chunk_len(int nz, int tiles)
+// z planes per workgroup when the caller does not fix it: enough workgroups
+// to cover the chip about four times over, but no chunk shorter than 16
+// planes (each chunk refills its 9-plane rings).
+inline int chunk_len(int nz, int tiles, int zchunk)
 {
+    if (zchunk > 0) return zchunk > nz ? nz : zchunk;
     const int target = 256 * 4;
     int chunks = (target + tiles - 1) / tiles;
     chunks = chunks < 1 ? 1 : chunks;
@@ -465,37 +585,119 @@ inline int chunk_len(int nz, int tiles)
     return len > nz ? nz : len;
 }
 
-}  // namespace
-
-template <typename R>
-hipError_t launch_stress_tiled(const Shot<R> &s, hipStream_t st)
+template <typename K>
+hipError_t allow_lds(K kernel, size_t lds)
 {
-    constexpr int TX = Tiles<R>::STX, TY = Tiles<R>::STY;
-    using T = StressTile<R, TX, TY>;
+    if (lds <= 65536) return hipSuccess;
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+template <typename R, int TX, int TY, int PAD>
+hipError_t stress_launch(const Shot<R> &s, int zchunk, hipStream_t st)
+{
+    using T = StressTile<R, TX, TY, PAD>;
     const int tiles_x = (s.nx + TX - 1) / TX, tiles_y = (s.ny + TY - 1) / TY;
-    const int zl = chunk_len(s.nz, tiles_x * tiles_y);
+    const int zl = chunk_len(s.nz, tiles_x * tiles_y, zchunk);
     const int nb = tiles_x * tiles_y * ((s.nz + zl - 1) / zl);
-    hipLaunchKernelGGL((k_stress_tiled<R, TX, TY>), dim3(nb), dim3(T::NT), T::LDS, st, s, tiles_x,
-                       tiles_y, zl);
+    hipError_t e = allow_lds(k_stress_tiled<R, TX, TY, PAD>, T::LDS);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_stress_tiled<R, TX, TY, PAD>), dim3(nb), dim3(T::NT), T::LDS, st, s,
+                       tiles_x, tiles_y, zl);
     return hipGetLastError();
 }
 
-template <typename R>
-hipError_t launch_velocity_tiled(const Shot<R> &s, hipStream_t st)
+template <typename R, int TX, int TY>
+hipError_t velocity_launch(const Shot<R> &s, int zchunk, hipStream_t st)
 {
-    constexpr int TX = Tiles<R>::VTX, TY = Tiles<R>::VTY;
     using T = VelTile<R, TX, TY>;
     const int tiles_x = (s.nx + TX - 1) / TX, tiles_y = (s.ny + TY - 1) / TY;
-    const int zl = chunk_len(s.nz, tiles_x * tiles_y);
+    const int zl = chunk_len(s.nz, tiles_x * tiles_y, zchunk);
     const int nb = tiles_x * tiles_y * ((s.nz + zl - 1) / zl);
+    hipError_t e = allow_lds(k_velocity_tiled<R, TX, TY>, T::LDS);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_velocity_tiled<R, TX, TY>), dim3(nb), dim3(T::NT), T::LDS, st, s,
                        tiles_x, tiles_y, zl);
     return hipGetLastError();
 }
 
-template hipError_t launch_stress_tiled<float>(const Shot<float> &, hipStream_t);
-template hipError_t launch_stress_tiled<double>(const Shot<double> &, hipStream_t);
-template hipError_t launch_velocity_tiled<float>(const Shot<float> &, hipStream_t);
-template hipError_t launch_velocity_tiled<double>(const Shot<double> &, hipStream_t);
+// Tile variants (TX, TY, LDS row padding); variant 0 is the default.
+struct TileInfo {
+    int tx, ty, pad;
+};
+constexpr TileInfo kStressF32[] = {{32, 16, 0}, {32, 16, 24}, {32, 8, 0}, {32, 8, 24},
+                                   {64, 4, 0},  {64, 4, 24},  {16, 16, 0}, {64, 8, 0}};
+constexpr TileInfo kStressF64[] = {{32, 6, 0}, {32, 4, 0}, {16, 8, 0}, {32, 6, 24}};
+constexpr TileInfo kVelF32[] = {{32, 16, 0}, {64, 8, 0}, {32, 8, 0}, {64, 4, 0}, {128, 4, 0},
+                                {32, 32, 0}};
+constexpr TileInfo kVelF64[] = {{32, 8, 0}, {64, 4, 0}, {32, 16, 0}, {32, 4, 0}};
+
+}  // namespace
+
+int tile_variants(int precision, int half, int variant, int *tx, int *ty, int *pad)
+{
+    const TileInfo *t = nullptr;
+    int n = 0;
+    if (half == 0 && precision == PE_F32) { t = kStressF32; n = sizeof kStressF32 / sizeof *t; }
+    if (half == 0 && precision == PE_F64) { t = kStressF64; n = sizeof kStressF64 / sizeof *t; }
+    if (half == 1 && precision == PE_F32) { t = kVelF32; n = sizeof kVelF32 / sizeof *t; }
+    if (half == 1 && precision == PE_F64) { t = kVelF64; n = sizeof kVelF64 / sizeof *t; }
+    if (variant >= 0 && variant < n) {
+        *tx = t[variant].tx;
+        *ty = t[variant].ty;
+        *pad = t[variant].pad;
+    }
+    return n;
+}
+
+template <>
+hipError_t launch_stress_tiled<float>(const Shot<float> &s, int v, int zc, hipStream_t st)
+{
+    switch (v) {
+    case 1: return stress_launch<float, 32, 16, 24>(s, zc, st);
+    case 2: return stress_launch<float, 32, 8, 0>(s, zc, st);
+    case 3: return stress_launch<float, 32, 8, 24>(s, zc, st);
+    case 4: return stress_launch<float, 64, 4, 0>(s, zc, st);
+    case 5: return stress_launch<float, 64, 4, 24>(s, zc, st);
+    case 6: return stress_launch<float, 16, 16, 0>(s, zc, st);
+    case 7: return stress_launch<float, 64, 8, 0>(s, zc, st);
+    default: return stress_launch<float, 32, 16, 0>(s, zc, st);
+    }
+}
+
+template <>
+hipError_t launch_stress_tiled<double>(const Shot<double> &s, int v, int zc, hipStream_t st)
+{
+    switch (v) {
+    case 1: return stress_launch<double, 32, 4, 0>(s, zc, st);
+    case 2: return stress_launch<double, 16, 8, 0>(s, zc, st);
+    case 3: return stress_launch<double, 32, 6, 24>(s, zc, st);
+    default: return stress_launch<double, 32, 6, 0>(s, zc, st);
+    }
+}
+
+template <>
+hipError_t launch_velocity_tiled<float>(const Shot<float> &s, int v, int zc, hipStream_t st)
+{
+    switch (v) {
+    case 1: return velocity_launch<float, 64, 8>(s, zc, st);
+    case 2: return velocity_launch<float, 32, 8>(s, zc, st);
+    case 3: return velocity_launch<float, 64, 4>(s, zc, st);
+    case 4: return velocity_launch<float, 128, 4>(s, zc, st);
+    case 5: return velocity_launch<float, 32, 32>(s, zc, st);
+    default: return velocity_launch<float, 32, 16>(s, zc, st);
+    }
+}
+
+template <>
+hipError_t launch_velocity_tiled<double>(const Shot<double> &s, int v, int zc, hipStream_t st)
+{
+    switch (v) {
+    case 1: return velocity_launch<double, 64, 4>(s, zc, st);
+    case 2: return velocity_launch<double, 32, 16>(s, zc, st);
+    case 3: return velocity_launch<double, 32, 4>(s, zc, st);
+    default: return velocity_launch<double, 32, 8>(s, zc, st);
+    }
+}
 
 }  // namespace pe

@@ -37,7 +37,7 @@ class DeviceShot:
 
     def __init__(self, mats, Ct, rho_t, damp, coeffs, h, dt, src_idx, src_w,
                  rec_ptr, rec_idx, rec_w, record="pressure", precision="f32",
-                 kernel="tiled", act_mask=None, device=None):
+                 kernel="tiled", act_mask=None, device=None, tile=(0, 0), zchunk=(0, 0)):
         torch = _torch()
         self.torch = torch
         dev = torch.device("cuda", torch.cuda.current_device() if device is None else device)
@@ -102,6 +102,8 @@ class DeviceShot:
         d.rec_w = self.rec_w.data_ptr() if self.rec_w.numel() else None
         d.record_mode = _abi.RECORD_MODES[record]
         d.kernel = _abi.KERNELS[kernel]
+        d.tile[0], d.tile[1] = int(tile[0]), int(tile[1])
+        d.zchunk[0], d.zchunk[1] = int(zchunk[0]), int(zchunk[1])
         self.desc = d
         self.lib = _abi.load()
         _abi.check(self.lib.pe_check_shot(C.byref(d)), "pe_check_shot")
@@ -232,7 +234,7 @@ def forward_fused_labels(lab, axes, h, dt, nt, src_pts, wavelet, rec_groups,
                          c_couplant=1500.0, rho_couplant=300.0,
                          record="pressure", slab=32, stress_smem=False,
                          progress=None, mat_tables=None, *, precision="f32",
-                         kernel="tiled", compat_uint8=False, progress_every=200):
+                         kernel="tiled", compat_uint8=False, progress_every=200, tune=False):
     """Label-indexed anisotropic elastic FDTD, drop-in for the upstream
     ``fdtd.forward_fused_labels`` (sim/core/fdtd.py:1618-1816).
 
@@ -240,7 +242,8 @@ def forward_fused_labels(lab, axes, h, dt, nt, src_pts, wavelet, rec_groups,
     ``precision="f64"``).  ``slab`` and ``stress_smem`` are accepted for
     signature compatibility; the gfx950 kernels march z without slabs.
     ``progress(frac)`` is called on the host every ``progress_every`` steps
-    (at n = 0, 200, ...) and may raise to abort, as upstream.
+    (at n = 0, 200, ...) and may raise to abort, as upstream.  ``tune=True``
+    times the tiled kernels' launch shapes on this grid first (see autotune).
     """
     if record not in _abi.RECORD_MODES:
         raise ValueError(f"record must be one of {sorted(_abi.RECORD_MODES)}")
@@ -255,6 +258,9 @@ def forward_fused_labels(lab, axes, h, dt, nt, src_pts, wavelet, rec_groups,
                       h, dt, prep["src_idx"], prep["src_w"], prep["rec_ptr"],
                       prep["rec_idx"], prep["rec_w"], record=record, precision=precision,
                       kernel=kernel)
+    if tune and kernel == "tiled":
+        from . import autotune
+        autotune.tune(shot)
     wav = shot.wavelet(wavelet[:nt])
     traces = shot.new_traces(nt)
     # upstream calls progress(n / nt) after step n whenever n % 200 == 0
