@@ -91,6 +91,7 @@ def main():
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
     ap.add_argument("--config", default="c1", choices=["c1", "c0"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="also run a plain device copy of this many GiB (PMC calibration)")
     args = ap.parse_args()
@@ -124,6 +125,14 @@ def main():
                              prep["coeffs"], cfg["h"], cfg["dt"], prep["src_idx"], prep["src_w"],
                              prep["rec_ptr"], prep["rec_idx"], prep["rec_w"],
                              precision=args.precision, kernel=args.kernel)
+    tuned = None
+    if args.kernel == "tiled" and not args.no_tune:
+        from pulse_echo_azimuthal_simulation_amd import _abi, autotune
+        best, _ = autotune.tune(shot)
+        tuned = {h: {"tile": _abi.tile_variants(args.precision, h)[best[h][0]],
+                     "zchunk": best[h][1], "ms": round(best[h][2], 4)}
+                 for h in (0, 1)}
+        log("autotune: stress %s, velocity %s" % (tuned[0], tuned[1]))
     wav = shot.wavelet(cfg["wavelet"])
     traces = shot.new_traces(nt)
     nz, ny, nx = shot.shape
@@ -181,7 +190,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload, "grid_zyx": [nz, ny, nx], "cells": ncell,
                    "grains": int(len(cfg["axes"])), "boundary": "sponge20+fluid-mask",
-                   "kernel": args.kernel, "mode": "forward"},
+                   "kernel": args.kernel, "mode": "forward",
+                   "launch": None if tuned is None else
+                   {"stress": tuned[0], "velocity": tuned[1]}},
         "hbm_frac_step": round(value * BYTES_STEP[p] / HBM_PEAK_GBS, 4),
         "kernels_ms_per_step": {"stress": round(t_s * 1e3, 5), "velocity": round(t_v * 1e3, 5),
                                 "source_record": round(ms_sr / args.steps, 5)},

@@ -50,7 +50,7 @@ struct StressTile {
     // onto disjoint LDS banks
     static constexpr int WX = GX + 8, VX = WX + PAD, VY = GY + 8, NV = VX * VY;
     static constexpr int NFRAME = 8 * (WX + GY);                   // v tile outside G
-    static constexpr size_t LDS = sizeof(R) * (3 * NV + 6 * 3 * NG);
+    static constexpr size_t LDS = sizeof(R) * (3 * NV + 6 * 3 * NT);
 };
 
 // Moved strain q_{P,J} read from the LDS strain ring (see moved_direct in
@@ -97,15 +97,49 @@ __device__ __forceinline__ R moved_lds(const R *__restrict__ gt, const int (&sl)
     }
 }
 
-template <int I, typename R, typename Row>
+// One stress component: acc = sum over active J (ascending) of C_IJ q_J, then
+// s = fma(dt, acc, s * damp) (stress_lab_u, sim/core/fdtd.py:643-657).
+// FULL = true: every C_IJ of the table is active (no per-term test).
+// Six consecutive row entries starting at an 8-byte (float) / 16-byte
+// (double) aligned offset: three paired loads per lane.
+__device__ __forceinline__ void load6(const float *p, float (&c)[6])
+{
+    const float2 *q = reinterpret_cast<const float2 *>(p);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float2 t = q[k];
+        c[2 * k] = t.x;
+        c[2 * k + 1] = t.y;
+    }
+}
+__device__ __forceinline__ void load6(const double *p, double (&c)[6])
+{
+    const double2 *q = reinterpret_cast<const double2 *>(p);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double2 t = q[k];
+        c[2 * k] = t.x;
+        c[2 * k + 1] = t.y;
+    }
+}
+template <typename R>
+__device__ __forceinline__ void load6(const __attribute__((address_space(4))) R *p, R (&c)[6])
+{
+#pragma unroll
+    for (int k = 0; k < 6; ++k) c[k] = p[k];
+}
+
+template <int I, bool FULL, typename R, typename Row>
 __device__ __forceinline__ void stress_out(const Shot<R> &S, Row cm, const R (&q)[6], R dmp, R &sv)
 {
+    R row[6];
+    load6(cm + 6 * I, row);
     R acc = 0;
-    bool any = false;
+    bool any = FULL;
 #pragma unroll
     for (int J = 0; J < 6; ++J) {
-        if ((S.act >> (6 * I + J)) & 1ull) {
-            acc = fma_r(cm[6 * I + J], q[J], acc);
+        if (FULL || ((S.act >> (6 * I + J)) & 1ull)) {
+            acc = fma_r(row[J], q[J], acc);
             any = true;
         }
     }
@@ -166,7 +200,7 @@ __device__ __forceinline__ void strains_col(const Shot<R> &S, const R *tx_, cons
 template <typename R>
 using ConstRow = const __attribute__((address_space(4))) R *;
 
-template <bool G, typename R, int GX, int NG, typename Row>
+template <bool G, bool FULL, typename R, int GX, int NG, typename Row>
 __device__ __forceinline__ void stress_cell(const Shot<R> &S, const R *__restrict__ gt,
                                             const int (&sl)[3], int c, int iz, int iy, int ix,
                                             Row cm, R dmp, R (&sv)[6])
@@ -174,27 +208,39 @@ __device__ __forceinline__ void stress_cell(const Shot<R> &S, const R *__restric
     const int nz = S.nz, ny = S.ny, nx = S.nx;
     R q[6];
     gather_lds<G, 0, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx, q);
-    stress_out<0>(S, cm, q, dmp, sv[0]);
-    stress_out<1>(S, cm, q, dmp, sv[1]);
-    stress_out<2>(S, cm, q, dmp, sv[2]);
+    stress_out<0, FULL>(S, cm, q, dmp, sv[0]);
+    stress_out<1, FULL>(S, cm, q, dmp, sv[1]);
+    stress_out<2, FULL>(S, cm, q, dmp, sv[2]);
     gather_lds<G, 1, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx, q);
-    stress_out<3>(S, cm, q, dmp, sv[3]);
+    stress_out<3, FULL>(S, cm, q, dmp, sv[3]);
     gather_lds<G, 2, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx, q);
-    stress_out<4>(S, cm, q, dmp, sv[4]);
+    stress_out<4, FULL>(S, cm, q, dmp, sv[4]);
     gather_lds<G, 3, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx, q);
-    stress_out<5>(S, cm, q, dmp, sv[5]);
+    stress_out<5, FULL>(S, cm, q, dmp, sv[5]);
 }
 
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Global loads are branch-free: every address is clamped into the grid and
+// the values a clamped address returns are never used (the edge guards of
+// the generic path exclude them; the fast path only runs where none occur).
+// Waves per SIMD the register allocator must leave room for so that two
+// workgroups of NT threads fit on a CU (float); double keeps the compiler's
+// choice (its larger register footprint would spill).
+template <typename R, int NT> struct MinWaves { static constexpr int v = 1; };
+template <int NT> struct MinWaves<float, NT> { static constexpr int v = (2 * (NT / 64) + 3) / 4; };
+
 template <typename R, int TX, int TY, int PAD>
-__global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT)) void k_stress_tiled(
+__global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
+                             (MinWaves<R, StressTile<R, TX, TY, PAD>::NT>::v)) void k_stress_tiled(
     Shot<R> S, int tiles_x, int tiles_y, int zc_len)
 {
     using T = StressTile<R, TX, TY, PAD>;
     constexpr int GX = T::GX, GY = T::GY, NG = T::NG, VX = T::VX, VY = T::VY, NV = T::NV;
-    constexpr int NT = T::NT, NF = (T::NFRAME + NT - 1) / NT;
+    constexpr int NT = T::NT, NF = (T::NFRAME + NT - 1) / NT, WX = T::WX;
     extern __shared__ unsigned char lds_raw[];
     R *vt = reinterpret_cast<R *>(lds_raw);       // [3][VY][VX]
-    R *gt = vt + 3 * NV;                           // [6][3][GY][GX]
+    R *gt = vt + 3 * NV;                           // [6][3][NT]: strains by thread
 
     const int nz = S.nz, ny = S.ny, nx = S.nx;
     const int sy = nx, sz = nx * ny;
@@ -205,6 +251,8 @@ __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT)) void k_stress_til
     const int z0 = chunk * zc_len, z1 = min(nz, z0 + zc_len);
     // every strain column at least 4 cells (every output 1 cell) from the x/y edges
     const bool inner_xy = x0 >= 5 && x0 + TX <= nx - 5 && y0 >= 5 && y0 + TY <= ny - 5;
+    const bool full = S.act == 0xFFFFFFFFFull;
+    const bool has_damp = S.damp != nullptr;
 
     const int tid = threadIdx.x;
     const bool gact = tid < NG;
@@ -212,22 +260,23 @@ __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT)) void k_stress_til
     const int ix = x0 - 1 + gx, iy = y0 - 1 + gy;                  // column of this thread
     const bool col_in = gact && ix >= 0 && ix < nx && iy >= 0 && iy < ny;
     const bool core = col_in && gx >= 1 && gx <= TX && gy >= 1 && gy <= TY;
-    const int col = col_in ? iy * sy + ix : 0;
-    const int vc = (gy + 4) * VX + (gx + 4);                        // this column in vt
+    const int col = clampi(iy, 0, ny - 1) * sy + clampi(ix, 0, nx - 1);
+    // LDS slot of this column in each v plane tile; idle threads write the
+    // far corner cell, which no stencil tap reads
+    const int vc = gact ? (gy + 4) * VX + (gx + 4) : NV - 1;
 
     const R *__restrict__ vx = S.v[0];
     const R *__restrict__ vy = S.v[1];
     const R *__restrict__ vz = S.v[2];
 
     // this thread's entries of the v-tile frame (the cells outside the
-    // strain columns): LDS slot, x-y offset and whether the column exists
+    // strain columns): LDS slot (the unread far corner if none) and clamped
+    // x-y offset
     int f_lds[NF], f_xy[NF];
-    bool f_ok[NF];
 #pragma unroll
     for (int k = 0; k < NF; ++k) {
         const int e = tid + k * NT;
         int r = 0, c = 0;
-        constexpr int WX = T::WX;
         if (e < 4 * WX) {
             r = e / WX;
             c = e % WX;
@@ -241,126 +290,119 @@ __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT)) void k_stress_til
             r = 4 + (e - 8 * WX - 4 * GY) / 4;
             c = GX + 4 + (e - 8 * WX - 4 * GY) % 4;
         }
-        const int gy_ = y0 - 5 + r, gx_ = x0 - 5 + c;
-        f_lds[k] = e < T::NFRAME ? r * VX + c : -1;
-        f_ok[k] = e < T::NFRAME && gy_ >= 0 && gy_ < ny && gx_ >= 0 && gx_ < nx;
-        f_xy[k] = f_ok[k] ? gy_ * sy + gx_ : 0;
+        f_lds[k] = e < T::NFRAME ? r * VX + c : NV - 1;
+        f_xy[k] = clampi(y0 - 5 + r, 0, ny - 1) * sy + clampi(x0 - 5 + c, 0, nx - 1);
     }
 
-    // z rings: r?[k] = v at plane zc - 4 + k; frame registers: plane zc
-    R rx[9], ry[9], rz[9], fv[NF][3];
+    // z rings: at the top of iteration zc, r?[k] holds plane zc - 5 + k and
+    // p? holds plane zc + 4 (loaded one iteration earlier); the shift at the
+    // top makes r?[k] = plane zc - 4 + k.  Frame registers hold plane zc.
+    R rx[9], ry[9], rz[9], px, py, pz, fv[NF][3];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-        const int z = z0 - 5 + k;
-        const bool ok = col_in && z >= 0 && z < nz;
-        rx[k] = ld(vx, col + z * sz, ok);
-        ry[k] = ld(vy, col + z * sz, ok);
-        rz[k] = ld(vz, col + z * sz, ok);
+        const int i = clampi(z0 - 6 + k, 0, nz - 1) * sz + col;
+        rx[k] = vx[i];
+        ry[k] = vy[i];
+        rz[k] = vz[i];
+    }
+    {
+        const int i = clampi(z0 + 3, 0, nz - 1) * sz + col;
+        px = vx[i];
+        py = vy[i];
+        pz = vz[i];
     }
 #pragma unroll
     for (int k = 0; k < NF; ++k) {
-        const bool ok = f_ok[k] && z0 - 1 >= 0;
-        const int gi = (z0 - 1) * sz + f_xy[k];
-        fv[k][0] = ld(vx, gi, ok);
-        fv[k][1] = ld(vy, gi, ok);
-        fv[k][2] = ld(vz, gi, ok);
+        const int i = clampi(z0 - 1, 0, nz - 1) * sz + f_xy[k];
+        fv[k][0] = vx[i];
+        fv[k][1] = vy[i];
+        fv[k][2] = vz[i];
     }
 
     for (int zc = z0 - 1; zc <= z1; ++zc) {
         const bool plane_in = zc >= 0 && zc < nz;
-        // (1) plane zc of v into LDS: own column from the ring, frame from
-        //     the registers loaded one plane ahead
-        if (gact) {
-            vt[0 * NV + vc] = rx[4];
-            vt[1 * NV + vc] = ry[4];
-            vt[2 * NV + vc] = rz[4];
-        }
-#pragma unroll
-        for (int k = 0; k < NF; ++k) {
-            if (f_lds[k] >= 0) {
-                vt[0 * NV + f_lds[k]] = fv[k][0];
-                vt[1 * NV + f_lds[k]] = fv[k][1];
-                vt[2 * NV + f_lds[k]] = fv[k][2];
-            }
-        }
-        // prefetch: frame of plane zc+1, ring plane zc+5, output plane operands
-        {
-            const int zf = zc + 1;
-#pragma unroll
-            for (int k = 0; k < NF; ++k) {
-                const bool ok = f_ok[k] && zf < nz && zf >= 0;
-                const int gi = zf * sz + f_xy[k];
-                fv[k][0] = ld(vx, gi, ok);
-                fv[k][1] = ld(vy, gi, ok);
-                fv[k][2] = ld(vz, gi, ok);
-            }
-        }
-        const int zn = zc + 5;
-        const bool okn = col_in && zn >= 0 && zn < nz;
-        const R nxv = ld(vx, col + zn * sz, okn);
-        const R nyv = ld(vy, col + zn * sz, okn);
-        const R nzv = ld(vz, col + zn * sz, okn);
-        const int zo = zc - 1;
-        const bool out = core && zo >= z0 && zo < z1;
-        const int io = zo * sz + col;
-        R sv[6];
-        int m = 0;
-        R dmp = (R)1;
-        if (out) {
-#pragma unroll
-            for (int k = 0; k < 6; ++k) sv[k] = S.s[k][io];
-            m = S.mats[io];
-            if (S.damp) dmp = S.damp[io];
-        }
-        lds_barrier();
-
-        // (2) strains of plane zc at every column of the tile and its ring
-        const int slot = (zc + 3) % 3;
-        if (gact) {
-            R a[6] = {0, 0, 0, 0, 0, 0};
-            const bool fast = inner_xy && zc >= 4 && zc <= nz - 5;
-            if (fast)
-                strains_col<false, R, VX>(S, vt + vc, vt + NV + vc, vt + 2 * NV + vc, rx, ry, rz,
-                                          ix, iy, zc, a);
-            else if (col_in && plane_in)
-                strains_col<true, R, VX>(S, vt + vc, vt + NV + vc, vt + 2 * NV + vc, rx, ry, rz,
-                                         ix, iy, zc, a);
-            R *gs = gt + slot * NG + tid;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) gs[k * 3 * NG] = a[k];
-        }
-        lds_barrier();
-
-        // (3) stress of plane zc - 1 from the strain planes zc-2, zc-1, zc
-        if (out) {
-            const int sl[3] = {(zc + 1) % 3, (zc + 2) % 3, slot};
-            const bool fast = inner_xy && zo >= 1 && zo <= nz - 2;
-            const int mu = __builtin_amdgcn_readfirstlane(m);
-            if (__all(m == mu)) {
-                ConstRow<R> cm = (ConstRow<R>)(S.Ct + (size_t)mu * 36);
-                if (fast)
-                    stress_cell<false, R, GX, NG>(S, gt, sl, tid, zo, iy, ix, cm, dmp, sv);
-                else
-                    stress_cell<true, R, GX, NG>(S, gt, sl, tid, zo, iy, ix, cm, dmp, sv);
-            } else {
-                const R *cm = S.Ct + (size_t)m * 36;
-                if (fast)
-                    stress_cell<false, R, GX, NG>(S, gt, sl, tid, zo, iy, ix, cm, dmp, sv);
-                else
-                    stress_cell<true, R, GX, NG>(S, gt, sl, tid, zo, iy, ix, cm, dmp, sv);
-            }
-#pragma unroll
-            for (int k = 0; k < 6; ++k) S.s[k][io] = sv[k];
-        }
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             rx[k] = rx[k + 1];
             ry[k] = ry[k + 1];
             rz[k] = rz[k + 1];
         }
-        rx[8] = nxv;
-        ry[8] = nyv;
-        rz[8] = nzv;
+        rx[8] = px;
+        ry[8] = py;
+        rz[8] = pz;
+        // (1) plane zc of v into LDS: own column from the ring, frame from
+        //     the registers loaded one plane ahead
+        vt[0 * NV + vc] = rx[4];
+        vt[1 * NV + vc] = ry[4];
+        vt[2 * NV + vc] = rz[4];
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+            vt[0 * NV + f_lds[k]] = fv[k][0];
+            vt[1 * NV + f_lds[k]] = fv[k][1];
+            vt[2 * NV + f_lds[k]] = fv[k][2];
+        }
+        // loads in the order they are needed: this iteration's output-plane
+        // operands, then the next plane's frame, then the ring's plane zc+5
+        const int zo = zc - 1;
+        const bool out = core && zo >= z0 && zo < z1;
+        const int io = clampi(zo, 0, nz - 1) * sz + col;
+        R sv[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) sv[k] = S.s[k][io];
+        const int m = S.mats[io];
+        const R dmp = has_damp ? S.damp[io] : (R)1;
+        {
+            const int zf = clampi(zc + 1, 0, nz - 1) * sz;
+#pragma unroll
+            for (int k = 0; k < NF; ++k) {
+                fv[k][0] = vx[zf + f_xy[k]];
+                fv[k][1] = vy[zf + f_xy[k]];
+                fv[k][2] = vz[zf + f_xy[k]];
+            }
+        }
+        {
+            const int i = clampi(zc + 5, 0, nz - 1) * sz + col;
+            px = vx[i];
+            py = vy[i];
+            pz = vz[i];
+        }
+        lds_barrier();
+
+        // (2) strains of plane zc at every column of the tile and its ring
+        const int slot = (zc + 3) % 3;
+        {
+            R a[6] = {0, 0, 0, 0, 0, 0};
+            if (inner_xy && zc >= 4 && zc <= nz - 5)
+                strains_col<false, R, VX>(S, vt + vc, vt + NV + vc, vt + 2 * NV + vc, rx, ry, rz,
+                                          ix, iy, zc, a);
+            else if (col_in && plane_in)
+                strains_col<true, R, VX>(S, vt + vc, vt + NV + vc, vt + 2 * NV + vc, rx, ry, rz,
+                                         ix, iy, zc, a);
+            R *gs = gt + slot * NT + tid;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) gs[k * 3 * NT] = a[k];
+        }
+        lds_barrier();
+
+        // (3) stress of plane zc - 1 from the strain planes zc-2, zc-1, zc
+        if (out) {
+            const int sl[3] = {(zc + 1) % 3, (zc + 2) % 3, slot};
+            const bool fast = full && inner_xy && zo >= 1 && zo <= nz - 2;
+            const int mu = __builtin_amdgcn_readfirstlane(m);
+            if (fast && __all(m == mu)) {
+                // every lane of the wave in one grain: wave-uniform row
+                ConstRow<R> cm = (ConstRow<R>)(S.Ct + (size_t)mu * 36);
+                stress_cell<false, true, R, GX, NT>(S, gt, sl, tid, zo, iy, ix, cm, dmp, sv);
+            } else if (fast) {
+                stress_cell<false, true, R, GX, NT>(S, gt, sl, tid, zo, iy, ix,
+                                                    S.Ct + (size_t)m * 36, dmp, sv);
+            } else {
+                stress_cell<true, false, R, GX, NT>(S, gt, sl, tid, zo, iy, ix,
+                                                    S.Ct + (size_t)m * 36, dmp, sv);
+            }
+#pragma unroll
+            for (int k = 0; k < 6; ++k) S.s[k][io] = sv[k];
+        }
     }
 }
 
@@ -407,7 +449,7 @@ __device__ __forceinline__ void accel_col(const Shot<R> &S, const R *p1, const R
 }
 
 template <typename R, int TX, int TY>
-__global__ __launch_bounds__(TX *TY) void k_velocity_tiled(Shot<R> S, int tiles_x, int tiles_y,
+__global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_velocity_tiled(Shot<R> S, int tiles_x, int tiles_y,
                                                           int zc_len)
 {
     using T = VelTile<R, TX, TY>;
@@ -433,7 +475,7 @@ __global__ __launch_bounds__(TX *TY) void k_velocity_tiled(Shot<R> S, int tiles_
     const int lx = tid % TX, ly = tid / TX;
     const int ix = x0 + lx, iy = y0 + ly;
     const bool col_in = ix < nx && iy < ny;
-    const int col = col_in ? iy * sy + ix : 0;
+    const int col = clampi(iy, 0, ny - 1) * sy + clampi(ix, 0, nx - 1);
     const int vc = (ly + 4) * VX + (lx + 4);
 
     const R *__restrict__ s1 = S.s[0];
@@ -444,13 +486,16 @@ __global__ __launch_bounds__(TX *TY) void k_velocity_tiled(Shot<R> S, int tiles_
     const R *__restrict__ s6 = S.s[5];
 
     // this thread's frame entries: x frame (s1, s5, s6 beside the tile
-    // rows) or y frame (s2, s4, s6 above / below the tile columns)
+    // rows) or y frame (s2, s4, s6 above / below the tile columns); LDS slot
+    // (the far corner cell, which no tap reads, if none) and clamped x-y
+    // offset.  Loads are branch-free: a clamped address only feeds terms the
+    // edge guards exclude.
     int f_lds[NF], f_xy[NF];
-    bool f_ok[NF], f_x[NF];
+    bool f_x[NF];
 #pragma unroll
     for (int k = 0; k < NF; ++k) {
         const int e = tid + k * NT;
-        int li = -1, gx_ = 0, gy_ = 0;
+        int li = NV - 1, gx_ = 0, gy_ = 0;
         const bool xf = e < 8 * TY;
         if (xf) {
             const int r = e / 8, q = e % 8;
@@ -467,53 +512,68 @@ __global__ __launch_bounds__(TX *TY) void k_velocity_tiled(Shot<R> S, int tiles_
         }
         f_lds[k] = li;
         f_x[k] = xf;
-        f_ok[k] = li >= 0 && gx_ >= 0 && gx_ < nx && gy_ >= 0 && gy_ < ny;
-        f_xy[k] = f_ok[k] ? gy_ * sy + gx_ : 0;
+        f_xy[k] = clampi(gy_, 0, ny - 1) * sy + clampi(gx_, 0, nx - 1);
     }
 
-    // z rings r?[k] = s? at plane z - 4 + k; registers for plane z: own
-    // column (s1, s2, s6, v, labels, damp) and frame
-    R r3[9], r4[9], r5[9], fv[NF][3];
+    // z rings: at the top of iteration z, r?[k] holds plane z - 5 + k and
+    // p? plane z + 4 (loaded one iteration earlier); after the shift at the
+    // top r?[k] = plane z - 4 + k.  Registers for plane z: own column (s1,
+    // s2, s6, v, labels, damp) and frame.
+    R r3[9], r4[9], r5[9], p3, p4, p5, fv[NF][3];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-        const int z = z0 - 4 + k;
-        const bool ok = col_in && z >= 0 && z < nz;
-        r3[k] = ld(s3, col + z * sz, ok);
-        r4[k] = ld(s4, col + z * sz, ok);
-        r5[k] = ld(s5, col + z * sz, ok);
+        const int i = clampi(z0 - 5 + k, 0, nz - 1) * sz + col;
+        r3[k] = s3[i];
+        r4[k] = s4[i];
+        r5[k] = s5[i];
     }
+    {
+        const int i = clampi(z0 + 4, 0, nz - 1) * sz + col;
+        p3 = s3[i];
+        p4 = s4[i];
+        p5 = s5[i];
+    }
+    const bool has_damp = S.damp != nullptr;
     auto load_frame = [&](int z) {
+        const int zo = clampi(z, 0, nz - 1) * sz;
 #pragma unroll
         for (int k = 0; k < NF; ++k) {
-            const bool ok = f_ok[k] && z < nz;
-            const int gi = z * sz + f_xy[k];
-            fv[k][0] = ld(f_x[k] ? s1 : s2, gi, ok);
-            fv[k][1] = ld(f_x[k] ? s5 : s4, gi, ok);
-            fv[k][2] = ld(s6, gi, ok);
+            const int gi = zo + f_xy[k];
+            fv[k][0] = (f_x[k] ? s1 : s2)[gi];
+            fv[k][1] = (f_x[k] ? s5 : s4)[gi];
+            fv[k][2] = s6[gi];
         }
     };
     R c1, c2, c6, cvx, cvy, cvz, cdmp;
     int cm0, cmx, cmy, cmz;
     auto load_col = [&](int z) {
-        const bool ok = col_in && z < nz;
-        const int i = z * sz + col;
-        c1 = ld(s1, i, ok);
-        c2 = ld(s2, i, ok);
-        c6 = ld(s6, i, ok);
-        cvx = ld(S.v[0], i, ok);
-        cvy = ld(S.v[1], i, ok);
-        cvz = ld(S.v[2], i, ok);
-        cdmp = ok && S.damp ? S.damp[i] : (R)1;
-        cm0 = ok ? S.mats[i] : 0;
-        cmx = ok && ix < nx - 1 ? S.mats[i + 1] : 0;
-        cmy = ok && iy < ny - 1 ? S.mats[i + sy] : 0;
-        cmz = ok && z < nz - 1 ? S.mats[i + sz] : 0;
+        const int i = clampi(z, 0, nz - 1) * sz + col;
+        c1 = s1[i];
+        c2 = s2[i];
+        c6 = s6[i];
+        cvx = S.v[0][i];
+        cvy = S.v[1][i];
+        cvz = S.v[2][i];
+        cdmp = has_damp ? S.damp[i] : (R)1;
+        cm0 = S.mats[i];
+        cmx = S.mats[ix < nx - 1 ? i + 1 : i];
+        cmy = S.mats[iy < ny - 1 ? i + sy : i];
+        cmz = S.mats[z < nz - 1 ? i + sz : i];
     };
     load_frame(z0);
     load_col(z0);
 
     for (int z = z0; z < z1; ++z) {
         const int i = z * sz + col;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            r3[k] = r3[k + 1];
+            r4[k] = r4[k + 1];
+            r5[k] = r5[k + 1];
+        }
+        r3[8] = p3;
+        r4[8] = p4;
+        r5[8] = p5;
         // (1) plane z of s1, s2, s4, s5, s6 into LDS
         t1[vc] = c1;
         t2[vc] = c2;
@@ -522,22 +582,21 @@ __global__ __launch_bounds__(TX *TY) void k_velocity_tiled(Shot<R> S, int tiles_
         t5[vc] = r5[4];
 #pragma unroll
         for (int k = 0; k < NF; ++k) {
-            if (f_lds[k] >= 0) {
-                (f_x[k] ? t1 : t2)[f_lds[k]] = fv[k][0];
-                (f_x[k] ? t5 : t4)[f_lds[k]] = fv[k][1];
-                t6[f_lds[k]] = fv[k][2];
-            }
+            (f_x[k] ? t1 : t2)[f_lds[k]] = fv[k][0];
+            (f_x[k] ? t5 : t4)[f_lds[k]] = fv[k][1];
+            t6[f_lds[k]] = fv[k][2];
         }
         const R vxo = cvx, vyo = cvy, vzo = cvz, dmp = cdmp;
         const int m0 = cm0, mxp = cmx, myp = cmy, mzp = cmz;
         // prefetch plane z+1 and the ring's plane z+5
         load_frame(z + 1);
         load_col(z + 1);
-        const int z5 = z + 5;
-        const bool ok5 = col_in && z5 < nz;
-        const R n3 = ld(s3, col + z5 * sz, ok5);
-        const R n4 = ld(s4, col + z5 * sz, ok5);
-        const R n5 = ld(s5, col + z5 * sz, ok5);
+        {
+            const int i5 = clampi(z + 5, 0, nz - 1) * sz + col;
+            p3 = s3[i5];
+            p4 = s4[i5];
+            p5 = s5[i5];
+        }
         lds_barrier();
 
         if (col_in) {
@@ -558,26 +617,17 @@ __global__ __launch_bounds__(TX *TY) void k_velocity_tiled(Shot<R> S, int tiles_
             S.v[2][i] = (vzo + S.dt_h * az / rvz) * dmp;
         }
         lds_barrier();
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            r3[k] = r3[k + 1];
-            r4[k] = r4[k + 1];
-            r5[k] = r5[k + 1];
-        }
-        r3[8] = n3;
-        r4[8] = n4;
-        r5[8] = n5;
     }
 }
 
 // ------------------------------------------------------------- launchers
-// z planes per workgroup when the caller does not fix it: enough workgroups
-// to cover the chip about four times over, but no chunk shorter than 16
-// planes (each chunk refills its 9-plane rings).
+// z planes per workgroup when the caller does not fix it: about a dozen
+// workgroups per CU (measured best on 400x400x200), but no chunk shorter than
+// 16 planes (each chunk refills its 9-plane rings).
 inline int chunk_len(int nz, int tiles, int zchunk)
 {
     if (zchunk > 0) return zchunk > nz ? nz : zchunk;
-    const int target = 256 * 4;
+    const int target = 256 * 12;
     int chunks = (target + tiles - 1) / tiles;
     chunks = chunks < 1 ? 1 : chunks;
     int len = (nz + chunks - 1) / chunks;
@@ -625,10 +675,10 @@ hipError_t velocity_launch(const Shot<R> &s, int zchunk, hipStream_t st)
 struct TileInfo {
     int tx, ty, pad;
 };
-constexpr TileInfo kStressF32[] = {{32, 16, 0}, {32, 16, 24}, {32, 8, 0}, {32, 8, 24},
-                                   {64, 4, 0},  {64, 4, 24},  {16, 16, 0}, {64, 8, 0}};
+constexpr TileInfo kStressF32[] = {{30, 14, 24}, {62, 6, 0}, {32, 13, 24}, {32, 16, 24},
+                                   {32, 8, 24},  {16, 16, 0}, {32, 16, 0}};
 constexpr TileInfo kStressF64[] = {{32, 6, 0}, {32, 4, 0}, {16, 8, 0}, {32, 6, 24}};
-constexpr TileInfo kVelF32[] = {{32, 16, 0}, {64, 8, 0}, {32, 8, 0}, {64, 4, 0}, {128, 4, 0},
+constexpr TileInfo kVelF32[] = {{64, 8, 0}, {32, 16, 0}, {32, 8, 0}, {64, 4, 0}, {128, 4, 0},
                                 {32, 32, 0}};
 constexpr TileInfo kVelF64[] = {{32, 8, 0}, {64, 4, 0}, {32, 16, 0}, {32, 4, 0}};
 
@@ -654,14 +704,13 @@ template <>
 hipError_t launch_stress_tiled<float>(const Shot<float> &s, int v, int zc, hipStream_t st)
 {
     switch (v) {
-    case 1: return stress_launch<float, 32, 16, 24>(s, zc, st);
-    case 2: return stress_launch<float, 32, 8, 0>(s, zc, st);
-    case 3: return stress_launch<float, 32, 8, 24>(s, zc, st);
-    case 4: return stress_launch<float, 64, 4, 0>(s, zc, st);
-    case 5: return stress_launch<float, 64, 4, 24>(s, zc, st);
-    case 6: return stress_launch<float, 16, 16, 0>(s, zc, st);
-    case 7: return stress_launch<float, 64, 8, 0>(s, zc, st);
-    default: return stress_launch<float, 32, 16, 0>(s, zc, st);
+    case 1: return stress_launch<float, 62, 6, 0>(s, zc, st);
+    case 2: return stress_launch<float, 32, 13, 24>(s, zc, st);
+    case 3: return stress_launch<float, 32, 16, 24>(s, zc, st);
+    case 4: return stress_launch<float, 32, 8, 24>(s, zc, st);
+    case 5: return stress_launch<float, 16, 16, 0>(s, zc, st);
+    case 6: return stress_launch<float, 32, 16, 0>(s, zc, st);
+    default: return stress_launch<float, 30, 14, 24>(s, zc, st);
     }
 }
 
@@ -680,12 +729,12 @@ template <>
 hipError_t launch_velocity_tiled<float>(const Shot<float> &s, int v, int zc, hipStream_t st)
 {
     switch (v) {
-    case 1: return velocity_launch<float, 64, 8>(s, zc, st);
+    case 1: return velocity_launch<float, 32, 16>(s, zc, st);
     case 2: return velocity_launch<float, 32, 8>(s, zc, st);
     case 3: return velocity_launch<float, 64, 4>(s, zc, st);
     case 4: return velocity_launch<float, 128, 4>(s, zc, st);
     case 5: return velocity_launch<float, 32, 32>(s, zc, st);
-    default: return velocity_launch<float, 32, 16>(s, zc, st);
+    default: return velocity_launch<float, 64, 8>(s, zc, st);
     }
 }
 
