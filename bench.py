@@ -92,6 +92,9 @@ def main():
     ap.add_argument("--config", default="c1", choices=["c1", "c0"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--launch", default=None,
+                    help="fixed launch shapes 'stress_variant,stress_zchunk,vel_variant,vel_zchunk' "
+                         "(skips the autotuner)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="also run a plain device copy of this many GiB (PMC calibration)")
     args = ap.parse_args()
@@ -126,7 +129,13 @@ def main():
                              prep["rec_ptr"], prep["rec_idx"], prep["rec_w"],
                              precision=args.precision, kernel=args.kernel)
     tuned = None
-    if args.kernel == "tiled" and not args.no_tune:
+    if args.launch:
+        from pulse_echo_azimuthal_simulation_amd import _abi
+        sv, sz_, vv, vz_ = (int(x) for x in args.launch.split(","))
+        shot.desc.tile[0], shot.desc.zchunk[0], shot.desc.tile[1], shot.desc.zchunk[1] = sv, sz_, vv, vz_
+        tuned = {0: {"tile": _abi.tile_variants(args.precision, 0)[sv], "zchunk": sz_},
+                 1: {"tile": _abi.tile_variants(args.precision, 1)[vv], "zchunk": vz_}}
+    elif args.kernel == "tiled" and not args.no_tune:
         from pulse_echo_azimuthal_simulation_amd import _abi, autotune
         best, _ = autotune.tune(shot)
         tuned = {h: {"tile": _abi.tile_variants(args.precision, h)[best[h][0]],
