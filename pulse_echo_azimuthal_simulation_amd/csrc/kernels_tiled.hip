@@ -296,8 +296,13 @@ __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
 
     // z rings: at the top of iteration zc, r?[k] holds plane zc - 5 + k and
     // p? holds plane zc + 4 (loaded one iteration earlier); the shift at the
-    // top makes r?[k] = plane zc - 4 + k.  Frame registers hold plane zc.
-    R rx[9], ry[9], rz[9], px, py, pz, fv[NF][3];
+    // top makes r?[k] = plane zc - 4 + k.
+    // Frame delay line: fq[d] holds plane zc + d of this thread's frame
+    // entries.  Frame planes are fetched FD planes ahead so that they are
+    // read within about one step of the neighbouring tiles streaming the
+    // same plane into their rings, while the lines are still in L2.
+    constexpr int FD = 1;
+    R rx[9], ry[9], rz[9], px, py, pz, fq[FD][NF][3];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const int i = clampi(z0 - 6 + k, 0, nz - 1) * sz + col;
@@ -311,13 +316,25 @@ __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
         py = vy[i];
         pz = vz[i];
     }
+    // output-plane operands of the next step: s, label, damping
+    R ns[6], nd;
+    int nm;
+    {
+        const int io1 = clampi(z0 - 2, 0, nz - 1) * sz + col;
 #pragma unroll
-    for (int k = 0; k < NF; ++k) {
-        const int i = clampi(z0 - 1, 0, nz - 1) * sz + f_xy[k];
-        fv[k][0] = vx[i];
-        fv[k][1] = vy[i];
-        fv[k][2] = vz[i];
+        for (int k = 0; k < 6; ++k) ns[k] = S.s[k][io1];
+        nm = S.mats[io1];
+        nd = has_damp ? S.damp[io1] : (R)1;
     }
+#pragma unroll
+    for (int d = 0; d < FD; ++d)
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+            const int i = clampi(z0 - 1 + d, 0, nz - 1) * sz + f_xy[k];
+            fq[d][k][0] = vx[i];
+            fq[d][k][1] = vy[i];
+            fq[d][k][2] = vz[i];
+        }
 
     for (int zc = z0 - 1; zc <= z1; ++zc) {
         const bool plane_in = zc >= 0 && zc < nz;
@@ -331,33 +348,50 @@ __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
         ry[8] = py;
         rz[8] = pz;
         // (1) plane zc of v into LDS: own column from the ring, frame from
-        //     the registers loaded one plane ahead
+        //     the delay line
         vt[0 * NV + vc] = rx[4];
         vt[1 * NV + vc] = ry[4];
         vt[2 * NV + vc] = rz[4];
 #pragma unroll
         for (int k = 0; k < NF; ++k) {
-            vt[0 * NV + f_lds[k]] = fv[k][0];
-            vt[1 * NV + f_lds[k]] = fv[k][1];
-            vt[2 * NV + f_lds[k]] = fv[k][2];
+            vt[0 * NV + f_lds[k]] = fq[0][k][0];
+            vt[1 * NV + f_lds[k]] = fq[0][k][1];
+            vt[2 * NV + f_lds[k]] = fq[0][k][2];
         }
+#pragma unroll
+        for (int d = 0; d + 1 < FD; ++d)
+#pragma unroll
+            for (int k = 0; k < NF; ++k) {
+                fq[d][k][0] = fq[d + 1][k][0];
+                fq[d][k][1] = fq[d + 1][k][1];
+                fq[d][k][2] = fq[d + 1][k][2];
+            }
         // loads in the order they are needed: this iteration's output-plane
-        // operands, then the next plane's frame, then the ring's plane zc+5
+        // operands, then the frame of plane zc+FD, then the ring's plane zc+5
+        // this step's output-plane operands (loaded one step ahead), then
+        // the next step's
         const int zo = zc - 1;
         const bool out = core && zo >= z0 && zo < z1;
         const int io = clampi(zo, 0, nz - 1) * sz + col;
         R sv[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) sv[k] = S.s[k][io];
-        const int m = S.mats[io];
-        const R dmp = has_damp ? S.damp[io] : (R)1;
+        for (int k = 0; k < 6; ++k) sv[k] = ns[k];
+        const int m = nm;
+        const R dmp = nd;
         {
-            const int zf = clampi(zc + 1, 0, nz - 1) * sz;
+            const int io1 = clampi(zo + 1, 0, nz - 1) * sz + col;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) ns[k] = S.s[k][io1];
+            nm = S.mats[io1];
+            nd = has_damp ? S.damp[io1] : (R)1;
+        }
+        {
+            const int zf = clampi(zc + FD, 0, nz - 1) * sz;
 #pragma unroll
             for (int k = 0; k < NF; ++k) {
-                fv[k][0] = vx[zf + f_xy[k]];
-                fv[k][1] = vy[zf + f_xy[k]];
-                fv[k][2] = vz[zf + f_xy[k]];
+                fq[FD - 1][k][0] = vx[zf + f_xy[k]];
+                fq[FD - 1][k][1] = vy[zf + f_xy[k]];
+                fq[FD - 1][k][2] = vz[zf + f_xy[k]];
             }
         }
         {
@@ -675,7 +709,7 @@ hipError_t velocity_launch(const Shot<R> &s, int zchunk, hipStream_t st)
 struct TileInfo {
     int tx, ty, pad;
 };
-constexpr TileInfo kStressF32[] = {{30, 14, 24}, {62, 6, 0}, {32, 13, 24}, {32, 16, 24},
+constexpr TileInfo kStressF32[] = {{32, 13, 24}, {62, 6, 0}, {30, 14, 24}, {32, 16, 24},
                                    {32, 8, 24},  {16, 16, 0}, {32, 16, 0}};
 constexpr TileInfo kStressF64[] = {{32, 6, 0}, {32, 4, 0}, {16, 8, 0}, {32, 6, 24}};
 constexpr TileInfo kVelF32[] = {{64, 8, 0}, {32, 16, 0}, {32, 8, 0}, {64, 4, 0}, {128, 4, 0},
@@ -705,12 +739,12 @@ hipError_t launch_stress_tiled<float>(const Shot<float> &s, int v, int zc, hipSt
 {
     switch (v) {
     case 1: return stress_launch<float, 62, 6, 0>(s, zc, st);
-    case 2: return stress_launch<float, 32, 13, 24>(s, zc, st);
+    case 2: return stress_launch<float, 30, 14, 24>(s, zc, st);
     case 3: return stress_launch<float, 32, 16, 24>(s, zc, st);
     case 4: return stress_launch<float, 32, 8, 24>(s, zc, st);
     case 5: return stress_launch<float, 16, 16, 0>(s, zc, st);
     case 6: return stress_launch<float, 32, 16, 0>(s, zc, st);
-    default: return stress_launch<float, 30, 14, 24>(s, zc, st);
+    default: return stress_launch<float, 32, 13, 24>(s, zc, st);
     }
 }
 
