@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PE_ABI_VERSION 1
+#define PE_ABI_VERSION 2
 
 enum { PE_OK = 0, PE_EINVAL = -1, PE_EHIP = -2, PE_ERANGE = -3 };
 enum { PE_F32 = 0, PE_F64 = 1 };
@@ -103,6 +103,13 @@ int pe_record(const pe_shot *shot, void *traces, int32_t n, void *stream);
 int pe_run(const pe_shot *shot, int32_t n0, int32_t n1, const double *wavelet,
            void *traces, void *stream);
 
+/* pe_run that also stores v at the start of every step n into
+ * vhist + (n-n0)*3*ncell (vx, vy, vz planes), the history the adjoint's
+ * stiffness gradient reads.  Replaces the history store of
+ * vendor/ringfwi/elastic3d.py:301-305 (_grad_forward). */
+int pe_run_store(const pe_shot *shot, int32_t n0, int32_t n1, const double *wavelet,
+                 void *traces, void *vhist, void *stream);
+
 /* Tile variants of the tiled kernels, for tuning.  half: 0 stress, 1
  * velocity.  Returns the number of variants at that precision (>= 1) and,
  * when info3 is not NULL, writes (tile x, tile y, LDS row padding) of
@@ -117,6 +124,57 @@ int pe_tile_info(int32_t precision, int32_t half, int32_t variant, int32_t *info
  * serves benchmarks that report per-kernel HBM roofline fractions. */
 int pe_run_timed(const pe_shot *shot, int32_t n0, int32_t n1, const double *wavelet,
                  void *traces, void *stream, double *ms_kernels);
+
+/* ------------------------------------------------------------------------
+ * Adjoint (reverse-mode) step.  The upstream repository's only exact
+ * adjoint is the vendored CPU array-path one,
+ * vendor/ringfwi/elastic3d.py:348-463 (misfit_and_gradient: reverse loop
+ * :386-461, stiffness gradient :412-419); these entry points give the same
+ * reverse-mode derivative for the label-indexed step of pe_run, on the GPU,
+ * with per-material gradients in place of its 21 per-voxel maps.
+ *
+ * Deterministic gradient reduction: the caller cuts every (y, x) column
+ * into runs of constant material (also cut at z % zbreak == 0 when
+ * zbreak > 0); run r of column c is run_ptr[c] + (its order in z).  Each
+ * run owns one row of run_acc (K doubles, K = n_proj ? n_proj : 36);
+ * mat_ptr / mat_runs list each material's runs in ascending order.
+ * ------------------------------------------------------------------------ */
+typedef struct pe_adjoint {
+    void *lv[3];                 /* adjoint of vx, vy, vz (ncell real each) */
+    void *ls[6];                 /* adjoint of s1..s6 */
+    void *work;                  /* 6*ncell real scratch */
+    int32_t n_proj;              /* 0: dJ/dCt, all 36 entries; 1..4: projections */
+    const void *proj;            /* n_mat x n_proj x 36 real (n_proj > 0) */
+    int32_t zbreak;              /* runs also cut at z % zbreak == 0 (0: never) */
+    int32_t n_runs;
+    const int32_t *run_ptr;      /* ny*nx + 1: first run of each column */
+    const int32_t *mat_ptr;      /* n_mat + 1 offsets into mat_runs */
+    const int32_t *mat_runs;     /* n_runs run indices grouped by material */
+    double *run_acc;             /* n_runs x K accumulators, or NULL: no gradient */
+} pe_adjoint;
+
+/* Validate an adjoint descriptor against its shot. */
+int pe_check_adjoint(const pe_shot *shot, const pe_adjoint *adj);
+
+/* Zero lv, ls and run_acc (fill kernels, no memset nodes). */
+int pe_adjoint_reset(const pe_shot *shot, const pe_adjoint *adj, void *stream);
+
+/* Reverse steps n = n_hi-1 down to n_lo.  On entry (lv, ls) are the
+ * adjoints of the state after step n_hi-1; on return, of the state before
+ * step n_lo.  vhist: (n_hi-n_lo) x 3 x ncell real, slot n-n_lo = v at the
+ * start of step n (read only when run_acc is set).  residual: (nt x n_rec)
+ * real trace adjoint dJ/dy indexed by absolute step, or NULL.
+ * grad_wavelet: double per absolute step, receives dJ/dwavelet[n], or NULL.
+ * Receiver cells must be unique within each group.  Replaces the reverse
+ * loop of vendor/ringfwi/elastic3d.py:386-461. */
+int pe_adjoint_run(const pe_shot *shot, const pe_adjoint *adj, int32_t n_hi, int32_t n_lo,
+                   const void *vhist, const void *residual, double *grad_wavelet, void *stream);
+
+/* grad[m*K + k] = sum over the runs of material m of run_acc (K as above):
+ * dJ/dCt[m][k] (n_proj = 0) or the projections.  Replaces the per-voxel
+ * gradient maps g[C_IJ] of vendor/ringfwi/elastic3d.py:412-419 summed over
+ * each grain. */
+int pe_adjoint_gradient(const pe_shot *shot, const pe_adjoint *adj, double *grad, void *stream);
 
 #ifdef __cplusplus
 }

@@ -97,12 +97,18 @@ static void move_table(move_t mv[4][6], int nz, int ny, int nx)
         }
 }
 
+/* Planted-mistake switch for the adjoint tests (0 = exact adjoint):
+ * 1 drops the face-density average, 2 weights 4-tap moves by 1/2,
+ * 3 forgets the damping of the stress adjoint. */
+int orc_adj_plant = 0;
+
 #define REAL float
 #define SUF _f32
 #define FMA fmaf
 #define FP_ENTER const unsigned int fp_saved_ = _mm_getcsr(); _mm_setcsr(fp_saved_ | 0x8040u);
 #define FP_EXIT _mm_setcsr(fp_saved_);
 #include "fdtd_oracle_body.h"
+#include "fdtd_adjoint_body.h"
 #undef REAL
 #undef SUF
 #undef FMA
@@ -115,6 +121,7 @@ static void move_table(move_t mv[4][6], int nz, int ny, int nx)
 #define FP_ENTER
 #define FP_EXIT
 #include "fdtd_oracle_body.h"
+#include "fdtd_adjoint_body.h"
 #undef REAL
 #undef SUF
 #undef FMA

@@ -71,6 +71,7 @@ def _load():
             getattr(lib, "orc_velocity_" + suf).argtypes = [C.POINTER(_Shot)]
             getattr(lib, "orc_source_" + suf).argtypes = [C.POINTER(_Shot), C.c_double]
             getattr(lib, "orc_record_" + suf).argtypes = [C.POINTER(_Shot), C.c_void_p]
+            getattr(lib, "orc_adjoint_step_" + suf).argtypes = [C.POINTER(_Shot)] + [C.c_void_p] * 8
         _lib = lib
     return _lib
 
@@ -164,6 +165,74 @@ class OracleShot:
         row = np.zeros(self._st.n_rec, self.real)
         getattr(_load(), "orc_record_" + self.suf)(C.byref(self._st), row.ctypes.data)
         return row
+
+
+def set_adjoint_plant(k):
+    """Planted mistakes in the adjoint (0 = none; see fdtd_oracle.c)."""
+    C.c_int.in_dll(_load(), "orc_adj_plant").value = int(k)
+
+
+class OracleAdjoint:
+    """Adjoint state (lv, ls) of one OracleShot and the reverse step.
+
+    ``step(vn, res)`` performs reverse step n given v^n (the forward state at
+    the start of step n, three (nz, ny, nx) arrays) and the trace adjoint row
+    dJ/dy_n (or None); it accumulates dJ/dCt into ``gC`` (n_mat, 36) float64
+    and returns dJ/dwavelet[n].
+    """
+
+    def __init__(self, shot):
+        self.shot = shot
+        r = shot.real
+        self.lv = [np.zeros(shot.shape, r) for _ in range(3)]
+        self.ls = np.zeros((6,) + shot.shape, r)
+        self.gC = np.zeros((len(shot.rho_t), 36))
+        self.gbar = np.zeros((6,) + shot.shape, r)
+        self._vn = [np.zeros(shot.shape, r) for _ in range(3)]
+
+    def step(self, vn, res=None, grad=True):
+        sh = self.shot
+        for k in range(3):
+            self._vn[k][...] = vn[k]
+        st = _Shot.from_buffer_copy(sh._st)
+        for k in range(3):
+            st.v[k] = self._vn[k].ctypes.data
+        gw = C.c_double(0.0)
+        resp = None
+        if res is not None:
+            res = np.ascontiguousarray(res, sh.real)
+            resp = res.ctypes.data
+        getattr(_load(), "orc_adjoint_step_" + sh.suf)(
+            C.byref(st), self.lv[0].ctypes.data, self.lv[1].ctypes.data, self.lv[2].ctypes.data,
+            self.ls.ctypes.data, resp, self.gC.ctypes.data if grad else None,
+            C.addressof(gw), self.gbar.ctypes.data)
+        return gw.value
+
+
+def forward_history(shot, wavelet, nt):
+    """Run nt steps from zero fields; returns (traces, v history (nt, 3, ...))
+    where history[n] is v at the start of step n."""
+    hist = np.zeros((nt, 3) + shot.shape, shot.real)
+    traces = np.zeros((nt, shot._st.n_rec), shot.real)
+    for n in range(nt):
+        for k in range(3):
+            hist[n, k] = shot.v[k]
+        shot.run(wavelet, n, n + 1, traces)
+    return traces, hist
+
+
+def adjoint_gradient(shot, wavelet, nt, residual_fn):
+    """Misfit gradient by forward (storing v) then the reverse loop.
+
+    ``residual_fn(traces)`` returns (J, dJ/dtraces).  Returns (J, gC (n_mat,
+    36), gw (nt,), traces)."""
+    traces, hist = forward_history(shot, wavelet, nt)
+    J, res = residual_fn(traces)
+    adj = OracleAdjoint(shot)
+    gw = np.zeros(nt)
+    for n in range(nt - 1, -1, -1):
+        gw[n] = adj.step(hist[n], res[n])
+    return J, adj.gC, gw, traces
 
 
 def run_forward(mats, Ct, rho_t, damp, coeffs, h, dt, nt, src, rec, wavelet,

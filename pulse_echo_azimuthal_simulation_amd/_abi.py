@@ -15,10 +15,11 @@ PE_F32, PE_F64 = 0, 1
 PE_KERNEL_DIRECT, PE_KERNEL_TILED = 0, 1
 RECORD_MODES = {"pressure": 0, "vx": 1, "vy": 2, "vz": 3}
 KERNELS = {"direct": PE_KERNEL_DIRECT, "tiled": PE_KERNEL_TILED}
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 EXPORTS = ("pe_abi_version", "pe_last_error", "pe_check_shot", "pe_stress", "pe_source",
-           "pe_velocity", "pe_record", "pe_run", "pe_run_timed", "pe_tile_info")
+           "pe_velocity", "pe_record", "pe_run", "pe_run_timed", "pe_tile_info", "pe_run_store",
+           "pe_check_adjoint", "pe_adjoint_reset", "pe_adjoint_run", "pe_adjoint_gradient")
 
 
 class PeShot(C.Structure):
@@ -41,6 +42,17 @@ class PeShot(C.Structure):
         ("kernel", C.c_int32),
         ("tile", C.c_int32 * 2),
         ("zchunk", C.c_int32 * 2),
+    ]
+
+
+class PeAdjoint(C.Structure):
+    """Mirror of struct pe_adjoint (include/pefdtd.h)."""
+    _fields_ = [
+        ("lv", C.c_void_p * 3), ("ls", C.c_void_p * 6), ("work", C.c_void_p),
+        ("n_proj", C.c_int32), ("proj", C.c_void_p),
+        ("zbreak", C.c_int32), ("n_runs", C.c_int32),
+        ("run_ptr", C.c_void_p), ("mat_ptr", C.c_void_p), ("mat_runs", C.c_void_p),
+        ("run_acc", C.c_void_p),
     ]
 
 
@@ -73,10 +85,19 @@ def load():
     lib.pe_run.argtypes = [P, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.pe_run_timed.argtypes = [P, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.POINTER(C.c_double)]
+    A = C.POINTER(PeAdjoint)
+    lib.pe_run_store.argtypes = [P, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_void_p]
+    lib.pe_check_adjoint.argtypes = [P, A]
+    lib.pe_adjoint_reset.argtypes = [P, A, C.c_void_p]
+    lib.pe_adjoint_run.argtypes = [P, A, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_void_p]
+    lib.pe_adjoint_gradient.argtypes = [P, A, C.c_void_p, C.c_void_p]
     lib.pe_tile_info.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
     lib.pe_tile_info.restype = C.c_int
     for name in ("pe_check_shot", "pe_stress", "pe_velocity", "pe_source", "pe_record", "pe_run",
-                 "pe_run_timed"):
+                 "pe_run_timed", "pe_run_store", "pe_check_adjoint", "pe_adjoint_reset",
+                 "pe_adjoint_run", "pe_adjoint_gradient"):
         getattr(lib, name).restype = C.c_int
     ver = lib.pe_abi_version()
     if ver != ABI_VERSION:

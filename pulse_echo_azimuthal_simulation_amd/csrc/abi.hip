@@ -54,15 +54,21 @@ int source_record_impl(const pe_shot *p, const double *wavelet, int32_t n, bool 
 
 template <typename R>
 int run_impl(const pe_shot *p, int32_t n0, int32_t n1, const double *wavelet, void *traces,
-             hipStream_t st)
+             hipStream_t st, void *vhist = nullptr)
 {
     const pe::Shot<R> s = pe::make_shot<R>(p);
     const bool direct = p->kernel == PE_KERNEL_DIRECT;
     // Pressure is read from s, which the velocity half-step does not touch,
     // so source and record share one launch before the velocity update.
     const bool rec_before = p->record_mode == PE_RECORD_PRESSURE;
+    const size_t ncell = (size_t)p->nz * p->ny * p->nx;
     for (int32_t n = n0; n < n1; ++n) {
-        hipError_t e = direct ? pe::launch_stress_direct<R>(s, st) : pe::launch_stress_tiled<R>(s, p->tile[0], p->zchunk[0], st);
+        hipError_t e;
+        if (vhist) {
+            e = pe::store_velocity<R>(s, (R *)vhist + (size_t)(n - n0) * 3 * ncell, st);
+            if (e != hipSuccess) return hip_fail(e, "velocity store launch");
+        }
+        e = direct ? pe::launch_stress_direct<R>(s, st) : pe::launch_stress_tiled<R>(s, p->tile[0], p->zchunk[0], st);
         if (e != hipSuccess) return hip_fail(e, "stress launch");
         e = pe::launch_source_record<R>(s, p, wavelet, n, true, (R *)traces,
                                         rec_before && traces != nullptr, st);
@@ -229,6 +235,19 @@ int pe_run(const pe_shot *p, int32_t n0, int32_t n1, const double *wavelet, void
                : run_impl<double>(p, n0, n1, wavelet, traces, (hipStream_t)stream);
 }
 
+int pe_run_store(const pe_shot *p, int32_t n0, int32_t n1, const double *wavelet, void *traces,
+                 void *vhist, void *stream)
+{
+    int rc = pe_check_shot(p);
+    if (rc) return rc;
+    if (n0 < 0 || n1 < n0) return fail(PE_EINVAL, "bad step range [%d, %d)", n0, n1);
+    if (!wavelet && p->n_src > 0) return fail(PE_EINVAL, "wavelet is NULL");
+    if (!vhist) return fail(PE_EINVAL, "vhist is NULL");
+    return p->precision == PE_F32
+               ? run_impl<float>(p, n0, n1, wavelet, traces, (hipStream_t)stream, vhist)
+               : run_impl<double>(p, n0, n1, wavelet, traces, (hipStream_t)stream, vhist);
+}
+
 int pe_tile_info(int32_t precision, int32_t half, int32_t variant, int32_t *info3)
 {
     if (precision != PE_F32 && precision != PE_F64)
@@ -256,6 +275,66 @@ int pe_run_timed(const pe_shot *p, int32_t n0, int32_t n1, const double *wavelet
     return p->precision == PE_F32
                ? run_timed_impl<float>(p, n0, n1, wavelet, traces, (hipStream_t)stream, ms_kernels)
                : run_timed_impl<double>(p, n0, n1, wavelet, traces, (hipStream_t)stream, ms_kernels);
+}
+
+int pe_check_adjoint(const pe_shot *p, const pe_adjoint *a)
+{
+    int rc = pe_check_shot(p);
+    if (rc) return rc;
+    if (!a) return fail(PE_EINVAL, "adjoint descriptor is NULL");
+    for (int k = 0; k < 3; ++k)
+        if (!a->lv[k]) return fail(PE_EINVAL, "velocity adjoint %d is NULL", k);
+    for (int k = 0; k < 6; ++k)
+        if (!a->ls[k]) return fail(PE_EINVAL, "stress adjoint %d is NULL", k + 1);
+    if (!a->work) return fail(PE_EINVAL, "the 6*ncell adjoint scratch is NULL");
+    if (a->n_proj < 0 || a->n_proj > 4)
+        return fail(PE_EINVAL, "n_proj = %d outside [0, 4]", a->n_proj);
+    if (a->n_proj > 0 && !a->proj) return fail(PE_EINVAL, "n_proj > 0 needs proj");
+    if (a->zbreak < 0) return fail(PE_EINVAL, "zbreak must be >= 0");
+    if (a->run_acc) {
+        if (!a->run_ptr || !a->mat_ptr || !a->mat_runs)
+            return fail(PE_EINVAL, "run_acc needs run_ptr, mat_ptr and mat_runs");
+        if (a->n_runs < p->ny * p->nx)
+            return fail(PE_EINVAL, "n_runs = %d is below the %d columns", a->n_runs,
+                        p->ny * p->nx);
+    }
+    return PE_OK;
+}
+
+int pe_adjoint_reset(const pe_shot *p, const pe_adjoint *a, void *stream)
+{
+    int rc = pe_check_adjoint(p, a);
+    if (rc) return rc;
+    hipError_t e = p->precision == PE_F32
+                       ? pe::adjoint_reset<float>(p, a, (hipStream_t)stream)
+                       : pe::adjoint_reset<double>(p, a, (hipStream_t)stream);
+    return e == hipSuccess ? PE_OK : hip_fail(e, "adjoint reset");
+}
+
+int pe_adjoint_run(const pe_shot *p, const pe_adjoint *a, int32_t n_hi, int32_t n_lo,
+                   const void *vhist, const void *residual, double *grad_wavelet, void *stream)
+{
+    int rc = pe_check_adjoint(p, a);
+    if (rc) return rc;
+    if (n_lo < 0 || n_hi < n_lo) return fail(PE_EINVAL, "bad step range [%d, %d)", n_lo, n_hi);
+    if (a->run_acc && !vhist && n_hi > n_lo)
+        return fail(PE_EINVAL, "the stiffness gradient needs the velocity history");
+    hipError_t e = p->precision == PE_F32
+                       ? pe::adjoint_run<float>(p, a, n_hi, n_lo, vhist, residual, grad_wavelet,
+                                                (hipStream_t)stream)
+                       : pe::adjoint_run<double>(p, a, n_hi, n_lo, vhist, residual, grad_wavelet,
+                                                 (hipStream_t)stream);
+    return e == hipSuccess ? PE_OK : hip_fail(e, "adjoint step launch");
+}
+
+int pe_adjoint_gradient(const pe_shot *p, const pe_adjoint *a, double *grad, void *stream)
+{
+    int rc = pe_check_adjoint(p, a);
+    if (rc) return rc;
+    if (!a->run_acc) return fail(PE_EINVAL, "no run accumulators (run_acc is NULL)");
+    if (!grad) return fail(PE_EINVAL, "grad is NULL");
+    hipError_t e = pe::adjoint_finalize(p, a, grad, (hipStream_t)stream);
+    return e == hipSuccess ? PE_OK : hip_fail(e, "adjoint gradient launch");
 }
 
 }  // extern "C"

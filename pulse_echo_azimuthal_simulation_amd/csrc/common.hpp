@@ -71,9 +71,54 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Strain J moved onto position P at flat index i (sim/core/fdtd.py:612-642):
+// per axis (z, y, x) with d = pos[P] - pos[J]: d > 0 taps {0,+1} and needs
+// i <= n-2; d < 0 taps {-1,0} and needs i >= 1.  Taps are summed z-major,
+// left to right, then scaled by 1/2 or 1/4; 0 if any guard fails.
+template <int P, int J, typename R>
+__device__ __forceinline__ R moved_direct(const R *__restrict__ gJ, int i, int iz, int iy,
+                                          int ix, int nz, int ny, int nx, int sy, int sz)
+{
+    constexpr int src = kVP[J];
+    if constexpr (src == P) {
+        return gJ[i];
+    } else {
+        constexpr int dz = kPos[P][0] - kPos[src][0];
+        constexpr int dy = kPos[P][1] - kPos[src][1];
+        constexpr int dx = kPos[P][2] - kPos[src][2];
+        bool ok = true;
+        if constexpr (dz > 0) ok = ok && (iz <= nz - 2);
+        if constexpr (dz < 0) ok = ok && (iz >= 1);
+        if constexpr (dy > 0) ok = ok && (iy <= ny - 2);
+        if constexpr (dy < 0) ok = ok && (iy >= 1);
+        if constexpr (dx > 0) ok = ok && (ix <= nx - 2);
+        if constexpr (dx < 0) ok = ok && (ix >= 1);
+        if (!ok) return (R)0;
+        constexpr int z0 = dz > 0 ? 0 : (dz < 0 ? -1 : 0), nzt = dz != 0 ? 2 : 1;
+        constexpr int y0 = dy > 0 ? 0 : (dy < 0 ? -1 : 0), nyt = dy != 0 ? 2 : 1;
+        constexpr int x0 = dx > 0 ? 0 : (dx < 0 ? -1 : 0), nxt = dx != 0 ? 2 : 1;
+        R acc = (R)0;
+        bool first = true;
+#pragma unroll
+        for (int a = 0; a < nzt; ++a)
+#pragma unroll
+            for (int b = 0; b < nyt; ++b)
+#pragma unroll
+                for (int c = 0; c < nxt; ++c) {
+                    const R t = gJ[i + (z0 + a) * sz + (y0 + b) * sy + (x0 + c)];
+                    acc = first ? t : acc + t;
+                    first = false;
+                }
+        constexpr int ntap = nzt * nyt * nxt;
+        return acc * (ntap == 4 ? (R)0.25 : (R)0.5);
+    }
+}
+
 // Launchers (defined in the kernel translation units).
 template <typename R> hipError_t launch_stress_direct(const Shot<R> &s, hipStream_t st);
 template <typename R> hipError_t launch_velocity_direct(const Shot<R> &s, hipStream_t st);
+// Strains of s.v into the 6*ncell scratch s.g (first kernel of the direct stress half-step).
+template <typename R> hipError_t launch_strain_direct(const Shot<R> &s, hipStream_t st);
 // Tiled launchers: `variant` picks a tile shape (see tile_variants),
 // `zchunk` the z planes per workgroup (0 = automatic).
 template <typename R>
@@ -87,5 +132,14 @@ template <typename R>
 hipError_t launch_source_record(const Shot<R> &s, const pe_shot *p, const double *wavelet,
                                 int32_t n, bool do_source, R *traces, bool do_record,
                                 hipStream_t st);
+
+// Adjoint (kernels_adjoint.hip).
+template <typename R> hipError_t store_velocity(const Shot<R> &s, R *dst, hipStream_t st);
+template <typename R>
+hipError_t adjoint_reset(const pe_shot *p, const pe_adjoint *a, hipStream_t st);
+template <typename R>
+hipError_t adjoint_run(const pe_shot *p, const pe_adjoint *a, int32_t n_hi, int32_t n_lo,
+                       const void *vhist, const void *residual, double *gw, hipStream_t st);
+hipError_t adjoint_finalize(const pe_shot *p, const pe_adjoint *a, double *grad, hipStream_t st);
 
 }  // namespace pe

@@ -1,0 +1,160 @@
+"""The oracle's reverse step against what the mathematics fixes.
+
+The adjoint of a linear map L satisfies <L x, y> = <x, L^T y> for every x, y;
+with random states, a random (non-symmetric) stiffness table, random face
+densities and random damping this pins every transposed stencil, move,
+guard and scaling of one step.  The wavelet-to-trace map F is linear too, so
+<F w, r> = <w, F^T r> pins the recording and source transposes over a whole
+run.  The stiffness gradient (not a transpose) is pinned by central finite
+differences of the misfit.  Each planted mistake must break the identity.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tests import helpers as H
+
+
+def _shot(oracle_lib, inp, precision="f64", record="pressure", Ct=None, rho_t=None):
+    return oracle_lib.OracleShot(inp["mats"], inp["Ct"] if Ct is None else Ct,
+                                 inp["rho_t"] if rho_t is None else rho_t, inp["damp"],
+                                 inp["coeffs"], inp["h"], inp["dt"], inp["src"], inp["rec"],
+                                 record, precision)
+
+
+def _random_tables(inp, rng):
+    """Non-symmetric stiffness rows and spread densities in units where h = 1
+    and C, rho, dt are O(1), so that every term of the step weighs in the
+    identity (the transpose must hold for any table, not only the physical
+    symmetric ones)."""
+    Ct = np.asarray(inp["Ct"], float).reshape(-1, 36)
+    Ct = Ct / np.abs(Ct).max() + 0.3 * rng.uniform(-1, 1, Ct.shape)
+    rho = rng.uniform(0.5, 2.0, len(inp["rho_t"]))
+    return Ct, rho
+
+
+def _one_step_dot(oracle_lib, inp, seed):
+    rng = np.random.default_rng(seed)
+    Ct, rho = _random_tables(inp, rng)
+    inp = dict(inp, damp=rng.uniform(0.6, 1.0, inp["shape"]), h=1.0, dt=0.25)
+    sh = _shot(oracle_lib, inp, Ct=Ct, rho_t=rho)
+    x_v = [rng.normal(size=inp["shape"]) for _ in range(3)]
+    x_s = [rng.normal(size=inp["shape"]) for _ in range(6)]
+    for k in range(3):
+        sh.v[k][...] = x_v[k]
+    for k in range(6):
+        sh.s[k][...] = x_s[k]
+    sh.run(np.zeros(1), 0, 1)                      # L x, wavelet 0
+    y_v = [rng.normal(size=inp["shape"]) for _ in range(3)]
+    y_s = rng.normal(size=(6,) + inp["shape"])
+    lhs = sum(np.vdot(sh.v[k], y_v[k]) for k in range(3)) + \
+        sum(np.vdot(sh.s[k], y_s[k]) for k in range(6))
+    adj = oracle_lib.OracleAdjoint(sh)
+    for k in range(3):
+        adj.lv[k][...] = y_v[k]
+    adj.ls[...] = y_s
+    adj.step(x_v, None, grad=False)                # L^T y
+    rhs = sum(np.vdot(x_v[k], adj.lv[k]) for k in range(3)) + \
+        sum(np.vdot(x_s[k], adj.ls[k]) for k in range(6))
+    return lhs, rhs
+
+
+@pytest.mark.parametrize("shape,seed", [((9, 10, 11), 0), ((12, 8, 14), 1), ((7, 13, 9), 2)])
+def test_one_step_dot_product(oracle_lib, shape, seed):
+    inp = H.random_medium(shape, n_grain=6, seed=seed, sponge=2)
+    lhs, rhs = _one_step_dot(oracle_lib, inp, seed)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs), (lhs, rhs)
+
+
+@pytest.mark.parametrize("order", [2, 4, 6])
+def test_one_step_dot_product_low_orders(oracle_lib, order):
+    inp = H.random_medium((10, 9, 11), n_grain=5, seed=order, order=order, sponge=2)
+    lhs, rhs = _one_step_dot(oracle_lib, inp, order)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs), (lhs, rhs)
+
+
+@pytest.mark.parametrize("plant", [1, 2, 3])
+def test_planted_adjoint_mistakes_break_the_identity(oracle_lib, plant):
+    inp = H.random_medium((9, 10, 11), n_grain=6, seed=0, sponge=2)
+    oracle_lib.set_adjoint_plant(plant)
+    try:
+        lhs, rhs = _one_step_dot(oracle_lib, inp, 0)
+    finally:
+        oracle_lib.set_adjoint_plant(0)
+    assert abs(lhs - rhs) > 1e-6 * abs(lhs)
+
+
+@pytest.mark.parametrize("record", ["pressure", "vx", "vz"])
+def test_wavelet_to_trace_dot_product(oracle_lib, record):
+    """<F w, r> = <w, F^T r> over a 40-step run with footprints, sponge and
+    fluid mask; F^T r is the wavelet gradient of J = <y, r>."""
+    inp = H.random_medium((10, 12, 11), n_grain=7, seed=3, sponge=2)
+    nt = inp["nt"]
+    rng = np.random.default_rng(5)
+    sh = _shot(oracle_lib, inp, record=record)
+    w = inp["wavelet"]
+    y = sh.run(w, 0, nt)
+    r = rng.normal(size=y.shape)
+    sh2 = _shot(oracle_lib, inp, record=record)
+    J, gC, gw, _ = oracle_lib.adjoint_gradient(sh2, w, nt, lambda tr: (float(np.vdot(tr, r)), r))
+    lhs = float(np.vdot(y, r))
+    rhs = float(np.dot(w[:nt], gw))
+    assert abs(lhs) > 0
+    assert abs(lhs - rhs) <= 1e-11 * abs(lhs), (lhs, rhs)
+
+
+def _misfit_case(oracle_lib, seed=4):
+    inp = H.random_medium((10, 11, 12), n_grain=5, seed=seed, sponge=2)
+    rng = np.random.default_rng(seed)
+    Ct_obs = np.asarray(inp["Ct"], float).reshape(-1, 36).copy()
+    Ct_obs[1:] *= 1.0 + 0.05 * rng.uniform(-1, 1, Ct_obs[1:].shape)
+    d = _shot(oracle_lib, inp, Ct=Ct_obs).run(inp["wavelet"], 0, inp["nt"])
+
+    def J_of(Ct):
+        y = _shot(oracle_lib, inp, Ct=Ct).run(inp["wavelet"], 0, inp["nt"])
+        return 0.5 * float(np.sum((y - d) ** 2))
+
+    return inp, d, J_of
+
+
+def test_stiffness_gradient_matches_central_differences(oracle_lib):
+    """dJ/dCt[m, k] from the reverse loop equals the central difference of
+    J = 0.5 ||y(Ct) - d||^2 (f64) for entries of grains and of the couplant,
+    diagonal and off-diagonal, used and unused by the step."""
+    inp, d, J_of = _misfit_case(oracle_lib)
+    Ct0 = np.asarray(inp["Ct"], float).reshape(-1, 36)
+    sh = _shot(oracle_lib, inp)
+    J, gC, _, _ = oracle_lib.adjoint_gradient(sh, inp["wavelet"], inp["nt"],
+                                              lambda tr: (0.5 * float(np.sum((tr - d) ** 2)),
+                                                          tr - d))
+    assert J > 0
+    scale = np.abs(Ct0).max()
+    checked = 0
+    for m, k in [(1, 0), (1, 7), (2, 2), (2, 21), (3, 14), (3, 5), (0, 0), (0, 1), (4, 35), (5, 33)]:
+        if m >= Ct0.shape[0]:
+            continue
+        hstep = 1e-4 * scale
+        Cp = Ct0.copy()
+        Cp[m, k] += hstep
+        Cm = Ct0.copy()
+        Cm[m, k] -= hstep
+        fd = (J_of(Cp) - J_of(Cm)) / (2 * hstep)
+        tol = 1e-6 * np.abs(gC).max()
+        assert abs(fd - gC[m, k]) <= max(tol, 1e-5 * abs(fd)), (m, k, fd, gC[m, k])
+        checked += 1
+    assert checked >= 8
+
+
+def test_f32_adjoint_close_to_f64(oracle_lib):
+    inp, d, _ = _misfit_case(oracle_lib, seed=6)
+    out = {}
+    for prec in ("f32", "f64"):
+        sh = _shot(oracle_lib, inp, precision=prec)
+        dd = d.astype(sh.real)
+        out[prec] = oracle_lib.adjoint_gradient(
+            sh, inp["wavelet"], inp["nt"],
+            lambda tr: (0.5 * float(np.sum((tr - dd) ** 2)), tr - dd))
+    g32, g64 = out["f32"][1], out["f64"][1]
+    assert H.rel_l2(g32, g64) < 2e-4
+    assert H.rel_l2(out["f32"][2], out["f64"][2]) < 2e-4
