@@ -89,7 +89,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--kernel", default="tiled", choices=["tiled", "direct"])
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
-    ap.add_argument("--config", default="c1", choices=["c1", "c0"])
+    ap.add_argument("--config", default=None, choices=["c1", "c0", "c3"],
+                    help="workload (default c1 forward; c3 for --mode adjoint)")
+    ap.add_argument("--mode", default="forward", choices=["forward", "adjoint"],
+                    help="forward: single-shot forward run; adjoint: misfit gradient w.r.t. "
+                         "per-grain c-axis angles (forward + checkpointed reverse pass)")
+    ap.add_argument("--checkpoint-every", type=int, default=200)
+    ap.add_argument("--history-gb", type=float, default=0.0,
+                    help="adjoint mode: HBM budget for velocity history kept from the first "
+                         "pass (segments that fit are not recomputed)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--launch", default=None,
@@ -109,6 +117,10 @@ def main():
 
     from pulse_echo_azimuthal_simulation_amd import solver, specimen
 
+    if args.mode == "adjoint":
+        return bench_adjoint(args)
+    if args.config is None:
+        args.config = "c1"
     t_setup = time.perf_counter()
     nt = max(args.steps, args.warmup, 1)
     if args.config == "c1":
@@ -225,6 +237,99 @@ def main():
         log("CPU baseline %.4f Gcell-updates/s" % out["cpu_baseline"]["value"])
     else:
         out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+def bench_adjoint(args):
+    """Misfit gradient w.r.t. per-grain (theta, phi) on the adjoint workload:
+    forward with checkpoints, then per segment a recompute storing v and the
+    reverse kernels, reducing dJ/dCt projected on dC/dtheta, dC/dphi per grain.
+    value = forward cell-updates per second of the whole gradient
+    (nz*ny*nx*nt / wall); cost ratio = gradient wall / forward wall."""
+    import numpy as np
+    import torch
+
+    from pulse_echo_azimuthal_simulation_amd import adjoint, autotune, fabric, solver, specimen
+
+    cfgname = args.config or "c3"
+    nt = max(args.steps, 1)
+    t_setup = time.perf_counter()
+    if cfgname != "c3":
+        raise SystemExit("--mode adjoint runs the c3 workload")
+    cfg = specimen.config_c3(nt=nt)
+    theta, phi = fabric.angles_from_axes(cfg["axes"])
+    Ct, rho = fabric.stiffness_table(theta, phi)
+    prep = solver.prepare_shot(cfg["lab"], cfg["axes"], cfg["h"], cfg["dt"], cfg["src_pts"],
+                               cfg["rec_groups"], coeffs=cfg["coeffs"],
+                               sponge_width=cfg["sponge_width"], damp_mask=cfg["damp_mask"],
+                               mat_tables=(Ct.astype(np.float32), rho.astype(np.float32)))
+
+    def mk(table):
+        return solver.DeviceShot(prep["mats"], table, prep["rho_t"], prep["damp"],
+                                 prep["coeffs"], cfg["h"], cfg["dt"], prep["src_idx"],
+                                 prep["src_w"], prep["rec_ptr"], prep["rec_idx"], prep["rec_w"],
+                                 precision=args.precision, kernel="tiled")
+
+    tt, pt = fabric.angles_from_axes(cfg["axes_true"])
+    target = mk(fabric.stiffness_table(tt, pt)[0])
+    best, _ = autotune.tune(target)
+    for h in (0, 1):
+        target.desc.tile[h], target.desc.zchunk[h] = best[h][0], best[h][1]
+    wav = target.wavelet(cfg["wavelet"])
+    d = target.new_traces(nt)
+    target.run(wav, 0, nt, d)
+    shot = mk(Ct)
+    for h in (0, 1):
+        shot.desc.tile[h], shot.desc.zchunk[h] = best[h][0], best[h][1]
+    del target
+    proj = fabric.angle_projections(theta, phi)
+    adj = adjoint.DeviceAdjoint(shot, n_proj=2, proj=proj)
+    nz, ny, nx = shot.shape
+    ncell = nz * ny * nx
+    log("adjoint setup %.1f s: grid %s, %d grains, %d steps, dt %.4g"
+        % (time.perf_counter() - t_setup, shot.shape, len(cfg["axes"]), nt, cfg["dt"]))
+
+    def resid(tr):
+        r = tr - d[:tr.shape[0]]
+        return 0.5 * float((r.double() ** 2).sum()), r
+
+    # warm-up (compiles nothing, but pages in every kernel once)
+    adjoint.misfit_gradient(shot, wav, min(nt, 2 * args.checkpoint_every), resid,
+                            checkpoint_every=args.checkpoint_every, adjoint=adj)
+    torch.cuda.synchronize()
+    tr = shot.new_traces(nt)
+    for f in shot.v + shot.s:
+        f.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    shot.run(wav, 0, nt, tr)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    J, g, _ = adjoint.misfit_gradient(shot, wav, nt, resid, checkpoint_every=args.checkpoint_every,
+                                      history_bytes=int(args.history_gb * 2 ** 30), adjoint=adj)
+    gnp = g.cpu().numpy()
+    t_grad = time.perf_counter() - t0
+    log("forward %.3f s, gradient %.3f s (ratio %.2f), J %.4e, |g| max %.3e"
+        % (t_fwd, t_grad, t_grad / t_fwd, J, np.abs(gnp).max()))
+    out = {
+        "metric": METRIC,
+        "value": round(ncell * nt / t_grad / 1e9, 4),
+        "unit": "Gcell-updates/s (fwd+adj, forward-equivalent)",
+        "n_gpus": 1, "steps": nt, "warmup": 0,
+        "ms_per_step": round(t_grad / nt * 1e3, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": args.precision, "data": "synthetic",
+        "config": {"workload": "misfit gradient w.r.t. 500 per-grain (theta, phi), 80x160x160 "
+                               "grid, 1 MHz Hann, target traces from axes tilted N(0, 5 deg), "
+                               "checkpoints every %d steps, history budget %.1f GiB"
+                               % (args.checkpoint_every, args.history_gb),
+                   "grid_zyx": [nz, ny, nx], "cells": ncell, "mode": "fwd+adj"},
+        "forward_s": round(t_fwd, 4), "gradient_s": round(t_grad, 4),
+        "cost_ratio_vs_forward": round(t_grad / t_fwd, 3),
+        "forward_gcells": round(ncell * nt / t_fwd / 1e9, 3),
+        "misfit": J,
+    }
     print(json.dumps(out), flush=True)
 
 

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PE_ABI_VERSION 2
+#define PE_ABI_VERSION 3
 
 enum { PE_OK = 0, PE_EINVAL = -1, PE_EHIP = -2, PE_ERANGE = -3 };
 enum { PE_F32 = 0, PE_F64 = 1 };
@@ -134,20 +134,26 @@ int pe_run_timed(const pe_shot *shot, int32_t n0, int32_t n1, const double *wave
  * with per-material gradients in place of its 21 per-voxel maps.
  *
  * Deterministic gradient reduction: the caller cuts every (y, x) column
- * into runs of constant material (also cut at z % zbreak == 0 when
- * zbreak > 0); run r of column c is run_ptr[c] + (its order in z).  Each
- * run owns one row of run_acc (K doubles, K = n_proj ? n_proj : 36);
- * mat_ptr / mat_runs list each material's runs in ascending order.
+ * into runs of constant material, also cut at z % zbreak == 0 when
+ * zbreak > 0, and numbers the runs column by column (flat y*nx + x), in z
+ * order inside a column.  With nchunk = zbreak > 0 ? ceil(nz / zbreak) : 1,
+ * run_ptr[c*nchunk + k] is the first run of column c at or after
+ * z = k*zbreak.  Each run owns one row of run_acc (K doubles, K = n_proj ?
+ * n_proj : 36), updated by one thread once per run per step, in step
+ * order; mat_ptr / mat_runs list each material's runs in ascending order.
+ * The tiled gradient kernel (n_proj > 0) walks z-chunks of zbreak planes.
  * ------------------------------------------------------------------------ */
 typedef struct pe_adjoint {
     void *lv[3];                 /* adjoint of vx, vy, vz (ncell real each) */
     void *ls[6];                 /* adjoint of s1..s6 */
     void *work;                  /* 6*ncell real scratch */
+    void *wface;                 /* 3*ncell real scratch (PE_KERNEL_TILED) */
+    int32_t kernel;              /* PE_KERNEL_DIRECT or PE_KERNEL_TILED */
     int32_t n_proj;              /* 0: dJ/dCt, all 36 entries; 1..4: projections */
     const void *proj;            /* n_mat x n_proj x 36 real (n_proj > 0) */
     int32_t zbreak;              /* runs also cut at z % zbreak == 0 (0: never) */
     int32_t n_runs;
-    const int32_t *run_ptr;      /* ny*nx + 1: first run of each column */
+    const int32_t *run_ptr;      /* ny*nx*nchunk + 1: first run of each column chunk */
     const int32_t *mat_ptr;      /* n_mat + 1 offsets into mat_runs */
     const int32_t *mat_runs;     /* n_runs run indices grouped by material */
     double *run_acc;             /* n_runs x K accumulators, or NULL: no gradient */
@@ -156,7 +162,8 @@ typedef struct pe_adjoint {
 /* Validate an adjoint descriptor against its shot. */
 int pe_check_adjoint(const pe_shot *shot, const pe_adjoint *adj);
 
-/* Zero lv, ls and run_acc (fill kernels, no memset nodes). */
+/* Zero lv, ls and run_acc (fill kernels, no memset nodes); for
+ * PE_KERNEL_TILED also fill wface with d * dt_h / rho_face. */
 int pe_adjoint_reset(const pe_shot *shot, const pe_adjoint *adj, void *stream);
 
 /* Reverse steps n = n_hi-1 down to n_lo.  On entry (lv, ls) are the

@@ -15,7 +15,7 @@ PE_F32, PE_F64 = 0, 1
 PE_KERNEL_DIRECT, PE_KERNEL_TILED = 0, 1
 RECORD_MODES = {"pressure": 0, "vx": 1, "vy": 2, "vz": 3}
 KERNELS = {"direct": PE_KERNEL_DIRECT, "tiled": PE_KERNEL_TILED}
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 EXPORTS = ("pe_abi_version", "pe_last_error", "pe_check_shot", "pe_stress", "pe_source",
            "pe_velocity", "pe_record", "pe_run", "pe_run_timed", "pe_tile_info", "pe_run_store",
@@ -49,6 +49,7 @@ class PeAdjoint(C.Structure):
     """Mirror of struct pe_adjoint (include/pefdtd.h)."""
     _fields_ = [
         ("lv", C.c_void_p * 3), ("ls", C.c_void_p * 6), ("work", C.c_void_p),
+        ("wface", C.c_void_p), ("kernel", C.c_int32),
         ("n_proj", C.c_int32), ("proj", C.c_void_p),
         ("zbreak", C.c_int32), ("n_runs", C.c_int32),
         ("run_ptr", C.c_void_p), ("mat_ptr", C.c_void_p), ("mat_runs", C.c_void_p),

@@ -34,20 +34,26 @@ def run_tables(mats, zbreak=0):
 
     A run is a maximal z-interval of one (y, x) column with one material,
     also cut at z % zbreak == 0 when zbreak > 0.  Runs are numbered column
-    by column (flat y*nx + x), in z order inside a column.  Returns
-    (run_ptr (ncol+1,), mat_ptr (n_mat+1,), mat_runs (n_runs,), run_mat)
-    as int32, where mat_runs lists the runs of each material ascending.
+    by column (flat y*nx + x), in z order inside a column.  With nchunk =
+    ceil(nz / zbreak) (1 when zbreak = 0), run_ptr[c*nchunk + k] is the
+    first run of column c at or after z = k*zbreak.  Returns (run_ptr
+    (ncol*nchunk+1,), mat_ptr (n_mat+1,), mat_runs (n_runs,), run_mat), all
+    int32, where mat_runs lists the runs of each material ascending.
     """
     m = np.asarray(mats)
     nz = m.shape[0]
     cols = m.reshape(nz, -1).T                           # (ncol, nz)
+    ncol = cols.shape[0]
     start = np.ones(cols.shape, bool)
     start[:, 1:] = cols[:, 1:] != cols[:, :-1]
+    zb = zbreak if 0 < zbreak < nz else nz
     if zbreak > 0:
-        start[:, ::zbreak] = True
-    per_col = start.sum(1)
-    run_ptr = np.zeros(len(per_col) + 1, np.int64)
-    np.cumsum(per_col, out=run_ptr[1:])
+        start[:, ::zb] = True
+    nchunk = -(-nz // zb)
+    first = np.cumsum(start.ravel()).reshape(ncol, nz) - 1   # run index of every cell
+    run_ptr = np.empty(ncol * nchunk + 1, np.int64)
+    run_ptr[:-1] = first[:, ::zb].ravel()
+    run_ptr[-1] = int(start.sum())
     run_mat = cols[start].astype(np.int64)               # row-major: column, then z
     n_mat = int(m.max()) + 1 if m.size else 1
     order = np.argsort(run_mat, kind="stable")
@@ -67,7 +73,7 @@ class DeviceAdjoint:
     dJ/dCt[m, k] instead (``proj`` of shape (n_mat, n_proj, 36)).
     """
 
-    def __init__(self, shot, n_proj=0, proj=None, zbreak=0, gradient=True):
+    def __init__(self, shot, n_proj=0, proj=None, zbreak=None, gradient=True, kernel=None):
         torch = shot.torch
         self.shot = shot
         self.torch = torch
@@ -76,9 +82,21 @@ class DeviceAdjoint:
         if n_proj and proj is None:
             raise ValueError("n_proj > 0 needs a projection table")
         self.K = n_proj if n_proj else 36
+        if kernel is None:
+            kernel = "tiled" if shot.desc.kernel == _abi.PE_KERNEL_TILED else "direct"
+        if zbreak is None:
+            # z-chunk of the tiled gradient kernel: enough workgroups to fill
+            # the device for the grid size, at least 8 planes
+            nz, ny, nx = shot.shape
+            tiles = -(-nx // 32) * -(-ny // 8)
+            zbreak = 0 if kernel == "direct" else max(8, min(nz, -(-nz * tiles // (256 * 8))))
+        self.kernel = kernel
+        self.zbreak = int(zbreak)
         self.lv = [torch.zeros(shot.shape, dtype=real, device=dev) for _ in range(3)]
         self.ls = [torch.zeros(shot.shape, dtype=real, device=dev) for _ in range(6)]
         self.work = torch.zeros((6,) + shot.shape, dtype=real, device=dev)
+        self.wface = (torch.zeros((3,) + shot.shape, dtype=real, device=dev)
+                      if kernel == "tiled" else None)
         self.proj = None
         if n_proj:
             p = np.asarray(proj, float).reshape(n_mat, n_proj, 36)
@@ -86,7 +104,7 @@ class DeviceAdjoint:
         mats = shot.mats.cpu().numpy()
         if mats.max() >= n_mat:
             raise ValueError("material index beyond the table")
-        run_ptr, mat_ptr, mat_runs, _ = run_tables(mats, zbreak)
+        run_ptr, mat_ptr, mat_runs, _ = run_tables(mats, self.zbreak)
         mat_ptr = np.concatenate([mat_ptr, np.full(n_mat + 1 - len(mat_ptr), mat_ptr[-1],
                                                    np.int32)])
         self.n_runs = int(run_ptr[-1])
@@ -101,9 +119,11 @@ class DeviceAdjoint:
         for k in range(6):
             d.ls[k] = self.ls[k].data_ptr()
         d.work = self.work.data_ptr()
+        d.wface = None if self.wface is None else self.wface.data_ptr()
+        d.kernel = _abi.KERNELS[kernel]
         d.n_proj = int(n_proj)
         d.proj = None if self.proj is None else self.proj.data_ptr()
-        d.zbreak = int(zbreak)
+        d.zbreak = self.zbreak
         d.n_runs = self.n_runs
         d.run_ptr = self.run_ptr.data_ptr()
         d.mat_ptr = self.mat_ptr.data_ptr()

@@ -13,7 +13,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libpefdtd.so")
-SOURCES = ["abi.hip", "kernels_direct.hip", "kernels_tiled.hip", "kernels_adjoint.hip"]
+SOURCES = ["abi.hip", "kernels_direct.hip", "kernels_tiled.hip", "kernels_adjoint.hip",
+           "kernels_adjoint_tiled.hip"]
 ARCH = os.environ.get("PEFDTD_ARCH", "gfx950")
 
 # -ffp-contract=off: only the explicit fma() calls fuse, as in the oracle.

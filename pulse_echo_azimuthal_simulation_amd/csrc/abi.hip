@@ -291,12 +291,17 @@ int pe_check_adjoint(const pe_shot *p, const pe_adjoint *a)
         return fail(PE_EINVAL, "n_proj = %d outside [0, 4]", a->n_proj);
     if (a->n_proj > 0 && !a->proj) return fail(PE_EINVAL, "n_proj > 0 needs proj");
     if (a->zbreak < 0) return fail(PE_EINVAL, "zbreak must be >= 0");
+    if (a->kernel != PE_KERNEL_DIRECT && a->kernel != PE_KERNEL_TILED)
+        return fail(PE_EINVAL, "unknown adjoint kernel family %d", a->kernel);
+    if (a->kernel == PE_KERNEL_TILED && !a->wface)
+        return fail(PE_EINVAL, "the tiled adjoint needs the 3*ncell face-weight scratch");
     if (a->run_acc) {
         if (!a->run_ptr || !a->mat_ptr || !a->mat_runs)
             return fail(PE_EINVAL, "run_acc needs run_ptr, mat_ptr and mat_runs");
-        if (a->n_runs < p->ny * p->nx)
-            return fail(PE_EINVAL, "n_runs = %d is below the %d columns", a->n_runs,
-                        p->ny * p->nx);
+        const int64_t nchunk = a->zbreak > 0 ? (p->nz + a->zbreak - 1) / a->zbreak : 1;
+        if ((int64_t)a->n_runs < (int64_t)p->ny * p->nx * nchunk)
+            return fail(PE_EINVAL, "n_runs = %d is below the %lld column chunks", a->n_runs,
+                        (long long)p->ny * p->nx * nchunk);
     }
     return PE_OK;
 }

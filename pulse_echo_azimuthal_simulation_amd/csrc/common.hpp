@@ -133,6 +133,36 @@ hipError_t launch_source_record(const Shot<R> &s, const pe_shot *p, const double
                                 int32_t n, bool do_source, R *traces, bool do_record,
                                 hipStream_t st);
 
+// Kernel-side view of a pe_adjoint, passed by value.
+template <typename R>
+struct Adj {
+    R *lv[3];
+    R *ls[6];
+    R *work;
+    R *wface;
+    const R *proj;
+    int n_proj;
+    int zbreak;
+    const int32_t *run_ptr;
+    double *run_acc;
+};
+
+template <typename R>
+inline Adj<R> make_adj(const pe_adjoint *a)
+{
+    Adj<R> A;
+    for (int k = 0; k < 3; ++k) A.lv[k] = (R *)a->lv[k];
+    for (int k = 0; k < 6; ++k) A.ls[k] = (R *)a->ls[k];
+    A.work = (R *)a->work;
+    A.wface = (R *)a->wface;
+    A.proj = (const R *)a->proj;
+    A.n_proj = a->n_proj;
+    A.zbreak = a->zbreak;
+    A.run_ptr = a->run_ptr;
+    A.run_acc = a->run_acc;
+    return A;
+}
+
 // Adjoint (kernels_adjoint.hip).
 template <typename R> hipError_t store_velocity(const Shot<R> &s, R *dst, hipStream_t st);
 template <typename R>
@@ -141,5 +171,15 @@ template <typename R>
 hipError_t adjoint_run(const pe_shot *p, const pe_adjoint *a, int32_t n_hi, int32_t n_lo,
                        const void *vhist, const void *residual, double *gw, hipStream_t st);
 hipError_t adjoint_finalize(const pe_shot *p, const pe_adjoint *a, double *grad, hipStream_t st);
+// Tiled adjoint (kernels_adjoint_tiled.hip).  stage 0: D^T (damp_ls: apply
+// the deferred stress-adjoint damping); 1: projected stiffness gradient on
+// Sv (v = history slot); 2: M^T C^T then G^T.
+template <typename R>
+hipError_t adjoint_face_weights(const pe_shot *p, const pe_adjoint *a, hipStream_t st);
+template <typename R>
+hipError_t adjoint_step_tiled(const Shot<R> &S, const Shot<R> &Sv, const Adj<R> &A, int stage,
+                              int damp_ls, hipStream_t st);
+template <typename R>
+hipError_t adjoint_scale_ls(const pe_shot *p, const pe_adjoint *a, hipStream_t st);
 
 }  // namespace pe

@@ -46,33 +46,6 @@ constexpr int kBlock = 256;
 constexpr int kColBlock = 64;
 constexpr int kSrBlock = 1024;
 
-template <typename R>
-struct Adj {
-    R *lv[3];
-    R *ls[6];
-    R *work;
-    const R *proj;
-    int n_proj;
-    int zbreak;
-    const int32_t *run_ptr;
-    double *run_acc;
-};
-
-template <typename R>
-Adj<R> make_adj(const pe_adjoint *a)
-{
-    Adj<R> A;
-    for (int k = 0; k < 3; ++k) A.lv[k] = (R *)a->lv[k];
-    for (int k = 0; k < 6; ++k) A.ls[k] = (R *)a->ls[k];
-    A.work = (R *)a->work;
-    A.proj = (const R *)a->proj;
-    A.n_proj = a->n_proj;
-    A.zbreak = a->zbreak;
-    A.run_ptr = a->run_ptr;
-    A.run_acc = a->run_acc;
-    return A;
-}
-
 // e_a at cell i: dt_h * (d * lv_a) / rho_face_a, with the forward's face
 // density rule (mean with the +a neighbour, the last slice keeps rho[m]).
 template <int A, typename R>
@@ -293,7 +266,8 @@ __global__ __launch_bounds__(kColBlock) void k_adj_grad_direct(Shot<R> S, Adj<R>
     double acc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = 0.0;
-    int run = A.run_ptr[col];
+    const int nchunk = A.zbreak > 0 ? (nz + A.zbreak - 1) / A.zbreak : 1;
+    int run = A.run_ptr[(size_t)col * nchunk];
     int prev = S.mats[col];
     for (int iz = 0; iz < nz; ++iz) {
         const int i = iz * sz + col;
@@ -406,19 +380,30 @@ __global__ __launch_bounds__(kBlock) void k_store_v(R *__restrict__ dst, const R
     dst[2 * (size_t)n + i] = v2[i];
 }
 
-// grad[m][k] = sum of run_acc over the runs of material m, ascending.
-__global__ __launch_bounds__(kBlock) void k_adj_finalize(int n_mat, int K,
-                                                         const int32_t *__restrict__ mat_ptr,
+// grad[m][k] = sum of run_acc over the runs of material m.  One workgroup
+// per material: thread t sums runs t, t+256, ... in ascending order, then a
+// fixed tree over the 256 partials (deterministic).
+__global__ __launch_bounds__(kBlock) void k_adj_finalize(int K, const int32_t *__restrict__ mat_ptr,
                                                          const int32_t *__restrict__ mat_runs,
                                                          const double *__restrict__ run_acc,
                                                          double *__restrict__ grad)
 {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= n_mat * K) return;
-    const int m = t / K, k = t % K;
-    double s = 0.0;
-    for (int r = mat_ptr[m]; r < mat_ptr[m + 1]; ++r) s += run_acc[(size_t)mat_runs[r] * K + k];
-    grad[t] = s;
+    __shared__ double part[kBlock];
+    const int m = blockIdx.x;
+    const int r0 = mat_ptr[m], r1 = mat_ptr[m + 1];
+    for (int k = 0; k < K; ++k) {
+        double s = 0.0;
+        for (int r = r0 + (int)threadIdx.x; r < r1; r += kBlock)
+            s += run_acc[(size_t)mat_runs[r] * K + k];
+        part[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = kBlock / 2; w >= 1; w >>= 1) {
+            if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) grad[(size_t)m * K + k] = part[0];
+        __syncthreads();
+    }
 }
 
 inline unsigned blocks_of(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -452,6 +437,7 @@ hipError_t adjoint_reset(const pe_shot *p, const pe_adjoint *a, hipStream_t st)
     for (int k = 0; k < 6 && e == hipSuccess; ++k) e = fill_zero((R *)a->ls[k], ncell, st);
     const int K = a->n_proj > 0 ? a->n_proj : 36;
     if (e == hipSuccess && a->run_acc) e = fill_zero(a->run_acc, (size_t)a->n_runs * K, st);
+    if (e == hipSuccess && a->kernel == PE_KERNEL_TILED) e = adjoint_face_weights<R>(p, a, st);
     return e;
 }
 
@@ -469,14 +455,22 @@ hipError_t adjoint_run(const pe_shot *p, const pe_adjoint *a, int32_t n_hi, int3
     Shot<R> Sv = S;
     Sv.g = A.work;
     hipError_t e = hipSuccess;
+    const bool tiled = a->kernel == PE_KERNEL_TILED;
     for (int32_t n = n_hi - 1; n >= n_lo && e == hipSuccess; --n) {
         if (rec && vel_mode) {
             hipLaunchKernelGGL(k_adj_record<R>, dim3(1), dim3(kSrBlock), 0, st, A, n, p->n_rec,
                                p->rec_ptr, p->rec_idx, (const R *)p->rec_w, p->record_mode, res,
                                0, (const int64_t *)nullptr, (const R *)nullptr, (double *)nullptr);
         }
-        hipLaunchKernelGGL(k_adj_velocity_direct<R>, dim3(blocks_of(ncell, kBlock)), dim3(kBlock),
-                           0, st, S, A);
+        if (tiled) {
+            // the first reverse step of a call starts from consistent adjoints;
+            // later ones apply the previous step's deferred damping of ls
+            e = adjoint_step_tiled<R>(S, Sv, A, 0, n != n_hi - 1, st);
+            if (e != hipSuccess) break;
+        } else {
+            hipLaunchKernelGGL(k_adj_velocity_direct<R>, dim3(blocks_of(ncell, kBlock)),
+                               dim3(kBlock), 0, st, S, A);
+        }
         const bool pres = rec && !vel_mode;
         const bool want_gw = gw && p->n_src > 0;
         if (pres || want_gw) {
@@ -488,6 +482,11 @@ hipError_t adjoint_run(const pe_shot *p, const pe_adjoint *a, int32_t n_hi, int3
         if (a->run_acc) {
             const R *vn = (const R *)vhist + (size_t)(n - n_lo) * 3 * ncell;
             for (int k = 0; k < 3; ++k) Sv.v[k] = (R *)(vn + (size_t)k * ncell);
+        }
+        if (a->run_acc && tiled && a->n_proj > 0) {
+            e = adjoint_step_tiled<R>(S, Sv, A, 1, 0, st);
+            if (e != hipSuccess) break;
+        } else if (a->run_acc) {
             e = launch_strain_direct<R>(Sv, st);
             if (e != hipSuccess) break;
             switch (a->n_proj) {
@@ -498,21 +497,27 @@ hipError_t adjoint_run(const pe_shot *p, const pe_adjoint *a, int32_t n_hi, int3
             default: hipLaunchKernelGGL((k_adj_grad_direct<R, 4>), dim3(blocks_of(ncol, kColBlock)), dim3(kColBlock), 0, st, S, A); break;
             }
         }
-        hipLaunchKernelGGL(k_adj_gbar_direct<R>, dim3(blocks_of(ncell, kBlock)), dim3(kBlock), 0,
-                           st, S, A);
-        hipLaunchKernelGGL(k_adj_strain_t_direct<R>, dim3(blocks_of(ncell, kBlock)), dim3(kBlock),
-                           0, st, S, A);
+        if (tiled) {
+            e = adjoint_step_tiled<R>(S, Sv, A, 2, 0, st);
+            if (e != hipSuccess) break;
+        } else {
+            hipLaunchKernelGGL(k_adj_gbar_direct<R>, dim3(blocks_of(ncell, kBlock)), dim3(kBlock),
+                               0, st, S, A);
+            hipLaunchKernelGGL(k_adj_strain_t_direct<R>, dim3(blocks_of(ncell, kBlock)),
+                               dim3(kBlock), 0, st, S, A);
+        }
         e = hipGetLastError();
     }
+    // the tiled family defers the last step's damping of ls
+    if (e == hipSuccess && tiled && n_hi > n_lo) e = adjoint_scale_ls<R>(p, a, st);
     return e;
 }
 
 hipError_t adjoint_finalize(const pe_shot *p, const pe_adjoint *a, double *grad, hipStream_t st)
 {
     const int K = a->n_proj > 0 ? a->n_proj : 36;
-    const int total = p->n_mat * K;
-    hipLaunchKernelGGL(k_adj_finalize, dim3(blocks_of(total, kBlock)), dim3(kBlock), 0, st,
-                       p->n_mat, K, a->mat_ptr, a->mat_runs, (const double *)a->run_acc, grad);
+    hipLaunchKernelGGL(k_adj_finalize, dim3(p->n_mat), dim3(kBlock), 0, st, K, a->mat_ptr,
+                       a->mat_runs, (const double *)a->run_acc, grad);
     return hipGetLastError();
 }
 

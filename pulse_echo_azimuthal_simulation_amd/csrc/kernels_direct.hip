@@ -171,9 +171,10 @@ __global__ __launch_bounds__(kBlock) void k_velocity_direct(Shot<R> S)
 // Phase 1 adds (real)wavelet[n] * w_k to s1..s3 at the (unique) footprint
 // cells (sim/core/fdtd.py:1798-1800).  Phase 2, after a barrier, forms each
 // receiver group's weighted sum of -(s1+s2+s3)/3 or of one velocity
-// component (sim/core/fdtd.py:1803-1813): wave w takes groups w, w+nw, ...;
-// lane l sums cells l, l+64, ... in order; the 64 partials are combined by a
-// fixed butterfly, so the result is deterministic.
+// component (sim/core/fdtd.py:1803-1813).  With at least one group per wave,
+// wave w takes groups w, w+nw, ...; lane l sums cells l, l+64, ... in order;
+// the 64 partials are combined by a fixed butterfly.  With fewer groups all
+// 1024 threads share each group.  Either way the result is deterministic.
 constexpr int kSrBlock = 1024;
 
 template <typename R>
@@ -197,20 +198,43 @@ __global__ __launch_bounds__(kSrBlock) void k_source_record(
     if (!do_record) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     constexpr int nwave = kSrBlock / 64;
-    for (int r = wave; r < n_rec; r += nwave) {
-        R acc = 0;
-        for (int k = rec_ptr[r] + lane; k < rec_ptr[r + 1]; k += 64) {
-            const int64_t q = rec_idx[k];
-            R val;
-            if (mode == PE_RECORD_PRESSURE)
-                val = -(S.s[0][q] + S.s[1][q] + S.s[2][q]) / (R)3;
-            else
-                val = S.v[mode - 1][q];
-            acc = acc + val * rec_w[k];
+    auto cell = [&](int k) {
+        const int64_t q = rec_idx[k];
+        R val;
+        if (mode == PE_RECORD_PRESSURE)
+            val = -(S.s[0][q] + S.s[1][q] + S.s[2][q]) / (R)3;
+        else
+            val = S.v[mode - 1][q];
+        return val * rec_w[k];
+    };
+    if (n_rec >= nwave) {
+        for (int r = wave; r < n_rec; r += nwave) {
+            R acc = 0;
+            for (int k = rec_ptr[r] + lane; k < rec_ptr[r + 1]; k += 64) acc = acc + cell(k);
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) acc = acc + __shfl_xor(acc, off, 64);
+            if (lane == 0) traces[(size_t)n * n_rec + r] = acc;
         }
+        return;
+    }
+    // few groups (a piston): every wave sums a share of each group's cells;
+    // thread t takes cells t, t+1024, ...; lanes by a fixed butterfly, waves
+    // in order
+    __shared__ R part[nwave];
+    for (int r = 0; r < n_rec; ++r) {
+        R acc = 0;
+        for (int k = rec_ptr[r] + (int)threadIdx.x; k < rec_ptr[r + 1]; k += kSrBlock)
+            acc = acc + cell(k);
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) acc = acc + __shfl_xor(acc, off, 64);
-        if (lane == 0) traces[(size_t)n * n_rec + r] = acc;
+        if (lane == 0) part[wave] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            R t = part[0];
+            for (int w = 1; w < nwave; ++w) t = t + part[w];
+            traces[(size_t)n * n_rec + r] = t;
+        }
+        __syncthreads();
     }
 }
 

@@ -19,216 +19,13 @@
 // consecutive workgroup ids to the 8 XCDs round robin; the id is remapped so
 // that each XCD receives a contiguous run of neighbouring tiles and their
 // shared halo planes stay in that XCD's L2.
-#include "common.hpp"
+#include "tiled_common.hpp"
 
 namespace pe {
 
 namespace {
 
-constexpr int kXcd = 8;
-
-// Remap a workgroup id so consecutive logical tiles land on the same XCD.
-__device__ __forceinline__ int xcd_remap(int bid, int nb)
-{
-    const int per = nb / kXcd;
-    if (bid >= per * kXcd) return bid;
-    return (bid % kXcd) * per + bid / kXcd;
-}
-
-template <typename R>
-__device__ __forceinline__ R ld(const R *p, int i, bool ok)
-{
-    return ok ? p[i] : (R)0;
-}
-
-// ------------------------------------------------------------------ stress
-template <typename R, int TX, int TY, int PAD>
-struct StressTile {
-    static constexpr int GX = TX + 2, GY = TY + 2, NG = GX * GY;   // strain columns
-    static constexpr int NT = (NG + 63) / 64 * 64;                 // threads
-    // v plane tile; PAD extra words per row shift consecutive tile rows
-    // onto disjoint LDS banks
-    static constexpr int WX = GX + 8, VX = WX + PAD, VY = GY + 8, NV = VX * VY;
-    static constexpr int NFRAME = 8 * (WX + GY);                   // v tile outside G
-    static constexpr size_t LDS = sizeof(R) * (3 * NV + 6 * 3 * NT);
-};
-
-// Moved strain q_{P,J} read from the LDS strain ring (see moved_direct in
-// kernels_direct.hip for the rule).  sl[d+1] is the ring slot of plane z+d.
-// G = false drops the edge guards (the caller guarantees they all hold).
-template <bool G, int P, int J, typename R, int GX, int NG>
-__device__ __forceinline__ R moved_lds(const R *__restrict__ gt, const int (&sl)[3], int c,
-                                       int iz, int iy, int ix, int nz, int ny, int nx)
-{
-    const R *gJ = gt + J * 3 * NG;
-    constexpr int src = kVP[J];
-    if constexpr (src == P) {
-        return gJ[sl[1] * NG + c];
-    } else {
-        constexpr int dz = kPos[P][0] - kPos[src][0];
-        constexpr int dy = kPos[P][1] - kPos[src][1];
-        constexpr int dx = kPos[P][2] - kPos[src][2];
-        bool ok = true;
-        if constexpr (G) {
-            if constexpr (dz > 0) ok = ok && (iz <= nz - 2);
-            if constexpr (dz < 0) ok = ok && (iz >= 1);
-            if constexpr (dy > 0) ok = ok && (iy <= ny - 2);
-            if constexpr (dy < 0) ok = ok && (iy >= 1);
-            if constexpr (dx > 0) ok = ok && (ix <= nx - 2);
-            if constexpr (dx < 0) ok = ok && (ix >= 1);
-        }
-        constexpr int z0 = dz < 0 ? -1 : 0, nzt = dz != 0 ? 2 : 1;
-        constexpr int y0 = dy < 0 ? -1 : 0, nyt = dy != 0 ? 2 : 1;
-        constexpr int x0 = dx < 0 ? -1 : 0, nxt = dx != 0 ? 2 : 1;
-        R acc = (R)0;
-#pragma unroll
-        for (int a = 0; a < nzt; ++a)
-#pragma unroll
-            for (int b = 0; b < nyt; ++b)
-#pragma unroll
-                for (int e = 0; e < nxt; ++e) {
-                    const R t = gJ[sl[1 + z0 + a] * NG + c + (y0 + b) * GX + (x0 + e)];
-                    acc = (a == 0 && b == 0 && e == 0) ? t : acc + t;
-                }
-        constexpr int ntap = nzt * nyt * nxt;
-        const R q = acc * (ntap == 4 ? (R)0.25 : (R)0.5);
-        if constexpr (G) return ok ? q : (R)0;
-        return q;
-    }
-}
-
-// One stress component: acc = sum over active J (ascending) of C_IJ q_J, then
-// s = fma(dt, acc, s * damp) (stress_lab_u, sim/core/fdtd.py:643-657).
-// FULL = true: every C_IJ of the table is active (no per-term test).
-// Six consecutive row entries starting at an 8-byte (float) / 16-byte
-// (double) aligned offset: three paired loads per lane.
-__device__ __forceinline__ void load6(const float *p, float (&c)[6])
-{
-    const float2 *q = reinterpret_cast<const float2 *>(p);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const float2 t = q[k];
-        c[2 * k] = t.x;
-        c[2 * k + 1] = t.y;
-    }
-}
-__device__ __forceinline__ void load6(const double *p, double (&c)[6])
-{
-    const double2 *q = reinterpret_cast<const double2 *>(p);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const double2 t = q[k];
-        c[2 * k] = t.x;
-        c[2 * k + 1] = t.y;
-    }
-}
-template <typename R>
-__device__ __forceinline__ void load6(const __attribute__((address_space(4))) R *p, R (&c)[6])
-{
-#pragma unroll
-    for (int k = 0; k < 6; ++k) c[k] = p[k];
-}
-
-template <int I, bool FULL, typename R, typename Row>
-__device__ __forceinline__ void stress_out(const Shot<R> &S, Row cm, const R (&q)[6], R dmp, R &sv)
-{
-    R row[6];
-    load6(cm + 6 * I, row);
-    R acc = 0;
-    bool any = FULL;
-#pragma unroll
-    for (int J = 0; J < 6; ++J) {
-        if (FULL || ((S.act >> (6 * I + J)) & 1ull)) {
-            acc = fma_r(row[J], q[J], acc);
-            any = true;
-        }
-    }
-    sv = sv * dmp;
-    if (any) sv = fma_r(S.dt, acc, sv);
-}
-
-template <bool G, int P, typename R, int GX, int NG>
-__device__ __forceinline__ void gather_lds(const R *__restrict__ gt, const int (&sl)[3], int c,
-                                           int iz, int iy, int ix, int nz, int ny, int nx,
-                                           R (&q)[6])
-{
-    q[0] = moved_lds<G, P, 0, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx);
-    q[1] = moved_lds<G, P, 1, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx);
-    q[2] = moved_lds<G, P, 2, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx);
-    q[3] = moved_lds<G, P, 3, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx);
-    q[4] = moved_lds<G, P, 4, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx);
-    q[5] = moved_lds<G, P, 5, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx);
-}
-
-// Strains g1..g6 of plane zc at one column (strains_zp statement order,
-// sim/core/fdtd.py:1136-1164): x / y taps from the LDS plane tile (pointers
-// at the column), z taps from the register rings (r?[k] = plane zc-4+k).
-// G = false drops the per-term edge guards.
-template <bool G, typename R, int VX>
-__device__ __forceinline__ void strains_col(const Shot<R> &S, const R *tx_, const R *ty_,
-                                            const R *tz_, const R (&rx)[9], const R (&ry)[9],
-                                            const R (&rz)[9], int ix, int iy, int zc, R (&a)[6])
-{
-    const int nz = S.nz, ny = S.ny, nx = S.nx;
-    R a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0, a6 = 0;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        if (j >= S.nc) break;
-        const R cj = S.c[j];
-        if (!G || (ix >= j + 1 && ix < nx - j)) a1 = fma_r(cj, tx_[j] - tx_[-(j + 1)], a1);
-        if (!G || (iy >= j + 1 && iy < ny - j)) a2 = fma_r(cj, ty_[j * VX] - ty_[-(j + 1) * VX], a2);
-        if (!G || (zc >= j + 1 && zc < nz - j)) a3 = fma_r(cj, rz[4 + j] - rz[3 - j], a3);
-        if (!G || (iy >= j && iy < ny - j - 1)) a4 = fma_r(cj, tz_[(j + 1) * VX] - tz_[-j * VX], a4);
-        if (!G || (zc >= j && zc < nz - j - 1)) a4 = fma_r(cj, ry[5 + j] - ry[4 - j], a4);
-        if (!G || (ix >= j && ix < nx - j - 1)) a5 = fma_r(cj, tz_[j + 1] - tz_[-j], a5);
-        if (!G || (zc >= j && zc < nz - j - 1)) a5 = fma_r(cj, rx[5 + j] - rx[4 - j], a5);
-        if (!G || (ix >= j && ix < nx - j - 1)) a6 = fma_r(cj, ty_[j + 1] - ty_[-j], a6);
-        if (!G || (iy >= j && iy < ny - j - 1)) a6 = fma_r(cj, tx_[(j + 1) * VX] - tx_[-j * VX], a6);
-    }
-    a[0] = a1 * S.inv_h;
-    a[1] = a2 * S.inv_h;
-    a[2] = a3 * S.inv_h;
-    a[3] = a4 * S.inv_h;
-    a[4] = a5 * S.inv_h;
-    a[5] = a6 * S.inv_h;
-}
-
-// Stiffness rows are read in place.  A wave whose active lanes share one
-// material (the common case: grains span thousands of cells) reads its row
-// through a wave-uniform pointer in the constant address space, which the
-// compiler serves with scalar loads.
-template <typename R>
-using ConstRow = const __attribute__((address_space(4))) R *;
-
-template <bool G, bool FULL, typename R, int GX, int NG, typename Row>
-__device__ __forceinline__ void stress_cell(const Shot<R> &S, const R *__restrict__ gt,
-                                            const int (&sl)[3], int c, int iz, int iy, int ix,
-                                            Row cm, R dmp, R (&sv)[6])
-{
-    const int nz = S.nz, ny = S.ny, nx = S.nx;
-    R q[6];
-    gather_lds<G, 0, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx, q);
-    stress_out<0, FULL>(S, cm, q, dmp, sv[0]);
-    stress_out<1, FULL>(S, cm, q, dmp, sv[1]);
-    stress_out<2, FULL>(S, cm, q, dmp, sv[2]);
-    gather_lds<G, 1, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx, q);
-    stress_out<3, FULL>(S, cm, q, dmp, sv[3]);
-    gather_lds<G, 2, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx, q);
-    stress_out<4, FULL>(S, cm, q, dmp, sv[4]);
-    gather_lds<G, 3, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx, q);
-    stress_out<5, FULL>(S, cm, q, dmp, sv[5]);
-}
-
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
-
-// Global loads are branch-free: every address is clamped into the grid and
-// the values a clamped address returns are never used (the edge guards of
-// the generic path exclude them; the fast path only runs where none occur).
-// Waves per SIMD the register allocator must leave room for so that two
-// workgroups of NT threads fit on a CU (float); double keeps the compiler's
-// choice (its larger register footprint would spill).
-template <typename R, int NT> struct MinWaves { static constexpr int v = 1; };
-template <int NT> struct MinWaves<float, NT> { static constexpr int v = (2 * (NT / 64) + 3) / 4; };
+using namespace tiled;
 
 template <typename R, int TX, int TY, int PAD>
 __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
@@ -655,28 +452,6 @@ __global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_velocity_
 }
 
 // ------------------------------------------------------------- launchers
-// z planes per workgroup when the caller does not fix it: about a dozen
-// workgroups per CU (measured best on 400x400x200), but no chunk shorter than
-// 16 planes (each chunk refills its 9-plane rings).
-inline int chunk_len(int nz, int tiles, int zchunk)
-{
-    if (zchunk > 0) return zchunk > nz ? nz : zchunk;
-    const int target = 256 * 12;
-    int chunks = (target + tiles - 1) / tiles;
-    chunks = chunks < 1 ? 1 : chunks;
-    int len = (nz + chunks - 1) / chunks;
-    if (len < 16) len = 16;
-    return len > nz ? nz : len;
-}
-
-template <typename K>
-hipError_t allow_lds(K kernel, size_t lds)
-{
-    if (lds <= 65536) return hipSuccess;
-    return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-}
-
 template <typename R, int TX, int TY, int PAD>
 hipError_t stress_launch(const Shot<R> &s, int zchunk, hipStream_t st)
 {

@@ -151,3 +151,37 @@ def config_c1(seed=11, nt=12000, shape=(200, 400, 400), n_grains=3000):
     normal = rng.normal(size=3)
     axes = sample_watson(n_grains, normal / np.linalg.norm(normal), -3.0, rng)
     return shot_config(lab, axes, h, nt, lambda dt, n: host.ricker(f0, dt, n))
+
+
+def perturb_axes(axes, sigma_deg, rng):
+    """Tilt every axis by a Gaussian tangent-plane step of sigma_deg per component."""
+    a = np.asarray(axes, float)
+    a = a / np.linalg.norm(a, axis=1, keepdims=True)
+    ref = np.where(np.abs(a[:, 2:3]) < 0.9, np.array([[0.0, 0.0, 1.0]]), np.array([[1.0, 0.0, 0.0]]))
+    e1 = np.cross(a, ref)
+    e1 /= np.linalg.norm(e1, axis=1, keepdims=True)
+    e2 = np.cross(a, e1)
+    s = np.radians(sigma_deg)
+    b = a + s * (rng.normal(size=(len(a), 1)) * e1 + rng.normal(size=(len(a), 1)) * e2)
+    return b / np.linalg.norm(b, axis=1, keepdims=True)
+
+
+def config_c3(seed=7, nt=4000, shape=(80, 160, 160), n_grains=500, sigma_deg=5.0):
+    """Adjoint-gradient case: config_c0's geometry on an 80x160x160 grid
+    (0.1 mm; disk raster inside 23-cell couplant/sponge pads), 500 grains,
+    Watson kappa 5 about 30 deg, 1 MHz 3-cycle Hann, 4000 steps.  Adds
+    ``axes_true``: the axes tilted by N(0, sigma_deg), the fabric that makes
+    the target traces."""
+    h = 1e-4
+    m = 20 + 3
+    core = (shape[0] - 2 * m, shape[1] - 2 * m, shape[2] - 2 * m)
+    rng = np.random.default_rng(seed)
+    lab_core, _ = voronoi_disk(core, h, core[2] * h, core[0] * h, n_grains, rng)
+    lab = np.full(shape, -1, np.int32)
+    lab[m:m + core[0], m:m + core[1], m:m + core[2]] = lab_core
+    az = np.radians(30.0)
+    axes = sample_watson(n_grains, [np.cos(az), np.sin(az), 0.0], 5.0, rng)
+    cfg = shot_config(lab, axes, h, nt,
+                      lambda dt, n: host.hann_tone_burst(n, dt, 1e6, n_cycles=3))
+    cfg["axes_true"] = perturb_axes(axes, sigma_deg, rng)
+    return cfg

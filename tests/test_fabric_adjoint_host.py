@@ -62,17 +62,20 @@ def test_torch_table_and_autograd_chain():
 
 def _runs_brute(mats, zbreak):
     nz, ny, nx = mats.shape
-    runs, ptr = [], [0]
+    zb = zbreak if 0 < zbreak < nz else nz
+    runs, ptr = [], []
     for y in range(ny):
         for x in range(nx):
             for z in range(nz):
+                if z % zb == 0:
+                    ptr.append(len(runs))
                 if z == 0 or mats[z, y, x] != mats[z - 1, y, x] or (zbreak and z % zbreak == 0):
                     runs.append(int(mats[z, y, x]))
-            ptr.append(len(runs))
+    ptr.append(len(runs))
     return np.array(ptr), np.array(runs)
 
 
-@pytest.mark.parametrize("zbreak", [0, 1, 4, 7])
+@pytest.mark.parametrize("zbreak", [0, 1, 4, 7, 13, 20])
 def test_run_tables_brute_force(zbreak):
     rng = np.random.default_rng(zbreak)
     mats = rng.integers(0, 4, size=(13, 5, 6)).astype(np.int32)
