@@ -91,7 +91,7 @@ def main():
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
     ap.add_argument("--config", default=None, choices=["c1", "c0", "c3"],
                     help="workload (default c1 forward; c3 for --mode adjoint)")
-    ap.add_argument("--mode", default="forward", choices=["forward", "adjoint"],
+    ap.add_argument("--mode", default="forward", choices=["forward", "adjoint", "sweep"],
                     help="forward: single-shot forward run; adjoint: misfit gradient w.r.t. "
                          "per-grain c-axis angles (forward + checkpointed reverse pass)")
     ap.add_argument("--checkpoint-every", type=int, default=200)
@@ -119,6 +119,8 @@ def main():
 
     if args.mode == "adjoint":
         return bench_adjoint(args)
+    if args.mode == "sweep":
+        return bench_sweep(args)
     if args.config is None:
         args.config = "c1"
     t_setup = time.perf_counter()
@@ -237,6 +239,133 @@ def main():
         log("CPU baseline %.4f Gcell-updates/s" % out["cpu_baseline"]["value"])
     else:
         out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+def bench_sweep(args):
+    """Batched azimuth sweep against one-at-a-time solves.
+
+    Workload: the 100 mm x 35 mm, 3000-grain, girdle-fabric disk of the
+    single-shot benchmark (h = 0.285 mm) rotated under a +x rim transducer at
+    N azimuths (360/N deg apart), each a (169, 397, 397) grid, plus 8 smaller
+    seeded disks whose grids run from 128x128x64 to 256x256x128; every shot
+    runs --steps steps (2.25 MHz Ricker, 20-cell sponge, couplant mask).
+    Batched: every shot prepared on the GPU (re-grid, distance transform,
+    dt) and all shots advanced together in one multi-shot launch per
+    half-step, replayed as a hipGraph.  One at a time: the upstream sweep
+    pattern, host preparation (rotated_grid, tables, dt, EDT mask,
+    footprint) then forward_fused_labels per azimuth; --plain-sample azimuths
+    and two of the small disks are timed and the sweep total extrapolated.
+    Traces of the timed one-at-a-time shots are compared with the batch."""
+    import numpy as np
+    import torch
+
+    from pulse_echo_azimuthal_simulation_amd import autotune, host, solver, specimen, sweep
+
+    nt = max(args.steps, 1)
+    f0 = 2.25e6
+    h = specimen.C_REF / f0 / 6
+    co = host.optimised_coeffs(8, kh_max=2.0, multistart=True)
+    t0 = time.perf_counter()
+    big = sweep.disk_build(100e-3, 35e-3, h, 3000, seed=11, kappa=-3.0,
+                           fabric_axis=np.random.default_rng(11).normal(size=3))
+    smalls = []
+    for k in range(8):
+        n = 128 + 128 * k // 7                       # grid side 128 .. 256
+        nd = n - 46
+        nzd = max(n // 2 - 46, 12)
+        smalls.append(sweep.disk_build(nd * h, nzd * h, h, 60 + 40 * k, seed=100 + k,
+                                       kappa=5.0, fabric_axis=(np.cos(0.5), np.sin(0.5), 0.0)))
+    log("specimens built in %.1f s" % (time.perf_counter() - t0))
+    azs = [360.0 * k / args.azimuths for k in range(args.azimuths)]
+
+    # ---------------- batched
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    prep_big = sweep.GpuAzimuthPrep(big, h, co, f0, sponge=20)
+    shots, wavs, nts = [], [], []
+    for az in azs:
+        sh, wav, n_, _ = prep_big.shot(az, nt=nt)
+        shots.append(sh)
+        wavs.append(wav)
+        nts.append(n_)
+    for b in smalls:
+        sh, wav, n_, _ = sweep.GpuAzimuthPrep(b, h, co, f0, sponge=20).shot(0.0, nt=nt)
+        shots.append(sh)
+        wavs.append(wav)
+        nts.append(n_)
+    torch.cuda.synchronize()
+    t_prep = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    best, _ = autotune.tune(shots[0], reps=3)
+    for sh in shots:
+        for hh in (0, 1):
+            sh.desc.tile[hh] = best[hh][0]
+            sh.desc.zchunk[hh] = best[hh][1] if sh.shape == shots[0].shape else 0
+        for f in sh.v + sh.s:
+            f.zero_()
+    batch = sweep.ShotBatch(shots, wavs, nts)
+    torch.cuda.synchronize()
+    t_tune = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    traces = batch.run(graph_steps=50)
+    torch.cuda.synchronize()
+    t_run = time.perf_counter() - t1
+    t_batched = time.perf_counter() - t0
+    cells = sum(int(np.prod(sh.shape)) * n_ for sh, n_ in zip(shots, nts))
+    log("batched: %d shots, prep %.2f s, tune %.2f s, run %.2f s (%.2f Gcell-updates/s), "
+        "total %.2f s" % (len(shots), t_prep, t_tune, t_run, cells / t_run / 1e9, t_batched))
+    tr_batch = [t.cpu().numpy() for t in traces]
+    del batch, shots, wavs, traces
+    torch.cuda.empty_cache()
+
+    # ---------------- one at a time (sample, extrapolated)
+    plain_az, plain_small = [], []
+    match = True
+    for k in range(min(args.plain_sample, len(azs))):
+        az = azs[(k * len(azs)) // max(args.plain_sample, 1)]
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        a = sweep.azimuth_shot(big, h, az, co, f0, sponge=20, nt=nt)
+        tr = solver.forward_fused_labels(a["lab"], a["axes"], h, a["dt"], nt, a["src_pts"],
+                                         a["wavelet"], a["rec_groups"], coeffs=co,
+                                         sponge_width=20, damp_mask=a["damp_mask"],
+                                         mat_tables=a["mat_tables"])
+        plain_az.append(time.perf_counter() - ts)
+        ref = tr_batch[azs.index(az)]
+        match &= bool(np.allclose(tr, ref, rtol=0, atol=1e-6 * np.abs(ref).max()))
+    for k in (0, 7):
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        a = sweep.azimuth_shot(smalls[k], h, 0.0, co, f0, sponge=20, nt=nt)
+        solver.forward_fused_labels(a["lab"], a["axes"], h, a["dt"], nt, a["src_pts"],
+                                    a["wavelet"], a["rec_groups"], coeffs=co, sponge_width=20,
+                                    damp_mask=a["damp_mask"], mat_tables=a["mat_tables"])
+        plain_small.append(time.perf_counter() - ts)
+    t_plain = float(np.mean(plain_az)) * len(azs) + float(np.mean(plain_small)) * len(smalls)
+    log("one at a time: %.2f s per azimuth, %.2f s per small disk (sampled) -> %.1f s; "
+        "traces match batch: %s" % (np.mean(plain_az), np.mean(plain_small), t_plain, match))
+    out = {
+        "metric": METRIC,
+        "value": round(t_batched, 3),
+        "unit": "s (whole sweep: %d azimuths + 8 disks, %d steps each)" % (len(azs), nt),
+        "n_gpus": 1, "steps": nt, "warmup": 0,
+        "ms_per_step": round(t_run / nt * 1e3, 4),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batched azimuth sweep: 100x35 mm 3000-grain disk at %d azimuths "
+                               "(169x397x397 each) + 8 disks 128x128x64..256x256x128" % len(azs),
+                   "shots": len(azs) + len(smalls), "mode": "sweep",
+                   "launch": {"stress": best[0][:2], "velocity": best[1][:2]}},
+        "batched_s": {"prep": round(t_prep, 3), "tune": round(t_tune, 3),
+                      "run": round(t_run, 3), "total": round(t_batched, 3)},
+        "batched_gcells": round(cells / t_run / 1e9, 3),
+        "one_at_a_time_s_estimate": round(t_plain, 2),
+        "one_at_a_time_sample_s": {"azimuth": [round(x, 3) for x in plain_az],
+                                   "small_disk": [round(x, 3) for x in plain_small]},
+        "speedup_vs_one_at_a_time": round(t_plain / t_batched, 3),
+        "traces_match": match,
+    }
     print(json.dumps(out), flush=True)
 
 

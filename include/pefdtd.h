@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PE_ABI_VERSION 3
+#define PE_ABI_VERSION 5
 
 enum { PE_OK = 0, PE_EINVAL = -1, PE_EHIP = -2, PE_ERANGE = -3 };
 enum { PE_F32 = 0, PE_F64 = 1 };
@@ -182,6 +182,79 @@ int pe_adjoint_run(const pe_shot *shot, const pe_adjoint *adj, int32_t n_hi, int
  * gradient maps g[C_IJ] of vendor/ringfwi/elastic3d.py:412-419 summed over
  * each grain. */
 int pe_adjoint_gradient(const pe_shot *shot, const pe_adjoint *adj, double *grad, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Multi-shot batches.  The upstream sweep engine solves azimuths one after
+ * another, each a separate forward_fused_labels call
+ * (sim/pipeline/sweep_runner.py:322-388).  A batch advances many shots (of
+ * any grid sizes, dt and step counts) together: every batched step is one
+ * launch per half-step over the workgroups of all shots, plus two small
+ * per-shot launches (source / pressure record; velocity record and step
+ * advance).  Each shot keeps its own step counter on the device, so the
+ * launches take no per-step host arguments and a captured graph of k steps
+ * can be replayed until every shot is done (finished shots exit at once).
+ * All shots share one precision and the tile variants of shots[0]; every
+ * shot must be PE_KERNEL_TILED.  Results are bit-identical to pe_run of
+ * each shot alone.
+ * ------------------------------------------------------------------------ */
+typedef struct pe_batch {
+    int32_t n_shot;
+    const pe_shot *shots;            /* host array of n_shot descriptors */
+    const double *const *wavelet;    /* host array: device wavelet of each shot (>= nt) */
+    void *const *traces;             /* host array: device (nt x n_rec) real traces, or NULL */
+    const int32_t *nt;               /* host array: step count of each shot */
+    void *table;                     /* device scratch of pe_batch_table_bytes() bytes */
+    int32_t *step;                   /* device, n_shot counters: next step of each shot */
+    int64_t nb[2];                   /* set by pe_batch_setup: workgroups per half-step */
+} pe_batch;
+
+/* Device scratch needed by a batch of these shots (0 on invalid input). */
+int64_t pe_batch_table_bytes(const pe_shot *shots, int32_t n_shot);
+
+/* Validate, write the device table (blocking copy) and zero the step
+ * counters.  Must precede pe_batch_step / pe_batch_graph. */
+int pe_batch_setup(pe_batch *batch, void *stream);
+
+/* Enqueue `steps` batched steps (asynchronous, graph-capturable). */
+int pe_batch_step(const pe_batch *batch, int32_t steps, void *stream);
+
+/* Capture `steps` batched steps on `stream` into an executable graph
+ * (*graph_exec, a hipGraphExec_t); replay with pe_graph_launch. */
+int pe_batch_graph(const pe_batch *batch, int32_t steps, void *stream, void **graph_exec);
+int pe_graph_launch(void *graph_exec, void *stream);
+int pe_graph_destroy(void *graph_exec);
+
+/* ------------------------------------------------------------------------
+ * Per-azimuth shot preparation on the device (the upstream does these on
+ * the host for every azimuth, sim/pipeline/sweep_runner.py:331-341).  All
+ * pointers are device pointers; results equal the host computation.
+ * ------------------------------------------------------------------------ */
+
+/* Re-grid the disk labels src (nx_s x ny_s x nz_s, x-major, -1 outside)
+ * rotated by the angle whose cosine / sine are ca / sa, into lab
+ * (nz x n x n int32, nz = ndz + 2m, n = nd + 2m, couplant -1 in the
+ * m-cell pad).  Replaces the nearest-neighbour path of
+ * sim/core/rotation_test.py:77-91, 125-132. */
+int pe_prep_rotate(const int32_t *src, int32_t nx_s, int32_t ny_s, int32_t nz_s, double h_spec,
+                   double h, double ca, double sa, int32_t nd, int32_t ndz, int32_t m,
+                   int32_t *lab, void *stream);
+
+/* damp = (pz[z] * py[y]) * px[x] * (float)exp(-fluid_damp * d), d the
+ * Euclidean distance from each couplant cell (lab < 0) to the nearest
+ * specimen cell (exact, three separable passes); s0, s1: ncell int32
+ * scratch.  Replaces sweep_runner.py:340-341 (scipy EDT) composed with the
+ * sponge of sim/core/fdtd.py:1693-1700.  Grid sides up to 1024. */
+int pe_prep_edt_damp(const int32_t *lab, int32_t nz, int32_t ny, int32_t nx, double fluid_damp,
+                     const float *pz, const float *py, const float *px, int32_t *s0, int32_t *s1,
+                     float *damp, void *stream);
+
+/* partial[b] = block maxima of max(C11, C22, C33)[m] / min(rho, face
+ * densities) over the cells (float64); the stable dt is
+ * safety * cfl * h / sqrt(max partial).  Replaces the volume pass of
+ * safe_dt_labels, sim/core/fdtd.py:1606-1615. */
+int pe_prep_vmax(const int32_t *mats, int32_t nz, int32_t ny, int32_t nx, const void *Ct,
+                 const void *rho_t, int32_t precision, double *partial, int32_t n_partial,
+                 void *stream);
 
 #ifdef __cplusplus
 }

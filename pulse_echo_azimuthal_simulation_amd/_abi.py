@@ -15,11 +15,14 @@ PE_F32, PE_F64 = 0, 1
 PE_KERNEL_DIRECT, PE_KERNEL_TILED = 0, 1
 RECORD_MODES = {"pressure": 0, "vx": 1, "vy": 2, "vz": 3}
 KERNELS = {"direct": PE_KERNEL_DIRECT, "tiled": PE_KERNEL_TILED}
-ABI_VERSION = 3
+ABI_VERSION = 5
 
 EXPORTS = ("pe_abi_version", "pe_last_error", "pe_check_shot", "pe_stress", "pe_source",
            "pe_velocity", "pe_record", "pe_run", "pe_run_timed", "pe_tile_info", "pe_run_store",
-           "pe_check_adjoint", "pe_adjoint_reset", "pe_adjoint_run", "pe_adjoint_gradient")
+           "pe_check_adjoint", "pe_adjoint_reset", "pe_adjoint_run", "pe_adjoint_gradient",
+           "pe_batch_table_bytes", "pe_batch_setup", "pe_batch_step", "pe_batch_graph",
+           "pe_graph_launch", "pe_graph_destroy", "pe_prep_rotate", "pe_prep_edt_damp",
+           "pe_prep_vmax")
 
 
 class PeShot(C.Structure):
@@ -54,6 +57,16 @@ class PeAdjoint(C.Structure):
         ("zbreak", C.c_int32), ("n_runs", C.c_int32),
         ("run_ptr", C.c_void_p), ("mat_ptr", C.c_void_p), ("mat_runs", C.c_void_p),
         ("run_acc", C.c_void_p),
+    ]
+
+
+class PeBatch(C.Structure):
+    """Mirror of struct pe_batch (include/pefdtd.h)."""
+    _fields_ = [
+        ("n_shot", C.c_int32),
+        ("shots", C.c_void_p), ("wavelet", C.c_void_p), ("traces", C.c_void_p),
+        ("nt", C.c_void_p), ("table", C.c_void_p), ("step", C.c_void_p),
+        ("nb", C.c_int64 * 2),
     ]
 
 
@@ -94,11 +107,28 @@ def load():
     lib.pe_adjoint_run.argtypes = [P, A, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
                                    C.c_void_p, C.c_void_p]
     lib.pe_adjoint_gradient.argtypes = [P, A, C.c_void_p, C.c_void_p]
+    B = C.POINTER(PeBatch)
+    lib.pe_batch_table_bytes.argtypes = [C.POINTER(PeShot), C.c_int32]
+    lib.pe_batch_table_bytes.restype = C.c_int64
+    lib.pe_batch_setup.argtypes = [B, C.c_void_p]
+    lib.pe_batch_step.argtypes = [B, C.c_int32, C.c_void_p]
+    lib.pe_batch_graph.argtypes = [B, C.c_int32, C.c_void_p, C.POINTER(C.c_void_p)]
+    lib.pe_graph_launch.argtypes = [C.c_void_p, C.c_void_p]
+    lib.pe_graph_destroy.argtypes = [C.c_void_p]
+    V = C.c_void_p
+    lib.pe_prep_rotate.argtypes = [V, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                   C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32, V, V]
+    lib.pe_prep_edt_damp.argtypes = [V, C.c_int32, C.c_int32, C.c_int32, C.c_double, V, V, V, V, V,
+                                     V, V]
+    lib.pe_prep_vmax.argtypes = [V, C.c_int32, C.c_int32, C.c_int32, V, V, C.c_int32, V, C.c_int32,
+                                 V]
     lib.pe_tile_info.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
     lib.pe_tile_info.restype = C.c_int
     for name in ("pe_check_shot", "pe_stress", "pe_velocity", "pe_source", "pe_record", "pe_run",
                  "pe_run_timed", "pe_run_store", "pe_check_adjoint", "pe_adjoint_reset",
-                 "pe_adjoint_run", "pe_adjoint_gradient"):
+                 "pe_adjoint_run", "pe_adjoint_gradient", "pe_batch_setup", "pe_batch_step",
+                 "pe_batch_graph", "pe_graph_launch", "pe_graph_destroy", "pe_prep_rotate",
+                 "pe_prep_edt_damp", "pe_prep_vmax"):
         getattr(lib, name).restype = C.c_int
     ver = lib.pe_abi_version()
     if ver != ABI_VERSION:

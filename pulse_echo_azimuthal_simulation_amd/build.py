@@ -14,7 +14,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libpefdtd.so")
 SOURCES = ["abi.hip", "kernels_direct.hip", "kernels_tiled.hip", "kernels_adjoint.hip",
-           "kernels_adjoint_tiled.hip"]
+           "kernels_adjoint_tiled.hip", "kernels_prep.hip"]
 ARCH = os.environ.get("PEFDTD_ARCH", "gfx950")
 
 # -ffp-contract=off: only the explicit fma() calls fuse, as in the oracle.

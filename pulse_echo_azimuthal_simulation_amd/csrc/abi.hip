@@ -5,6 +5,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <vector>
+
 #include "common.hpp"
 
 namespace {
@@ -133,6 +135,117 @@ int run_timed_impl(const pe_shot *p, int32_t n0, int32_t n1, const double *wavel
     for (int b = 0; b < kBatch; ++b)
         for (int k = 0; k < 4; ++k) (void)hipEventDestroy(ev[b][k]);
     return rc;
+}
+
+// Layout of a batch table: entries (256-byte aligned block), then the
+// workgroup -> shot maps of the stress and velocity launches.
+struct BatchLayout {
+    size_t entry_bytes, blk_s_off, blk_v_off, total;
+    int64_t nb[2];
+};
+
+int batch_layout(const pe_shot *shots, int32_t n, BatchLayout *L)
+{
+    if (!shots || n < 1) return fail(PE_EINVAL, "a batch needs at least one shot");
+    const int prec = shots[0].precision;
+    L->nb[0] = L->nb[1] = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        const pe_shot *p = &shots[i];
+        int rc = pe_check_shot(p);
+        if (rc) return fail(rc, "shot %d: %s", i, pe_last_error());
+        if (p->precision != prec) return fail(PE_EINVAL, "shot %d: precision differs from shot 0", i);
+        if (p->kernel != PE_KERNEL_TILED)
+            return fail(PE_EINVAL, "shot %d: batches run the tiled kernels only", i);
+        for (int h = 0; h < 2; ++h) {
+            int tx, ty, zl;
+            const int nb = pe::tiled_geometry(prec, h, shots[0].tile[h], p->zchunk[h], p->nz, p->ny,
+                                              p->nx, &tx, &ty, &zl);
+            if (nb < 0) return fail(PE_EINVAL, "shot %d: tile variant", i);
+            L->nb[h] += nb;
+        }
+    }
+    if (L->nb[0] >= ((int64_t)1 << 31) || L->nb[1] >= ((int64_t)1 << 31))
+        return fail(PE_ERANGE, "batch exceeds 2^31 workgroups");
+    const size_t esz = prec == PE_F32 ? sizeof(pe::BatchEntry<float>) : sizeof(pe::BatchEntry<double>);
+    L->entry_bytes = (esz * n + 255) / 256 * 256;
+    L->blk_s_off = L->entry_bytes;
+    L->blk_v_off = L->blk_s_off + ((size_t)L->nb[0] * 4 + 255) / 256 * 256;
+    L->total = L->blk_v_off + (size_t)L->nb[1] * 4;
+    return PE_OK;
+}
+
+template <typename R>
+int batch_setup_impl(pe_batch *b, const BatchLayout &L)
+{
+    std::vector<unsigned char> host(L.total, 0);
+    auto *ent = reinterpret_cast<pe::BatchEntry<R> *>(host.data());
+    auto *blk_s = reinterpret_cast<int32_t *>(host.data() + L.blk_s_off);
+    auto *blk_v = reinterpret_cast<int32_t *>(host.data() + L.blk_v_off);
+    int64_t off[2] = {0, 0};
+    for (int32_t i = 0; i < b->n_shot; ++i) {
+        const pe_shot *p = &b->shots[i];
+        pe::BatchEntry<R> &e = ent[i];
+        e.S = pe::make_shot<R>(p);
+        e.nt = b->nt[i];
+        for (int h = 0; h < 2; ++h) {
+            const int nb = pe::tiled_geometry(p->precision, h, b->shots[0].tile[h], p->zchunk[h],
+                                              p->nz, p->ny, p->nx, &e.tiles_x[h], &e.tiles_y[h],
+                                              &e.zc_len[h]);
+            e.blk_off[h] = (int)off[h];
+            int32_t *blk = h == 0 ? blk_s : blk_v;
+            for (int k = 0; k < nb; ++k) blk[off[h] + k] = i;
+            off[h] += nb;
+        }
+        e.n_src = p->n_src;
+        e.n_rec = p->n_rec;
+        e.mode = p->record_mode;
+        e.src_idx = p->src_idx;
+        e.src_w = (const R *)p->src_w;
+        e.wavelet = b->wavelet[i];
+        e.rec_ptr = p->rec_ptr;
+        e.rec_idx = p->rec_idx;
+        e.rec_w = (const R *)p->rec_w;
+        e.traces = b->traces ? (R *)b->traces[i] : nullptr;
+    }
+    hipError_t e = hipMemcpy(b->table, host.data(), L.total, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "batch table upload");
+    std::vector<int32_t> zeros(b->n_shot, 0);
+    e = hipMemcpy(b->step, zeros.data(), sizeof(int32_t) * b->n_shot, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "batch step reset");
+    b->nb[0] = L.nb[0];
+    b->nb[1] = L.nb[1];
+    return PE_OK;
+}
+
+template <typename R>
+int batch_step_impl(const pe_batch *b, const BatchLayout &L, int32_t steps, hipStream_t st)
+{
+    const auto *tab = reinterpret_cast<const pe::BatchEntry<R> *>(b->table);
+    const auto *blk_s = reinterpret_cast<const int32_t *>((const char *)b->table + L.blk_s_off);
+    const auto *blk_v = reinterpret_cast<const int32_t *>((const char *)b->table + L.blk_v_off);
+    const int vs = b->shots[0].tile[0], vv = b->shots[0].tile[1];
+    for (int32_t k = 0; k < steps; ++k) {
+        hipError_t e = pe::launch_stress_batched<R>(vs, tab, blk_s, (int)L.nb[0], b->step, st);
+        if (e == hipSuccess) e = pe::launch_source_record_batched<R>(tab, b->n_shot, b->step, 0, st);
+        if (e == hipSuccess) e = pe::launch_velocity_batched<R>(vv, tab, blk_v, (int)L.nb[1], b->step, st);
+        if (e == hipSuccess) e = pe::launch_source_record_batched<R>(tab, b->n_shot, b->step, 1, st);
+        if (e != hipSuccess) return hip_fail(e, "batched step launch");
+    }
+    return PE_OK;
+}
+
+int batch_check(const pe_batch *b, BatchLayout *L)
+{
+    if (!b) return fail(PE_EINVAL, "batch is NULL");
+    int rc = batch_layout(b->shots, b->n_shot, L);
+    if (rc) return rc;
+    if (!b->wavelet || !b->nt || !b->table || !b->step)
+        return fail(PE_EINVAL, "batch needs wavelet, nt, table and step");
+    for (int32_t i = 0; i < b->n_shot; ++i) {
+        if (b->nt[i] < 0) return fail(PE_EINVAL, "shot %d: negative step count", i);
+        if (!b->wavelet[i] && b->shots[i].n_src > 0) return fail(PE_EINVAL, "shot %d: wavelet is NULL", i);
+    }
+    return PE_OK;
 }
 
 }  // namespace
@@ -340,6 +453,117 @@ int pe_adjoint_gradient(const pe_shot *p, const pe_adjoint *a, double *grad, voi
     if (!grad) return fail(PE_EINVAL, "grad is NULL");
     hipError_t e = pe::adjoint_finalize(p, a, grad, (hipStream_t)stream);
     return e == hipSuccess ? PE_OK : hip_fail(e, "adjoint gradient launch");
+}
+
+int pe_prep_rotate(const int32_t *src, int32_t nx_s, int32_t ny_s, int32_t nz_s, double h_spec,
+                   double h, double ca, double sa, int32_t nd, int32_t ndz, int32_t m,
+                   int32_t *lab, void *stream)
+{
+    if (!src || !lab) return fail(PE_EINVAL, "src and lab are required");
+    if (nx_s < 1 || ny_s < 1 || nz_s < 1 || nd < 1 || ndz < 1 || m < 0 || h <= 0 || h_spec <= 0)
+        return fail(PE_EINVAL, "bad re-grid geometry");
+    const int64_t n = (int64_t)(ndz + 2 * m) * (nd + 2 * m) * (nd + 2 * m);
+    if (n >= ((int64_t)1 << 31)) return fail(PE_ERANGE, "re-grid exceeds 2^31-1 cells");
+    hipError_t e = pe::prep_rotate(src, nx_s, ny_s, nz_s, h_spec, h, ca, sa, nd, ndz, m, lab,
+                                   (hipStream_t)stream);
+    return e == hipSuccess ? PE_OK : hip_fail(e, "re-grid launch");
+}
+
+int pe_prep_edt_damp(const int32_t *lab, int32_t nz, int32_t ny, int32_t nx, double fluid_damp,
+                     const float *pz, const float *py, const float *px, int32_t *s0, int32_t *s1,
+                     float *damp, void *stream)
+{
+    if (!lab || !pz || !py || !px || !s0 || !s1 || !damp)
+        return fail(PE_EINVAL, "every array is required");
+    if (nz < 1 || ny < 1 || nx < 1 || nz > 1024 || ny > 1024 || nx > 1024)
+        return fail(PE_ERANGE, "grid sides must lie in [1, 1024]");
+    hipError_t e = pe::prep_edt_damp(lab, nz, ny, nx, fluid_damp, pz, py, px, s0, s1, damp,
+                                     (hipStream_t)stream);
+    return e == hipSuccess ? PE_OK : hip_fail(e, "distance-transform launch");
+}
+
+int pe_prep_vmax(const int32_t *mats, int32_t nz, int32_t ny, int32_t nx, const void *Ct,
+                 const void *rho_t, int32_t precision, double *partial, int32_t n_partial,
+                 void *stream)
+{
+    if (!mats || !Ct || !rho_t || !partial || n_partial < 1)
+        return fail(PE_EINVAL, "every array is required");
+    if (nz < 1 || ny < 1 || nx < 1) return fail(PE_EINVAL, "bad grid");
+    hipError_t e = precision == PE_F32
+                       ? pe::prep_vmax<float>(mats, nz, ny, nx, (const float *)Ct,
+                                              (const float *)rho_t, partial, n_partial,
+                                              (hipStream_t)stream)
+                       : pe::prep_vmax<double>(mats, nz, ny, nx, (const double *)Ct,
+                                               (const double *)rho_t, partial, n_partial,
+                                               (hipStream_t)stream);
+    return e == hipSuccess ? PE_OK : hip_fail(e, "vmax launch");
+}
+
+int64_t pe_batch_table_bytes(const pe_shot *shots, int32_t n_shot)
+{
+    BatchLayout L;
+    if (batch_layout(shots, n_shot, &L)) return 0;
+    return (int64_t)L.total;
+}
+
+int pe_batch_setup(pe_batch *b, void *stream)
+{
+    (void)stream;
+    BatchLayout L;
+    int rc = batch_check(b, &L);
+    if (rc) return rc;
+    return b->shots[0].precision == PE_F32 ? batch_setup_impl<float>(b, L)
+                                           : batch_setup_impl<double>(b, L);
+}
+
+int pe_batch_step(const pe_batch *b, int32_t steps, void *stream)
+{
+    BatchLayout L;
+    int rc = batch_check(b, &L);
+    if (rc) return rc;
+    if (b->nb[0] != L.nb[0] || b->nb[1] != L.nb[1])
+        return fail(PE_EINVAL, "batch changed since pe_batch_setup");
+    if (steps < 0) return fail(PE_EINVAL, "negative step count");
+    return b->shots[0].precision == PE_F32
+               ? batch_step_impl<float>(b, L, steps, (hipStream_t)stream)
+               : batch_step_impl<double>(b, L, steps, (hipStream_t)stream);
+}
+
+int pe_batch_graph(const pe_batch *b, int32_t steps, void *stream, void **graph_exec)
+{
+    if (!graph_exec) return fail(PE_EINVAL, "graph_exec is NULL");
+    hipStream_t st = (hipStream_t)stream;
+    if (!st) return fail(PE_EINVAL, "graph capture needs a non-default stream");
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+    int rc = pe_batch_step(b, steps, stream);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(st, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    *graph_exec = (void *)ex;
+    return PE_OK;
+}
+
+int pe_graph_launch(void *graph_exec, void *stream)
+{
+    if (!graph_exec) return fail(PE_EINVAL, "graph_exec is NULL");
+    hipError_t e = hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream);
+    return e == hipSuccess ? PE_OK : hip_fail(e, "hipGraphLaunch");
+}
+
+int pe_graph_destroy(void *graph_exec)
+{
+    if (!graph_exec) return PE_OK;
+    hipError_t e = hipGraphExecDestroy((hipGraphExec_t)graph_exec);
+    return e == hipSuccess ? PE_OK : hip_fail(e, "hipGraphExecDestroy");
 }
 
 }  // extern "C"

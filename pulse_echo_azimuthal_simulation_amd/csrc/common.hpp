@@ -114,6 +114,24 @@ __device__ __forceinline__ R moved_direct(const R *__restrict__ gJ, int i, int i
     }
 }
 
+// One shot of a multi-shot batch (device table entry): the kernel view of the
+// shot, its step count, per half-step tiling ([0] stress, [1] velocity) and
+// first workgroup in the batched launch, and its source / receiver arrays.
+template <typename R>
+struct BatchEntry {
+    Shot<R> S;
+    int nt;
+    int tiles_x[2], tiles_y[2], zc_len[2], blk_off[2];
+    int n_src, n_rec, mode;
+    const int64_t *src_idx;
+    const R *src_w;
+    const double *wavelet;
+    const int32_t *rec_ptr;
+    const int64_t *rec_idx;
+    const R *rec_w;
+    R *traces;
+};
+
 // Launchers (defined in the kernel translation units).
 template <typename R> hipError_t launch_stress_direct(const Shot<R> &s, hipStream_t st);
 template <typename R> hipError_t launch_velocity_direct(const Shot<R> &s, hipStream_t st);
@@ -128,6 +146,22 @@ hipError_t launch_velocity_tiled(const Shot<R> &s, int variant, int zchunk, hipS
 // Number of tile variants of a half-step (0 stress, 1 velocity) at a
 // precision; fills (tx, ty, pad) of `variant` when it exists.
 int tile_variants(int precision, int half, int variant, int *tx, int *ty, int *pad);
+// Tiling of one half-step of a grid; returns the workgroup count (-1: no
+// such variant).
+int tiled_geometry(int precision, int half, int variant, int zchunk, int nz, int ny, int nx,
+                   int *tiles_x, int *tiles_y, int *zc_len);
+template <typename R>
+hipError_t launch_stress_batched(int variant, const BatchEntry<R> *tab, const int32_t *blk, int nb,
+                                 const int32_t *step, hipStream_t st);
+template <typename R>
+hipError_t launch_velocity_batched(int variant, const BatchEntry<R> *tab, const int32_t *blk,
+                                   int nb, const int32_t *step, hipStream_t st);
+// Source / record of every active shot (phase 0, after the stress
+// half-step) and velocity-mode record plus step advance (phase 1, after the
+// velocity half-step); one workgroup per shot.
+template <typename R>
+hipError_t launch_source_record_batched(const BatchEntry<R> *tab, int n_shot, const int32_t *step,
+                                        int phase, hipStream_t st);
 template <typename R>
 hipError_t launch_source_record(const Shot<R> &s, const pe_shot *p, const double *wavelet,
                                 int32_t n, bool do_source, R *traces, bool do_record,
@@ -181,5 +215,15 @@ hipError_t adjoint_step_tiled(const Shot<R> &S, const Shot<R> &Sv, const Adj<R> 
                               int damp_ls, hipStream_t st);
 template <typename R>
 hipError_t adjoint_scale_ls(const pe_shot *p, const pe_adjoint *a, hipStream_t st);
+
+// Shot preparation (kernels_prep.hip).
+hipError_t prep_rotate(const int32_t *src, int nx_s, int ny_s, int nz_s, double h_spec, double h,
+                       double ca, double sa, int nd, int ndz, int m, int32_t *lab, hipStream_t st);
+hipError_t prep_edt_damp(const int32_t *lab, int nz, int ny, int nx, double fluid_damp,
+                         const float *pz, const float *py, const float *px, int32_t *s0,
+                         int32_t *s1, float *damp, hipStream_t st);
+template <typename R>
+hipError_t prep_vmax(const int32_t *mats, int nz, int ny, int nx, const R *Ct, const R *rho,
+                     double *partial, int nblocks, hipStream_t st);
 
 }  // namespace pe

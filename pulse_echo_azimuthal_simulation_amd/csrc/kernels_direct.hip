@@ -178,8 +178,8 @@ __global__ __launch_bounds__(kBlock) void k_velocity_direct(Shot<R> S)
 constexpr int kSrBlock = 1024;
 
 template <typename R>
-__global__ __launch_bounds__(kSrBlock) void k_source_record(
-    Shot<R> S, int n_src, const int64_t *__restrict__ src_idx, const R *__restrict__ src_w,
+__device__ __forceinline__ void source_record_body(
+    const Shot<R> &S, int n_src, const int64_t *__restrict__ src_idx, const R *__restrict__ src_w,
     const double *__restrict__ wavelet, int n, int do_source, int n_rec,
     const int32_t *__restrict__ rec_ptr, const int64_t *__restrict__ rec_idx,
     const R *__restrict__ rec_w, int mode, R *__restrict__ traces, int do_record)
@@ -238,6 +238,46 @@ __global__ __launch_bounds__(kSrBlock) void k_source_record(
     }
 }
 
+template <typename R>
+__global__ __launch_bounds__(kSrBlock) void k_source_record(
+    Shot<R> S, int n_src, const int64_t *__restrict__ src_idx, const R *__restrict__ src_w,
+    const double *__restrict__ wavelet, int n, int do_source, int n_rec,
+    const int32_t *__restrict__ rec_ptr, const int64_t *__restrict__ rec_idx,
+    const R *__restrict__ rec_w, int mode, R *__restrict__ traces, int do_record)
+{
+    source_record_body<R>(S, n_src, src_idx, src_w, wavelet, n, do_source, n_rec, rec_ptr, rec_idx,
+                          rec_w, mode, traces, do_record);
+}
+
+// Batched: workgroup s serves shot s at its own step.  Phase 0 (after the
+// stress half-step): source, and the pressure record.  Phase 1 (after the
+// velocity half-step): velocity-mode record, then the shot's step counter
+// advances (the last kernel of a batched step to read it).
+template <typename R>
+__global__ __launch_bounds__(kSrBlock) void k_source_record_batched(
+    const BatchEntry<R> *__restrict__ tab, int32_t *__restrict__ step, int phase)
+{
+    const int s = blockIdx.x;
+    const BatchEntry<R> *e = tab + s;
+    const int n = step[s];
+    if (n >= e->nt) return;
+    const Shot<R> S = e->S;
+    const bool pres = e->mode == PE_RECORD_PRESSURE;
+    const bool rec = e->traces != nullptr && e->n_rec > 0;
+    if (phase == 0) {
+        if (e->n_src > 0 || (pres && rec))
+            source_record_body<R>(S, e->n_src, e->src_idx, e->src_w, e->wavelet, n,
+                                  e->n_src > 0, e->n_rec, e->rec_ptr, e->rec_idx, e->rec_w,
+                                  e->mode, e->traces, pres && rec);
+        return;
+    }
+    if (!pres && rec)
+        source_record_body<R>(S, 0, nullptr, nullptr, nullptr, n, 0, e->n_rec, e->rec_ptr,
+                              e->rec_idx, e->rec_w, e->mode, e->traces, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) step[s] = n + 1;
+}
+
 inline unsigned blocks_for(int ncell) { return (unsigned)((ncell + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -281,6 +321,19 @@ hipError_t launch_source_record(const Shot<R> &s, const pe_shot *p, const double
     return hipGetLastError();
 }
 
+template <typename R>
+hipError_t launch_source_record_batched(const BatchEntry<R> *tab, int n_shot, const int32_t *step,
+                                        int phase, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_source_record_batched<R>, dim3(n_shot), dim3(kSrBlock), 0, st, tab,
+                       (int32_t *)step, phase);
+    return hipGetLastError();
+}
+
+template hipError_t launch_source_record_batched<float>(const BatchEntry<float> *, int,
+                                                        const int32_t *, int, hipStream_t);
+template hipError_t launch_source_record_batched<double>(const BatchEntry<double> *, int,
+                                                         const int32_t *, int, hipStream_t);
 template hipError_t launch_stress_direct<float>(const Shot<float> &, hipStream_t);
 template hipError_t launch_stress_direct<double>(const Shot<double> &, hipStream_t);
 template hipError_t launch_strain_direct<float>(const Shot<float> &, hipStream_t);

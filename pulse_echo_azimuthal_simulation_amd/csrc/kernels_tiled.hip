@@ -27,10 +27,11 @@ namespace {
 
 using namespace tiled;
 
+// Body of the stress kernel for logical tile t (tile x fastest, then tile
+// y, then z-chunk) of shot S.
 template <typename R, int TX, int TY, int PAD>
-__global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
-                             (MinWaves<R, StressTile<R, TX, TY, PAD>::NT>::v)) void k_stress_tiled(
-    Shot<R> S, int tiles_x, int tiles_y, int zc_len)
+__device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x, int tiles_y,
+                                                  int zc_len, int t)
 {
     using T = StressTile<R, TX, TY, PAD>;
     constexpr int GX = T::GX, GY = T::GY, NG = T::NG, VX = T::VX, VY = T::VY, NV = T::NV;
@@ -41,8 +42,6 @@ __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
 
     const int nz = S.nz, ny = S.ny, nx = S.nx;
     const int sy = nx, sz = nx * ny;
-    const int nb = tiles_x * tiles_y * ((nz + zc_len - 1) / zc_len);
-    const int t = xcd_remap(blockIdx.x, nb);
     const int tx = t % tiles_x, ty = (t / tiles_x) % tiles_y, chunk = t / (tiles_x * tiles_y);
     const int x0 = tx * TX, y0 = ty * TY;
     const int z0 = chunk * zc_len, z1 = min(nz, z0 + zc_len);
@@ -280,8 +279,8 @@ __device__ __forceinline__ void accel_col(const Shot<R> &S, const R *p1, const R
 }
 
 template <typename R, int TX, int TY>
-__global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_velocity_tiled(Shot<R> S, int tiles_x, int tiles_y,
-                                                          int zc_len)
+__device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_x, int tiles_y,
+                                                    int zc_len, int t)
 {
     using T = VelTile<R, TX, TY>;
     constexpr int VX = T::VX, NV = T::NV, NT = T::NT, NF = (T::NFRAME + NT - 1) / NT;
@@ -295,8 +294,6 @@ __global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_velocity_
 
     const int nz = S.nz, ny = S.ny, nx = S.nx;
     const int sy = nx, sz = nx * ny;
-    const int nb = tiles_x * tiles_y * ((nz + zc_len - 1) / zc_len);
-    const int t = xcd_remap(blockIdx.x, nb);
     const int tx = t % tiles_x, ty = (t / tiles_x) % tiles_y, chunk = t / (tiles_x * tiles_y);
     const int x0 = tx * TX, y0 = ty * TY;
     const int z0 = chunk * zc_len, z1 = min(nz, z0 + zc_len);
@@ -452,6 +449,56 @@ __global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_velocity_
 }
 
 // ------------------------------------------------------------- launchers
+// Single-shot kernels.
+template <typename R, int TX, int TY, int PAD>
+__global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
+                             (MinWaves<R, StressTile<R, TX, TY, PAD>::NT>::v)) void k_stress_tiled(
+    Shot<R> S, int tiles_x, int tiles_y, int zc_len)
+{
+    const int nb = tiles_x * tiles_y * ((S.nz + zc_len - 1) / zc_len);
+    stress_tiled_body<R, TX, TY, PAD>(S, tiles_x, tiles_y, zc_len, xcd_remap(blockIdx.x, nb));
+}
+
+template <typename R, int TX, int TY>
+__global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_velocity_tiled(
+    Shot<R> S, int tiles_x, int tiles_y, int zc_len)
+{
+    const int nb = tiles_x * tiles_y * ((S.nz + zc_len - 1) / zc_len);
+    velocity_tiled_body<R, TX, TY>(S, tiles_x, tiles_y, zc_len, xcd_remap(blockIdx.x, nb));
+}
+
+// Multi-shot kernels: the workgroups of every shot of a batch in one launch.
+// blk_shot maps a (remapped) workgroup id to its shot; a shot whose step
+// counter has reached its step count exits at once.
+template <typename R, int TX, int TY, int PAD>
+__global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
+                             (MinWaves<R, StressTile<R, TX, TY, PAD>::NT>::v)) void k_stress_batched(
+    const BatchEntry<R> *__restrict__ tab, const int32_t *__restrict__ blk_shot, int nb_total,
+    const int32_t *__restrict__ step)
+{
+    const int g = xcd_remap(blockIdx.x, nb_total);
+    const int s = blk_shot[g];
+    const BatchEntry<R> *e = tab + s;
+    if (step[s] >= e->nt) return;
+    const Shot<R> S = e->S;
+    stress_tiled_body<R, TX, TY, PAD>(S, e->tiles_x[0], e->tiles_y[0], e->zc_len[0],
+                                      g - e->blk_off[0]);
+}
+
+template <typename R, int TX, int TY>
+__global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_velocity_batched(
+    const BatchEntry<R> *__restrict__ tab, const int32_t *__restrict__ blk_shot, int nb_total,
+    const int32_t *__restrict__ step)
+{
+    const int g = xcd_remap(blockIdx.x, nb_total);
+    const int s = blk_shot[g];
+    const BatchEntry<R> *e = tab + s;
+    if (step[s] >= e->nt) return;
+    const Shot<R> S = e->S;
+    velocity_tiled_body<R, TX, TY>(S, e->tiles_x[1], e->tiles_y[1], e->zc_len[1],
+                                   g - e->blk_off[1]);
+}
+
 template <typename R, int TX, int TY, int PAD>
 hipError_t stress_launch(const Shot<R> &s, int zchunk, hipStream_t st)
 {
@@ -477,6 +524,30 @@ hipError_t velocity_launch(const Shot<R> &s, int zchunk, hipStream_t st)
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_velocity_tiled<R, TX, TY>), dim3(nb), dim3(T::NT), T::LDS, st, s,
                        tiles_x, tiles_y, zl);
+    return hipGetLastError();
+}
+
+template <typename R, int TX, int TY, int PAD>
+hipError_t stress_batched_launch(const BatchEntry<R> *tab, const int32_t *blk, int nb,
+                                 const int32_t *step, hipStream_t st)
+{
+    using T = StressTile<R, TX, TY, PAD>;
+    hipError_t e = allow_lds(k_stress_batched<R, TX, TY, PAD>, T::LDS);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_stress_batched<R, TX, TY, PAD>), dim3(nb), dim3(T::NT), T::LDS, st, tab,
+                       blk, nb, step);
+    return hipGetLastError();
+}
+
+template <typename R, int TX, int TY>
+hipError_t velocity_batched_launch(const BatchEntry<R> *tab, const int32_t *blk, int nb,
+                                   const int32_t *step, hipStream_t st)
+{
+    using T = VelTile<R, TX, TY>;
+    hipError_t e = allow_lds(k_velocity_batched<R, TX, TY>, T::LDS);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_velocity_batched<R, TX, TY>), dim3(nb), dim3(T::NT), T::LDS, st, tab,
+                       blk, nb, step);
     return hipGetLastError();
 }
 
@@ -507,6 +578,72 @@ int tile_variants(int precision, int half, int variant, int *tx, int *ty, int *p
         *pad = t[variant].pad;
     }
     return n;
+}
+
+int tiled_geometry(int precision, int half, int variant, int zchunk, int nz, int ny, int nx,
+                   int *tiles_x, int *tiles_y, int *zc_len)
+{
+    int tx = 0, ty = 0, pad = 0;
+    const int n = tile_variants(precision, half, variant, &tx, &ty, &pad);
+    if (variant < 0 || variant >= n) return -1;
+    *tiles_x = (nx + tx - 1) / tx;
+    *tiles_y = (ny + ty - 1) / ty;
+    *zc_len = chunk_len(nz, *tiles_x * *tiles_y, zchunk);
+    return *tiles_x * *tiles_y * ((nz + *zc_len - 1) / *zc_len);
+}
+
+template <>
+hipError_t launch_stress_batched<float>(int v, const BatchEntry<float> *tab, const int32_t *blk,
+                                        int nb, const int32_t *step, hipStream_t st)
+{
+    switch (v) {
+    case 1: return stress_batched_launch<float, 62, 6, 0>(tab, blk, nb, step, st);
+    case 2: return stress_batched_launch<float, 30, 14, 24>(tab, blk, nb, step, st);
+    case 3: return stress_batched_launch<float, 32, 16, 24>(tab, blk, nb, step, st);
+    case 4: return stress_batched_launch<float, 32, 8, 24>(tab, blk, nb, step, st);
+    case 5: return stress_batched_launch<float, 16, 16, 0>(tab, blk, nb, step, st);
+    case 6: return stress_batched_launch<float, 32, 16, 0>(tab, blk, nb, step, st);
+    default: return stress_batched_launch<float, 32, 13, 24>(tab, blk, nb, step, st);
+    }
+}
+
+template <>
+hipError_t launch_stress_batched<double>(int v, const BatchEntry<double> *tab, const int32_t *blk,
+                                         int nb, const int32_t *step, hipStream_t st)
+{
+    switch (v) {
+    case 1: return stress_batched_launch<double, 32, 4, 0>(tab, blk, nb, step, st);
+    case 2: return stress_batched_launch<double, 16, 8, 0>(tab, blk, nb, step, st);
+    case 3: return stress_batched_launch<double, 32, 6, 24>(tab, blk, nb, step, st);
+    default: return stress_batched_launch<double, 32, 6, 0>(tab, blk, nb, step, st);
+    }
+}
+
+template <>
+hipError_t launch_velocity_batched<float>(int v, const BatchEntry<float> *tab, const int32_t *blk,
+                                          int nb, const int32_t *step, hipStream_t st)
+{
+    switch (v) {
+    case 1: return velocity_batched_launch<float, 32, 16>(tab, blk, nb, step, st);
+    case 2: return velocity_batched_launch<float, 32, 8>(tab, blk, nb, step, st);
+    case 3: return velocity_batched_launch<float, 64, 4>(tab, blk, nb, step, st);
+    case 4: return velocity_batched_launch<float, 128, 4>(tab, blk, nb, step, st);
+    case 5: return velocity_batched_launch<float, 32, 32>(tab, blk, nb, step, st);
+    default: return velocity_batched_launch<float, 64, 8>(tab, blk, nb, step, st);
+    }
+}
+
+template <>
+hipError_t launch_velocity_batched<double>(int v, const BatchEntry<double> *tab,
+                                           const int32_t *blk, int nb, const int32_t *step,
+                                           hipStream_t st)
+{
+    switch (v) {
+    case 1: return velocity_batched_launch<double, 64, 4>(tab, blk, nb, step, st);
+    case 2: return velocity_batched_launch<double, 32, 16>(tab, blk, nb, step, st);
+    case 3: return velocity_batched_launch<double, 32, 4>(tab, blk, nb, step, st);
+    default: return velocity_batched_launch<double, 32, 8>(tab, blk, nb, step, st);
+    }
 }
 
 template <>

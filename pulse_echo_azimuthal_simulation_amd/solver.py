@@ -32,7 +32,8 @@ class DeviceShot:
     """Device-resident state of one shot, plus its pe_shot descriptor.
 
     All buffers are torch tensors on the current HIP device; the descriptor
-    holds their addresses.  Fields start at zero.
+    holds their addresses.  Fields start at zero.  ``mats`` and ``damp`` may
+    be given as device tensors (prepared on the GPU), else host arrays.
     """
 
     def __init__(self, mats, Ct, rho_t, damp, coeffs, h, dt, src_idx, src_w,
@@ -44,7 +45,9 @@ class DeviceShot:
         self.device = dev
         self.real = torch.float32 if precision == "f32" else torch.float64
         self.np_real = np.float32 if precision == "f32" else np.float64
-        mats = np.ascontiguousarray(mats, np.int32)
+        mats_t = mats if torch.is_tensor(mats) else None
+        if mats_t is None:
+            mats = np.ascontiguousarray(mats, np.int32)
         self.shape = tuple(int(s) for s in mats.shape)
         nz, ny, nx = self.shape
         ncell = nz * ny * nx
@@ -56,12 +59,18 @@ class DeviceShot:
             raise ValueError("at most 4 stencil coefficients (order 8) are supported")
 
         def dev_real(a):
+            if torch.is_tensor(a):
+                return a.to(device=dev, dtype=self.real).contiguous()
             return torch.as_tensor(np.ascontiguousarray(a, self.np_real), device=dev)
 
-        self.mats = torch.as_tensor(mats, device=dev)
+        self.mats = (mats_t.to(device=dev, dtype=torch.int32).contiguous() if mats_t is not None
+                     else torch.as_tensor(mats, device=dev))
         self.Ct = dev_real(Ct)
         self.rho_t = dev_real(rho_t)
-        self.damp = None if damp is None else dev_real(np.asarray(damp).reshape(self.shape))
+        self.damp = None
+        if damp is not None:
+            self.damp = (dev_real(damp).reshape(self.shape) if torch.is_tensor(damp)
+                         else dev_real(np.asarray(damp).reshape(self.shape)))
         self.v = [torch.zeros(self.shape, dtype=self.real, device=dev) for _ in range(3)]
         self.s = [torch.zeros(self.shape, dtype=self.real, device=dev) for _ in range(6)]
         self.g = (torch.zeros((6,) + self.shape, dtype=self.real, device=dev)

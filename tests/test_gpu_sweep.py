@@ -1,0 +1,131 @@
+"""GPU azimuth-sweep machinery: device-side shot preparation against the
+host restatement of the upstream per-azimuth setup, and the multi-shot
+batch (pe_batch, hipGraph replay) against single-shot solves.
+
+Bars: labels, dt and footprints bit-identical; damping volume equal up to a
+last-place difference of float64 exp (at most one float32 ulp, in at most a
+handful of cells); batched traces bit-identical to each shot run alone.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_gpu():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch
+
+
+def _case(name):
+    print("\n[gpu case] %s" % name, flush=True)
+
+
+@pytest.fixture(scope="module")
+def build():
+    from pulse_echo_azimuthal_simulation_amd import sweep
+    return sweep.disk_build(8.0e-3, 3.0e-3, 2.0e-4, 40, seed=5)
+
+
+def _coeffs():
+    from pulse_echo_azimuthal_simulation_amd import host
+    return host.optimised_coeffs(8, kh_max=2.0, multistart=True)
+
+
+@pytest.mark.parametrize("az", [0.0, 23.0, 90.0, 211.5])
+def test_gpu_prep_matches_host(torch_gpu, build, az):
+    _case("sweep prep az %.1f" % az)
+    from pulse_echo_azimuthal_simulation_amd import sweep
+    co = _coeffs()
+    h = 2.0e-4
+    ref = sweep.azimuth_shot(build, h, az, co, 2.25e6, sponge=6)
+    prep = sweep.GpuAzimuthPrep(build, h, co, 2.25e6, sponge=6)
+    shot, wav, nt, info = prep.shot(az)
+    torch_gpu.cuda.synchronize()
+    assert np.array_equal(info["lab"].cpu().numpy(), ref["lab"])
+    assert info["dt"] == ref["dt"] and nt == ref["nt"]
+    assert sorted(info["pts"]) == sorted(p for p, _ in ref["src_pts"])
+    from pulse_echo_azimuthal_simulation_amd import host
+    damp_ref = host.sponge_profile(ref["lab"].shape, 6) * ref["damp_mask"]
+    damp = shot.damp.cpu().numpy()
+    diff = damp != damp_ref
+    assert diff.sum() <= 8
+    if diff.any():
+        ulp = np.abs(damp.view(np.int32)[diff] - damp_ref.view(np.int32)[diff])
+        assert ulp.max() <= 1
+
+
+def test_batch_equals_single_shots(torch_gpu, build):
+    """Three azimuths of one disk and two small media of other sizes, step
+    counts and dt, in one batch with graph replays of 7 steps: every trace
+    bit-identical to that shot's own pe_run."""
+    _case("sweep batch vs single")
+    torch = torch_gpu
+    from pulse_echo_azimuthal_simulation_amd import sweep
+    from pulse_echo_azimuthal_simulation_amd.solver import DeviceShot
+    co = _coeffs()
+    prep = sweep.GpuAzimuthPrep(build, 2.0e-4, co, 2.25e6, sponge=6)
+    items = []
+    for az, nt in ((0.0, 60), (40.0, 45), (75.0, 52)):
+        shot, wav, nt, _ = prep.shot(az, nt=nt)
+        items.append((shot, wav, nt))
+    for shape, seed, nt in (((20, 26, 31), 1, 38), ((17, 22, 19), 2, 55)):
+        inp = H.random_medium(shape, n_grain=6, seed=seed)
+        sh = DeviceShot(inp["mats"], inp["Ct"], inp["rho_t"], inp["damp"], inp["coeffs"],
+                        inp["h"], inp["dt"], inp["src"][0], inp["src"][1], *inp["rec"])
+        items.append((sh, sh.wavelet(np.resize(inp["wavelet"], nt)), nt))
+    singles = []
+    for sh, wav, nt in items:
+        tr = sh.new_traces(nt)
+        sh.run(wav, 0, nt, tr)
+        singles.append(tr.cpu().numpy())
+        for f in sh.v + sh.s:
+            f.zero_()
+    batch = sweep.ShotBatch([i[0] for i in items], [i[1] for i in items], [i[2] for i in items])
+    out = batch.run(graph_steps=7)
+    torch.cuda.synchronize()
+    for k, (tr, ref) in enumerate(zip(out, singles)):
+        assert np.abs(ref).max() > 0, k
+        assert np.array_equal(tr.cpu().numpy(), ref), k
+    # a second run from zeroed fields reproduces the traces (graph reuse)
+    for sh, _, _ in items:
+        for f in sh.v + sh.s:
+            f.zero_()
+    out2 = batch.run(graph_steps=7)
+    torch.cuda.synchronize()
+    for tr, ref in zip(out2, singles):
+        assert np.array_equal(tr.cpu().numpy(), ref)
+
+
+def test_batch_vmode_and_uneven_graph(torch_gpu):
+    """Velocity-record shots (record after the velocity half-step) batch the
+    same way; a graph of more steps than any shot needs is harmless."""
+    _case("sweep batch vx")
+    torch = torch_gpu
+    from pulse_echo_azimuthal_simulation_amd import sweep
+    from pulse_echo_azimuthal_simulation_amd.solver import DeviceShot
+    items, singles = [], []
+    for shape, seed, nt in (((18, 20, 22), 3, 30), ((21, 16, 25), 4, 41)):
+        inp = H.random_medium(shape, n_grain=5, seed=seed)
+        sh = DeviceShot(inp["mats"], inp["Ct"], inp["rho_t"], inp["damp"], inp["coeffs"],
+                        inp["h"], inp["dt"], inp["src"][0], inp["src"][1], *inp["rec"],
+                        record="vx")
+        wav = sh.wavelet(np.resize(inp["wavelet"], nt))
+        tr = sh.new_traces(nt)
+        sh.run(wav, 0, nt, tr)
+        singles.append(tr.cpu().numpy())
+        for f in sh.v + sh.s:
+            f.zero_()
+        items.append((sh, wav, nt))
+    batch = sweep.ShotBatch([i[0] for i in items], [i[1] for i in items], [i[2] for i in items])
+    out = batch.run(graph_steps=64)
+    torch.cuda.synchronize()
+    for tr, ref in zip(out, singles):
+        assert np.array_equal(tr.cpu().numpy(), ref)
