@@ -93,8 +93,13 @@ def main():
                     help="workload (default c1 forward; c3 for --mode adjoint)")
     ap.add_argument("--mode", default="forward", choices=["forward", "adjoint", "sweep"],
                     help="forward: single-shot forward run; adjoint: misfit gradient w.r.t. "
-                         "per-grain c-axis angles (forward + checkpointed reverse pass)")
+                         "per-grain c-axis angles (forward + checkpointed reverse pass); sweep: "
+                         "batched azimuth sweep of the 400x400x200 disk plus 8 smaller disks")
     ap.add_argument("--checkpoint-every", type=int, default=200)
+    ap.add_argument("--azimuths", type=int, default=72)
+    ap.add_argument("--plain-sample", type=int, default=2,
+                    help="sweep mode: azimuths timed one at a time (host prep + "
+                         "forward_fused_labels each) to estimate the one-at-a-time sweep")
     ap.add_argument("--history-gb", type=float, default=0.0,
                     help="adjoint mode: HBM budget for velocity history kept from the first "
                          "pass (segments that fit are not recomputed)")
@@ -283,9 +288,10 @@ def bench_sweep(args):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     prep_big = sweep.GpuAzimuthPrep(big, h, co, f0, sponge=20)
+    tables = sweep.rotated_tables(big["axes"], azs)
     shots, wavs, nts = [], [], []
-    for az in azs:
-        sh, wav, n_, _ = prep_big.shot(az, nt=nt)
+    for az, tab in zip(azs, tables):
+        sh, wav, n_, _ = prep_big.shot(az, nt=nt, tables=tab)
         shots.append(sh)
         wavs.append(wav)
         nts.append(n_)

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PE_ABI_VERSION 5
+#define PE_ABI_VERSION 6
 
 enum { PE_OK = 0, PE_EINVAL = -1, PE_EHIP = -2, PE_ERANGE = -3 };
 enum { PE_F32 = 0, PE_F64 = 1 };
@@ -238,6 +238,17 @@ int pe_graph_destroy(void *graph_exec);
 int pe_prep_rotate(const int32_t *src, int32_t nx_s, int32_t ny_s, int32_t nz_s, double h_spec,
                    double h, double ca, double sa, int32_t nd, int32_t ndz, int32_t m,
                    int32_t *lab, void *stream);
+
+/* Single rasterisation of the rotated disk: every solver point inside the
+ * disk of diameter D takes the seed minimising |x - s|^2 - w_s (float64,
+ * all n_seed seeds, lowest index on ties); seeds (n_seed x 3, metres,
+ * already rotated) and weights w are device arrays.  Same grid and pad as
+ * pe_prep_rotate.  Replaces the single_raster path of
+ * sim/core/rotation_test.py:93-124 and the labeller of
+ * sim/core/specimen.py:298-335. */
+int pe_prep_label_power(const double *seeds, const double *w, int32_t n_seed, double h,
+                        double D, int32_t nd, int32_t ndz, int32_t m, int32_t *lab,
+                        void *stream);
 
 /* damp = (pz[z] * py[y]) * px[x] * (float)exp(-fluid_damp * d), d the
  * Euclidean distance from each couplant cell (lab < 0) to the nearest

@@ -15,14 +15,14 @@ PE_F32, PE_F64 = 0, 1
 PE_KERNEL_DIRECT, PE_KERNEL_TILED = 0, 1
 RECORD_MODES = {"pressure": 0, "vx": 1, "vy": 2, "vz": 3}
 KERNELS = {"direct": PE_KERNEL_DIRECT, "tiled": PE_KERNEL_TILED}
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 EXPORTS = ("pe_abi_version", "pe_last_error", "pe_check_shot", "pe_stress", "pe_source",
            "pe_velocity", "pe_record", "pe_run", "pe_run_timed", "pe_tile_info", "pe_run_store",
            "pe_check_adjoint", "pe_adjoint_reset", "pe_adjoint_run", "pe_adjoint_gradient",
            "pe_batch_table_bytes", "pe_batch_setup", "pe_batch_step", "pe_batch_graph",
            "pe_graph_launch", "pe_graph_destroy", "pe_prep_rotate", "pe_prep_edt_damp",
-           "pe_prep_vmax")
+           "pe_prep_vmax", "pe_prep_label_power")
 
 
 class PeShot(C.Structure):
@@ -118,6 +118,8 @@ def load():
     V = C.c_void_p
     lib.pe_prep_rotate.argtypes = [V, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,
                                    C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32, V, V]
+    lib.pe_prep_label_power.argtypes = [V, V, C.c_int32, C.c_double, C.c_double, C.c_int32,
+                                        C.c_int32, C.c_int32, V, V]
     lib.pe_prep_edt_damp.argtypes = [V, C.c_int32, C.c_int32, C.c_int32, C.c_double, V, V, V, V, V,
                                      V, V]
     lib.pe_prep_vmax.argtypes = [V, C.c_int32, C.c_int32, C.c_int32, V, V, C.c_int32, V, C.c_int32,
@@ -128,7 +130,7 @@ def load():
                  "pe_run_timed", "pe_run_store", "pe_check_adjoint", "pe_adjoint_reset",
                  "pe_adjoint_run", "pe_adjoint_gradient", "pe_batch_setup", "pe_batch_step",
                  "pe_batch_graph", "pe_graph_launch", "pe_graph_destroy", "pe_prep_rotate",
-                 "pe_prep_edt_damp", "pe_prep_vmax"):
+                 "pe_prep_edt_damp", "pe_prep_vmax", "pe_prep_label_power"):
         getattr(lib, name).restype = C.c_int
     ver = lib.pe_abi_version()
     if ver != ABI_VERSION:

@@ -469,6 +469,18 @@ int pe_prep_rotate(const int32_t *src, int32_t nx_s, int32_t ny_s, int32_t nz_s,
     return e == hipSuccess ? PE_OK : hip_fail(e, "re-grid launch");
 }
 
+int pe_prep_label_power(const double *seeds, const double *w, int32_t n_seed, double h,
+                        double D, int32_t nd, int32_t ndz, int32_t m, int32_t *lab, void *stream)
+{
+    if (!seeds || !w || !lab || n_seed < 1) return fail(PE_EINVAL, "seeds, weights and lab are required");
+    if (nd < 1 || ndz < 1 || m < 0 || h <= 0 || D <= 0) return fail(PE_EINVAL, "bad grid");
+    const int64_t n = (int64_t)(ndz + 2 * m) * (nd + 2 * m) * (nd + 2 * m);
+    if (n >= ((int64_t)1 << 31)) return fail(PE_ERANGE, "grid exceeds 2^31-1 cells");
+    hipError_t e = pe::prep_label_power(seeds, w, n_seed, h, D, nd, ndz, m, lab,
+                                        (hipStream_t)stream);
+    return e == hipSuccess ? PE_OK : hip_fail(e, "labeller launch");
+}
+
 int pe_prep_edt_damp(const int32_t *lab, int32_t nz, int32_t ny, int32_t nx, double fluid_damp,
                      const float *pz, const float *py, const float *px, int32_t *s0, int32_t *s1,
                      float *damp, void *stream)

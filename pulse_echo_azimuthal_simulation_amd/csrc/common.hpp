@@ -219,6 +219,8 @@ hipError_t adjoint_scale_ls(const pe_shot *p, const pe_adjoint *a, hipStream_t s
 // Shot preparation (kernels_prep.hip).
 hipError_t prep_rotate(const int32_t *src, int nx_s, int ny_s, int nz_s, double h_spec, double h,
                        double ca, double sa, int nd, int ndz, int m, int32_t *lab, hipStream_t st);
+hipError_t prep_label_power(const double *seeds, const double *w, int n_seed, double h, double D,
+                            int nd, int ndz, int m, int32_t *lab, hipStream_t st);
 hipError_t prep_edt_damp(const int32_t *lab, int nz, int ny, int nx, double fluid_damp,
                          const float *pz, const float *py, const float *px, int32_t *s0,
                          int32_t *s1, float *damp, hipStream_t st);

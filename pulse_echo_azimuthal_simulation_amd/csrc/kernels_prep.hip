@@ -161,6 +161,47 @@ __global__ __launch_bounds__(kB) void k_vmax(const int32_t *__restrict__ mats, i
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// Single rasterisation: the power diagram (Laguerre; Voronoi for zero
+// weights) evaluated at the solver points.  Solver column (i, j) of plane z
+// sits at (u_i, u_j, zc_z), u_i = (i + 0.5) h - D/2, zc_z = (z - (ndz-1)/2) h;
+// inside the disk (u_i^2 + u_j^2 <= (D/2)^2) it takes the seed s minimising
+// |x - s|^2 - w_s over all seeds (float64, (dx^2 + dy^2) + dz^2, lowest
+// index on ties), outside -1.  Seeds already rotated by the caller
+// (rotation_test.py:108-124).
+__global__ __launch_bounds__(kB) void k_label_power(const double *__restrict__ seeds,
+                                                    const double *__restrict__ w, int n_seed,
+                                                    double h, double D, int nd, int ndz, int m,
+                                                    int nzg, int nxg, int32_t *__restrict__ lab)
+{
+    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    const int64_t n = (int64_t)nzg * nxg * nxg;
+    if (i >= n) return;
+    const int x = (int)(i % nxg), y = (int)((i / nxg) % nxg), z = (int)(i / ((int64_t)nxg * nxg));
+    const int ci = x - m, cj = y - m, cz = z - m;
+    int32_t out = -1;
+    if (ci >= 0 && ci < nd && cj >= 0 && cj < nd && cz >= 0 && cz < ndz) {
+        const double X = ((double)ci + 0.5) * h - D / 2.0;
+        const double Y = ((double)cj + 0.5) * h - D / 2.0;
+        const double Z = ((double)cz - (double)(ndz - 1) / 2.0) * h;
+        const double R = 0.5 * D;
+        if (X * X + Y * Y <= R * R) {
+            double best = 0.0;
+            int arg = -1;
+            for (int s = 0; s < n_seed; ++s) {
+                const double dx = X - seeds[3 * s], dy = Y - seeds[3 * s + 1],
+                             dz = Z - seeds[3 * s + 2];
+                const double d = (dx * dx + dy * dy) + dz * dz - w[s];
+                if (arg < 0 || d < best) {
+                    best = d;
+                    arg = s;
+                }
+            }
+            out = arg;
+        }
+    }
+    lab[i] = out;
+}
+
 inline unsigned nblk(int64_t n) { return (unsigned)((n + kB - 1) / kB); }
 
 }  // namespace
@@ -172,6 +213,16 @@ hipError_t prep_rotate(const int32_t *src, int nx_s, int ny_s, int nz_s, double 
     const int64_t n = (int64_t)nzg * nxg * nxg;
     hipLaunchKernelGGL(k_rotate_nn, dim3(nblk(n)), dim3(kB), 0, st, src, nx_s, ny_s, nz_s, h_spec,
                        h, ca, sa, nd, ndz, m, nzg, nxg, lab);
+    return hipGetLastError();
+}
+
+hipError_t prep_label_power(const double *seeds, const double *w, int n_seed, double h, double D,
+                            int nd, int ndz, int m, int32_t *lab, hipStream_t st)
+{
+    const int nzg = ndz + 2 * m, nxg = nd + 2 * m;
+    const int64_t n = (int64_t)nzg * nxg * nxg;
+    hipLaunchKernelGGL(k_label_power, dim3(nblk(n)), dim3(kB), 0, st, seeds, w, n_seed, h, D, nd,
+                       ndz, m, nzg, nxg, lab);
     return hipGetLastError();
 }
 

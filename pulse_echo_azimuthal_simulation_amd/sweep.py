@@ -137,18 +137,43 @@ def azimuth_shot(build, h, az, coeffs, f0, sponge=10, fluid_damp=0.02, element_d
                 m=m)
 
 
+def _tables_job(args):
+    axes, ca, sa = args
+    R = np.array([[ca, -sa, 0.0], [sa, ca, 0.0], [0.0, 0.0, 1.0]])
+    return materials.material_tables(np.asarray(axes) @ R)
+
+
+def rotated_tables(axes, azimuths, workers=None):
+    """material_tables(axes @ R(az)) for every azimuth, computed in parallel
+    worker processes (the per-grain 4-tensor rotation is the host's largest
+    per-azimuth cost); identical to computing them one by one."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    import os
+    jobs = []
+    for az in azimuths:
+        a = np.radians(az)
+        jobs.append((np.asarray(axes, float), float(np.cos(a)), float(np.sin(a))))
+    workers = workers or min(16, os.cpu_count() or 1, max(1, len(jobs)))
+    if workers <= 1 or len(jobs) <= 1:
+        return [_tables_job(j) for j in jobs]
+    with cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+        return list(ex.map(_tables_job, jobs))
+
+
 class GpuAzimuthPrep:
     """``azimuth_shot`` computed on the device: the build labels are uploaded
     once; per azimuth the re-grid, couplant distance transform, damping
     volume and stable-dt reduction run as kernels (pe_prep_*), and only the
     rotated stiffness table (O(grains)), the wavelet and the rim footprint
     (one plane of labels) are formed on the host.  Gives the same labels,
-    dt, footprint and damping volume as ``azimuth_shot`` with the
-    nearest-neighbour re-grid.
+    dt, footprint and damping volume as ``azimuth_shot`` (nearest-neighbour
+    re-grid, or with ``single_raster`` the power diagram evaluated at the
+    rotated solver points).
     """
 
     def __init__(self, build, h, coeffs, f0, sponge=10, fluid_damp=0.02, element_d_mm=6.35,
-                 record_factor=2.7, device=None):
+                 record_factor=2.7, single_raster=False, device=None):
         import torch
         self.torch = torch
         self.dev = torch.device("cuda", torch.cuda.current_device() if device is None else device)
@@ -159,6 +184,10 @@ class GpuAzimuthPrep:
         lab = np.ascontiguousarray(build["labels"], np.int32)
         self.src_shape = lab.shape
         self.src = torch.as_tensor(lab, device=self.dev)
+        self.single_raster = bool(single_raster)
+        if self.single_raster:
+            self.w_dev = torch.as_tensor(np.ascontiguousarray(
+                build.get("weights", np.zeros(len(build["seeds"]))), np.float64), device=self.dev)
         self.lib = _abi.load()
         self._scratch = None
 
@@ -181,10 +210,20 @@ class GpuAzimuthPrep:
         nd, ndz, m, ca, sa = self.geometry(az)
         n = nd + 2 * m
         lab = torch.empty((ndz + 2 * m, n, n), dtype=torch.int32, device=self.dev)
-        _abi.check(self.lib.pe_prep_rotate(self.src.data_ptr(), nx_s, ny_s, nz_s,
-                                           float(self.build["h"]), self.h, ca, sa, nd, ndz, m,
-                                           lab.data_ptr(), self._stream()), "pe_prep_rotate")
         R = np.array([[ca, -sa, 0.0], [sa, ca, 0.0], [0.0, 0.0, 1.0]])
+        if self.single_raster:
+            sd = torch.as_tensor(np.ascontiguousarray(np.asarray(self.build["seeds"], float) @ R),
+                                 device=self.dev)
+            D = nx_s * float(self.build["h"])
+            _abi.check(self.lib.pe_prep_label_power(sd.data_ptr(), self.w_dev.data_ptr(),
+                                                    int(sd.shape[0]), self.h, D, nd, ndz, m,
+                                                    lab.data_ptr(), self._stream()),
+                       "pe_prep_label_power")
+        else:
+            _abi.check(self.lib.pe_prep_rotate(self.src.data_ptr(), nx_s, ny_s, nz_s,
+                                               float(self.build["h"]), self.h, ca, sa, nd, ndz,
+                                               m, lab.data_ptr(), self._stream()),
+                       "pe_prep_rotate")
         return lab, nd, m, np.asarray(self.build["axes"]) @ R
 
     def damping(self, lab):
@@ -218,11 +257,12 @@ class GpuAzimuthPrep:
         vmax = float(np.sqrt(float(part.max())))
         return 0.5 * host.cfl_limit(self.coeffs) * self.h / vmax
 
-    def shot(self, az, precision="f32", nt=None, kernel="tiled"):
-        """(DeviceShot, device wavelet, nt, info dict) of azimuth ``az``."""
+    def shot(self, az, precision="f32", nt=None, kernel="tiled", tables=None):
+        """(DeviceShot, device wavelet, nt, info dict) of azimuth ``az``;
+        ``tables``: its (Ct, rho_t) if already computed (rotated_tables)."""
         from .solver import DeviceShot, _csr
         lab, nd, m, axes_rot = self.labels(az)
-        Ct, rho_t = materials.material_tables(axes_rot)
+        Ct, rho_t = materials.material_tables(axes_rot) if tables is None else tables
         mats = lab + 1
         dt = self.stable_dt(mats, Ct, rho_t)
         if nt is None:

@@ -73,3 +73,13 @@ def test_rim_footprint_brute_force(build):
     ref = sorted((z, y, ixp) for z in range(nz) for y in range(ny)
                  if (z - cz) ** 2 + (y - cy) ** 2 <= er * er and lab[z, y, ixp] >= 0)
     assert sorted(pts) == ref and w == pytest.approx(1.0 / len(ref))
+
+
+def test_rotated_tables_equal_serial(build):
+    from pulse_echo_azimuthal_simulation_amd import materials
+    azs = [0.0, 10.0, 95.0]
+    par = sweep.rotated_tables(build["axes"], azs, workers=2)
+    for az, (Ct, rho) in zip(azs, par):
+        _, _, _, axes = sweep.rotated_grid(build, build["h"], 4, az)
+        C2, r2 = materials.material_tables(axes)
+        assert np.array_equal(Ct, C2) and np.array_equal(rho, r2)
