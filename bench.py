@@ -96,6 +96,8 @@ def main():
                          "per-grain c-axis angles (forward + checkpointed reverse pass); sweep: "
                          "batched azimuth sweep of the 400x400x200 disk plus 8 smaller disks")
     ap.add_argument("--checkpoint-every", type=int, default=200)
+    ap.add_argument("--hist", default="auto", choices=["auto", "strain", "velocity"],
+                    help="adjoint mode: per-step history of the reverse pass")
     ap.add_argument("--azimuths", type=int, default=72)
     ap.add_argument("--plain-sample", type=int, default=2,
                     help="sweep mode: azimuths timed one at a time (host prep + "
@@ -418,7 +420,9 @@ def bench_adjoint(args):
         shot.desc.tile[h], shot.desc.zchunk[h] = best[h][0], best[h][1]
     del target
     proj = fabric.angle_projections(theta, phi)
-    adj = adjoint.DeviceAdjoint(shot, n_proj=2, proj=proj)
+    hist = args.hist if args.hist != "auto" else adjoint.choose_history(shot,
+                                                                        args.checkpoint_every)
+    adj = adjoint.DeviceAdjoint(shot, n_proj=2, proj=proj, hist=hist)
     nz, ny, nx = shot.shape
     ncell = nz * ny * nx
     log("adjoint setup %.1f s: grid %s, %d grains, %d steps, dt %.4g"
@@ -457,9 +461,9 @@ def bench_adjoint(args):
         "dtype": args.precision, "data": "synthetic",
         "config": {"workload": "misfit gradient w.r.t. 500 per-grain (theta, phi), 80x160x160 "
                                "grid, 1 MHz Hann, target traces from axes tilted N(0, 5 deg), "
-                               "checkpoints every %d steps, history budget %.1f GiB"
-                               % (args.checkpoint_every, args.history_gb),
-                   "grid_zyx": [nz, ny, nx], "cells": ncell, "mode": "fwd+adj"},
+                               "checkpoints every %d steps, %s history, first-pass history "
+                               "budget %.1f GiB" % (args.checkpoint_every, hist, args.history_gb),
+                   "grid_zyx": [nz, ny, nx], "cells": ncell, "mode": "fwd+adj", "history": hist},
         "forward_s": round(t_fwd, 4), "gradient_s": round(t_grad, 4),
         "cost_ratio_vs_forward": round(t_grad / t_fwd, 3),
         "forward_gcells": round(ncell * nt / t_fwd / 1e9, 3),

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PE_ABI_VERSION 6
+#define PE_ABI_VERSION 7
 
 enum { PE_OK = 0, PE_EINVAL = -1, PE_EHIP = -2, PE_ERANGE = -3 };
 enum { PE_F32 = 0, PE_F64 = 1 };
@@ -39,6 +39,8 @@ enum { PE_RECORD_PRESSURE = 0, PE_RECORD_VX = 1, PE_RECORD_VY = 2, PE_RECORD_VZ 
  *  PE_KERNEL_TILED:  z-marching LDS-tiled kernels, strains computed on the
  *                    fly inside the stress kernel (no g). */
 enum { PE_KERNEL_DIRECT = 0, PE_KERNEL_TILED = 1 };
+/* Per-step history the adjoint's stiffness gradient reads. */
+enum { PE_HIST_VELOCITY = 0, PE_HIST_STRAIN = 1 };
 
 /* One shot: medium, fields, source and receivers.  Replaces the device state
  * forward_fused_labels builds at sim/core/fdtd.py:1678-1784. */
@@ -110,6 +112,14 @@ int pe_run(const pe_shot *shot, int32_t n0, int32_t n1, const double *wavelet,
 int pe_run_store(const pe_shot *shot, int32_t n0, int32_t n1, const double *wavelet,
                  void *traces, void *vhist, void *stream);
 
+/* pe_run that also stores the strains g1..g6 of v at the start of every
+ * step n (the values its stress half-step computes, 6 x ncell per step) into
+ * ghist + (n-n0)*6*ncell: the history of PE_HIST_STRAIN adjoints, which then
+ * need not recompute them.  No upstream counterpart (the upstream keeps
+ * velocities, vendor/ringfwi/elastic3d.py:301-305). */
+int pe_run_store_strains(const pe_shot *shot, int32_t n0, int32_t n1, const double *wavelet,
+                         void *traces, void *ghist, void *stream);
+
 /* Tile variants of the tiled kernels, for tuning.  half: 0 stress, 1
  * velocity.  Returns the number of variants at that precision (>= 1) and,
  * when info3 is not NULL, writes (tile x, tile y, LDS row padding) of
@@ -149,6 +159,7 @@ typedef struct pe_adjoint {
     void *work;                  /* 6*ncell real scratch */
     void *wface;                 /* 3*ncell real scratch (PE_KERNEL_TILED) */
     int32_t kernel;              /* PE_KERNEL_DIRECT or PE_KERNEL_TILED */
+    int32_t hist_kind;           /* PE_HIST_VELOCITY (3 x ncell per step) or PE_HIST_STRAIN (6) */
     int32_t n_proj;              /* 0: dJ/dCt, all 36 entries; 1..4: projections */
     const void *proj;            /* n_mat x n_proj x 36 real (n_proj > 0) */
     int32_t zbreak;              /* runs also cut at z % zbreak == 0 (0: never) */
@@ -168,8 +179,10 @@ int pe_adjoint_reset(const pe_shot *shot, const pe_adjoint *adj, void *stream);
 
 /* Reverse steps n = n_hi-1 down to n_lo.  On entry (lv, ls) are the
  * adjoints of the state after step n_hi-1; on return, of the state before
- * step n_lo.  vhist: (n_hi-n_lo) x 3 x ncell real, slot n-n_lo = v at the
- * start of step n (read only when run_acc is set).  residual: (nt x n_rec)
+ * step n_lo.  vhist: (n_hi-n_lo) slots, slot n-n_lo = v at the start of step n
+ * (3 x ncell real, PE_HIST_VELOCITY) or its strains (6 x ncell,
+ * PE_HIST_STRAIN, as written by pe_run_store_strains); read only when
+ * run_acc is set.  residual: (nt x n_rec)
  * real trace adjoint dJ/dy indexed by absolute step, or NULL.
  * grad_wavelet: double per absolute step, receives dJ/dwavelet[n], or NULL.
  * Receiver cells must be unique within each group.  Replaces the reverse

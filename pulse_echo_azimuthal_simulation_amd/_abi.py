@@ -13,16 +13,17 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpefdtd.s
 PE_OK, PE_EINVAL, PE_EHIP, PE_ERANGE = 0, -1, -2, -3
 PE_F32, PE_F64 = 0, 1
 PE_KERNEL_DIRECT, PE_KERNEL_TILED = 0, 1
+PE_HIST_VELOCITY, PE_HIST_STRAIN = 0, 1
 RECORD_MODES = {"pressure": 0, "vx": 1, "vy": 2, "vz": 3}
 KERNELS = {"direct": PE_KERNEL_DIRECT, "tiled": PE_KERNEL_TILED}
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 EXPORTS = ("pe_abi_version", "pe_last_error", "pe_check_shot", "pe_stress", "pe_source",
            "pe_velocity", "pe_record", "pe_run", "pe_run_timed", "pe_tile_info", "pe_run_store",
            "pe_check_adjoint", "pe_adjoint_reset", "pe_adjoint_run", "pe_adjoint_gradient",
            "pe_batch_table_bytes", "pe_batch_setup", "pe_batch_step", "pe_batch_graph",
            "pe_graph_launch", "pe_graph_destroy", "pe_prep_rotate", "pe_prep_edt_damp",
-           "pe_prep_vmax", "pe_prep_label_power")
+           "pe_prep_vmax", "pe_prep_label_power", "pe_run_store_strains")
 
 
 class PeShot(C.Structure):
@@ -52,7 +53,7 @@ class PeAdjoint(C.Structure):
     """Mirror of struct pe_adjoint (include/pefdtd.h)."""
     _fields_ = [
         ("lv", C.c_void_p * 3), ("ls", C.c_void_p * 6), ("work", C.c_void_p),
-        ("wface", C.c_void_p), ("kernel", C.c_int32),
+        ("wface", C.c_void_p), ("kernel", C.c_int32), ("hist_kind", C.c_int32),
         ("n_proj", C.c_int32), ("proj", C.c_void_p),
         ("zbreak", C.c_int32), ("n_runs", C.c_int32),
         ("run_ptr", C.c_void_p), ("mat_ptr", C.c_void_p), ("mat_runs", C.c_void_p),
@@ -102,6 +103,8 @@ def load():
     A = C.POINTER(PeAdjoint)
     lib.pe_run_store.argtypes = [P, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p]
+    lib.pe_run_store_strains.argtypes = [P, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p]
     lib.pe_check_adjoint.argtypes = [P, A]
     lib.pe_adjoint_reset.argtypes = [P, A, C.c_void_p]
     lib.pe_adjoint_run.argtypes = [P, A, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
@@ -130,7 +133,8 @@ def load():
                  "pe_run_timed", "pe_run_store", "pe_check_adjoint", "pe_adjoint_reset",
                  "pe_adjoint_run", "pe_adjoint_gradient", "pe_batch_setup", "pe_batch_step",
                  "pe_batch_graph", "pe_graph_launch", "pe_graph_destroy", "pe_prep_rotate",
-                 "pe_prep_edt_damp", "pe_prep_vmax", "pe_prep_label_power"):
+                 "pe_prep_edt_damp", "pe_prep_vmax", "pe_prep_label_power",
+                 "pe_run_store_strains"):
         getattr(lib, name).restype = C.c_int
     ver = lib.pe_abi_version()
     if ver != ABI_VERSION:

@@ -10,9 +10,11 @@ by vendor/ringfwi/axisfield.py:37-161).  Here the label-indexed GPU step of
   (per-column material runs, see ``run_tables``);
 * wavefield checkpointing: the forward pass keeps the state every
   ``checkpoint_every`` steps; each segment is recomputed from its checkpoint
-  while the velocity at the start of every step is stored, then the reverse
-  steps of that segment run; segments whose velocity history fits in
-  ``history_bytes`` are stored during the first pass and not recomputed;
+  while a per-step history is stored -- the strains the stress half-step
+  computes from v (6 fields; the reverse pass then forms only their moves)
+  or v itself (3 fields, when memory is short) -- then the reverse steps of
+  that segment run; segments whose history fits in ``history_bytes`` are
+  stored during the first pass and not recomputed;
 * a ``torch.autograd.Function`` (``traces_from_table``) whose backward is that
   adjoint, so a misfit written in torch differentiates down to the stiffness
   table and, through ``fabric.stiffness_table`` (Bond rotation in torch), to
@@ -73,7 +75,8 @@ class DeviceAdjoint:
     dJ/dCt[m, k] instead (``proj`` of shape (n_mat, n_proj, 36)).
     """
 
-    def __init__(self, shot, n_proj=0, proj=None, zbreak=None, gradient=True, kernel=None):
+    def __init__(self, shot, n_proj=0, proj=None, zbreak=None, gradient=True, kernel=None,
+                 hist="velocity"):
         torch = shot.torch
         self.shot = shot
         self.torch = torch
@@ -92,6 +95,10 @@ class DeviceAdjoint:
             zbreak = 0 if kernel == "direct" else max(8, min(nz, -(-nz * tiles // (256 * 8))))
         self.kernel = kernel
         self.zbreak = int(zbreak)
+        if hist not in ("velocity", "strain"):
+            raise ValueError("hist must be 'velocity' or 'strain'")
+        self.hist = hist
+        self.hist_fields = 6 if hist == "strain" else 3
         self.lv = [torch.zeros(shot.shape, dtype=real, device=dev) for _ in range(3)]
         self.ls = [torch.zeros(shot.shape, dtype=real, device=dev) for _ in range(6)]
         self.work = torch.zeros((6,) + shot.shape, dtype=real, device=dev)
@@ -121,6 +128,7 @@ class DeviceAdjoint:
         d.work = self.work.data_ptr()
         d.wface = None if self.wface is None else self.wface.data_ptr()
         d.kernel = _abi.KERNELS[kernel]
+        d.hist_kind = _abi.PE_HIST_STRAIN if hist == "strain" else _abi.PE_HIST_VELOCITY
         d.n_proj = int(n_proj)
         d.proj = None if self.proj is None else self.proj.data_ptr()
         d.zbreak = self.zbreak
@@ -161,11 +169,13 @@ def _segments(nt, every):
     return [(a, min(a + every, nt)) for a in range(0, nt, every)]
 
 
-def forward_checkpointed(shot, wav, nt, traces, checkpoint_every=200, history=None):
+def forward_checkpointed(shot, wav, nt, traces, checkpoint_every=200, history=None,
+                         hist_fields=3):
     """Forward run from zero fields.  Returns (checkpoints, stored) where
     checkpoints[a] is the state (v, s) at the start of segment [a, b) and
-    stored[a] the (b-a, 3, nz, ny, nx) velocity history of a segment kept
-    during this pass (segments listed in ``history``)."""
+    stored[a] the (b-a, hist_fields, nz, ny, nx) history of a segment kept
+    during this pass (segments listed in ``history``): v (3 fields) or the
+    strains of v (6 fields)."""
     torch = shot.torch
     for f in shot.v + shot.s:
         f.zero_()
@@ -176,16 +186,18 @@ def forward_checkpointed(shot, wav, nt, traces, checkpoint_every=200, history=No
             ckpt[a] = torch.stack(shot.v + shot.s)
             shot.run(wav, a, b, traces)
         else:
-            h = torch.empty((b - a, 3) + shot.shape, dtype=shot.real, device=shot.device)
+            h = torch.empty((b - a, hist_fields) + shot.shape, dtype=shot.real,
+                            device=shot.device)
             _run_storing(shot, wav, a, b, traces, h)
             stored[a] = h
     return ckpt, stored
 
 
 def _run_storing(shot, wav, a, b, traces, hist):
-    _abi.check(shot.lib.pe_run_store(C.byref(shot.desc), int(a), int(b), wav.data_ptr(),
-                                     None if traces is None else traces.data_ptr(),
-                                     hist.data_ptr(), shot._stream()), "pe_run_store")
+    fn = shot.lib.pe_run_store_strains if hist.shape[1] == 6 else shot.lib.pe_run_store
+    _abi.check(fn(C.byref(shot.desc), int(a), int(b), wav.data_ptr(),
+                  None if traces is None else traces.data_ptr(), hist.data_ptr(),
+                  shot._stream()), "pe_run_store")
 
 
 def _restore(shot, state):
@@ -206,16 +218,17 @@ def reverse_pass(shot, adj, wav, nt, residual, ckpt, stored, checkpoint_every=20
             h = stored.pop(a)
         else:
             if hist is None or hist.shape[0] < b - a:
-                hist = torch.empty((b - a, 3) + shot.shape, dtype=shot.real, device=shot.device)
+                hist = torch.empty((b - a, adj.hist_fields) + shot.shape, dtype=shot.real,
+                                   device=shot.device)
             _restore(shot, ckpt[a])
             _run_storing(shot, wav, a, b, None, hist)
             h = hist
         adj.run(b, a, h, residual, grad_wavelet)
 
 
-def history_segments(nt, checkpoint_every, shape, real_bytes, budget_bytes):
-    """Start steps of the last segments whose velocity history fits the budget."""
-    per_step = 3 * int(np.prod(shape)) * real_bytes
+def history_segments(nt, checkpoint_every, shape, real_bytes, budget_bytes, fields=3):
+    """Start steps of the last segments whose history fits the budget."""
+    per_step = fields * int(np.prod(shape)) * real_bytes
     keep, used = [], 0
     for a, b in reversed(_segments(nt, checkpoint_every)):
         need = (b - a) * per_step
@@ -226,23 +239,41 @@ def history_segments(nt, checkpoint_every, shape, real_bytes, budget_bytes):
     return keep
 
 
+def choose_history(shot, checkpoint_every, budget_fraction=0.3):
+    """'strain' (6 fields per step, no strain recompute in the reverse pass)
+    when a segment of it fits in budget_fraction of the free device memory,
+    else 'velocity' (3 fields)."""
+    torch = shot.torch
+    real_bytes = 4 if shot.real == torch.float32 else 8
+    need = int(checkpoint_every) * 6 * int(np.prod(shot.shape)) * real_bytes
+    free = torch.cuda.mem_get_info(shot.device)[0]
+    return "strain" if need <= budget_fraction * free else "velocity"
+
+
 def misfit_gradient(shot, wavelet, nt, residual_fn, checkpoint_every=200, n_proj=0, proj=None,
-                    history_bytes=0, with_wavelet=False, adjoint=None):
+                    history_bytes=0, with_wavelet=False, adjoint=None, hist=None):
     """Misfit and its gradient for one DeviceShot.
 
     ``residual_fn(traces)`` receives the (nt, n_rec) device traces and returns
     (J, dJ/dtraces) with the second a device tensor of the traces' dtype.
-    Returns (J, grad (n_mat, K) float64 device tensor, traces[, dJ/dwavelet]).
+    ``hist``: 'strain' or 'velocity' history (default: ``choose_history``;
+    an explicit ``adjoint`` brings its own).  Returns (J, grad (n_mat, K)
+    float64 device tensor, traces[, dJ/dwavelet]).
     """
     torch = shot.torch
     wav = shot.wavelet(wavelet) if not torch.is_tensor(wavelet) else wavelet
     traces = shot.new_traces(nt)
+    if adjoint is None:
+        hist = hist or choose_history(shot, checkpoint_every)
+        adjoint = DeviceAdjoint(shot, n_proj=n_proj, proj=proj, hist=hist)
+    adj = adjoint
     real_bytes = 4 if shot.real == torch.float32 else 8
-    keep = history_segments(nt, checkpoint_every, shot.shape, real_bytes, history_bytes)
-    ckpt, stored = forward_checkpointed(shot, wav, nt, traces, checkpoint_every, keep)
+    keep = history_segments(nt, checkpoint_every, shot.shape, real_bytes, history_bytes,
+                            adj.hist_fields)
+    ckpt, stored = forward_checkpointed(shot, wav, nt, traces, checkpoint_every, keep,
+                                        adj.hist_fields)
     J, res = residual_fn(traces)
     res = res.to(shot.real).contiguous()
-    adj = adjoint or DeviceAdjoint(shot, n_proj=n_proj, proj=proj)
     gw = torch.zeros(nt, dtype=torch.float64, device=shot.device) if with_wavelet else None
     reverse_pass(shot, adj, wav, nt, res, ckpt, stored, checkpoint_every, gw)
     grad = adj.gradient()
@@ -261,8 +292,13 @@ def _autograd_function():
             shot.Ct.copy_(Ct.detach().reshape(shot.Ct.shape).to(shot.real))
             traces = shot.new_traces(nt)
             real_bytes = 4 if shot.real == torch.float32 else 8
-            keep = history_segments(nt, checkpoint_every, shot.shape, real_bytes, history_bytes)
-            ckpt, stored = forward_checkpointed(shot, wav, nt, traces, checkpoint_every, keep)
+            hist = choose_history(shot, checkpoint_every)
+            fields = 6 if hist == "strain" else 3
+            keep = history_segments(nt, checkpoint_every, shot.shape, real_bytes, history_bytes,
+                                    fields)
+            ckpt, stored = forward_checkpointed(shot, wav, nt, traces, checkpoint_every, keep,
+                                                fields)
+            ctx.hist = hist
             ctx.state = (shot, wav, nt, checkpoint_every, ckpt, stored)
             ctx.ct_shape, ctx.ct_dtype, ctx.ct_device = Ct.shape, Ct.dtype, Ct.device
             return traces
@@ -270,7 +306,7 @@ def _autograd_function():
         @staticmethod
         def backward(ctx, grad_traces):
             shot, wav, nt, every, ckpt, stored = ctx.state
-            adj = DeviceAdjoint(shot)
+            adj = DeviceAdjoint(shot, hist=ctx.hist)
             res = grad_traces.to(shot.real).contiguous()
             reverse_pass(shot, adj, wav, nt, res, ckpt, stored, every)
             g = adj.gradient()

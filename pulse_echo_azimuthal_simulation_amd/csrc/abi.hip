@@ -54,9 +54,12 @@ int source_record_impl(const pe_shot *p, const double *wavelet, int32_t n, bool 
     return e == hipSuccess ? PE_OK : hip_fail(e, "source/record launch");
 }
 
+// hist: NULL, or a per-step history written at the start of every step n
+// into slot n - n0: hist_kind 0 = v (3 x ncell), 1 = the strains the stress
+// half-step computes from v (6 x ncell).
 template <typename R>
 int run_impl(const pe_shot *p, int32_t n0, int32_t n1, const double *wavelet, void *traces,
-             hipStream_t st, void *vhist = nullptr)
+             hipStream_t st, void *vhist = nullptr, int hist_kind = 0)
 {
     const pe::Shot<R> s = pe::make_shot<R>(p);
     const bool direct = p->kernel == PE_KERNEL_DIRECT;
@@ -66,11 +69,19 @@ int run_impl(const pe_shot *p, int32_t n0, int32_t n1, const double *wavelet, vo
     const size_t ncell = (size_t)p->nz * p->ny * p->nx;
     for (int32_t n = n0; n < n1; ++n) {
         hipError_t e;
-        if (vhist) {
+        if (vhist && hist_kind == 0) {
             e = pe::store_velocity<R>(s, (R *)vhist + (size_t)(n - n0) * 3 * ncell, st);
             if (e != hipSuccess) return hip_fail(e, "velocity store launch");
         }
-        e = direct ? pe::launch_stress_direct<R>(s, st) : pe::launch_stress_tiled<R>(s, p->tile[0], p->zchunk[0], st);
+        if (vhist && hist_kind == 1) {
+            // the strain scratch of this step is the history slot itself
+            pe::Shot<R> sg = s;
+            sg.g = (R *)vhist + (size_t)(n - n0) * 6 * ncell;
+            e = direct ? pe::launch_stress_direct<R>(sg, st)
+                       : pe::launch_stress_tiled<R>(sg, p->tile[0], p->zchunk[0], st, true);
+        } else {
+            e = direct ? pe::launch_stress_direct<R>(s, st) : pe::launch_stress_tiled<R>(s, p->tile[0], p->zchunk[0], st);
+        }
         if (e != hipSuccess) return hip_fail(e, "stress launch");
         e = pe::launch_source_record<R>(s, p, wavelet, n, true, (R *)traces,
                                         rec_before && traces != nullptr, st);
@@ -361,6 +372,19 @@ int pe_run_store(const pe_shot *p, int32_t n0, int32_t n1, const double *wavelet
                : run_impl<double>(p, n0, n1, wavelet, traces, (hipStream_t)stream, vhist);
 }
 
+int pe_run_store_strains(const pe_shot *p, int32_t n0, int32_t n1, const double *wavelet,
+                         void *traces, void *ghist, void *stream)
+{
+    int rc = pe_check_shot(p);
+    if (rc) return rc;
+    if (n0 < 0 || n1 < n0) return fail(PE_EINVAL, "bad step range [%d, %d)", n0, n1);
+    if (!wavelet && p->n_src > 0) return fail(PE_EINVAL, "wavelet is NULL");
+    if (!ghist) return fail(PE_EINVAL, "ghist is NULL");
+    return p->precision == PE_F32
+               ? run_impl<float>(p, n0, n1, wavelet, traces, (hipStream_t)stream, ghist, 1)
+               : run_impl<double>(p, n0, n1, wavelet, traces, (hipStream_t)stream, ghist, 1);
+}
+
 int pe_tile_info(int32_t precision, int32_t half, int32_t variant, int32_t *info3)
 {
     if (precision != PE_F32 && precision != PE_F64)
@@ -408,6 +432,8 @@ int pe_check_adjoint(const pe_shot *p, const pe_adjoint *a)
         return fail(PE_EINVAL, "unknown adjoint kernel family %d", a->kernel);
     if (a->kernel == PE_KERNEL_TILED && !a->wface)
         return fail(PE_EINVAL, "the tiled adjoint needs the 3*ncell face-weight scratch");
+    if (a->hist_kind != PE_HIST_VELOCITY && a->hist_kind != PE_HIST_STRAIN)
+        return fail(PE_EINVAL, "unknown history kind %d", a->hist_kind);
     if (a->run_acc) {
         if (!a->run_ptr || !a->mat_ptr || !a->mat_runs)
             return fail(PE_EINVAL, "run_acc needs run_ptr, mat_ptr and mat_runs");

@@ -139,8 +139,10 @@ template <typename R> hipError_t launch_velocity_direct(const Shot<R> &s, hipStr
 template <typename R> hipError_t launch_strain_direct(const Shot<R> &s, hipStream_t st);
 // Tiled launchers: `variant` picks a tile shape (see tile_variants),
 // `zchunk` the z planes per workgroup (0 = automatic).
+// store_g: also write the strains of every cell into s.g (6 x ncell).
 template <typename R>
-hipError_t launch_stress_tiled(const Shot<R> &s, int variant, int zchunk, hipStream_t st);
+hipError_t launch_stress_tiled(const Shot<R> &s, int variant, int zchunk, hipStream_t st,
+                               bool store_g = false);
 template <typename R>
 hipError_t launch_velocity_tiled(const Shot<R> &s, int variant, int zchunk, hipStream_t st);
 // Number of tile variants of a half-step (0 stress, 1 velocity) at a
@@ -175,6 +177,7 @@ struct Adj {
     R *work;
     R *wface;
     const R *proj;
+    int hist_kind;
     int n_proj;
     int zbreak;
     const int32_t *run_ptr;
@@ -189,6 +192,7 @@ inline Adj<R> make_adj(const pe_adjoint *a)
     for (int k = 0; k < 6; ++k) A.ls[k] = (R *)a->ls[k];
     A.work = (R *)a->work;
     A.wface = (R *)a->wface;
+    A.hist_kind = a->hist_kind;
     A.proj = (const R *)a->proj;
     A.n_proj = a->n_proj;
     A.zbreak = a->zbreak;
@@ -212,7 +216,7 @@ template <typename R>
 hipError_t adjoint_face_weights(const pe_shot *p, const pe_adjoint *a, hipStream_t st);
 template <typename R>
 hipError_t adjoint_step_tiled(const Shot<R> &S, const Shot<R> &Sv, const Adj<R> &A, int stage,
-                              int damp_ls, hipStream_t st);
+                              int damp_ls, hipStream_t st, const R *ghist = nullptr);
 template <typename R>
 hipError_t adjoint_scale_ls(const pe_shot *p, const pe_adjoint *a, hipStream_t st);
 

@@ -253,7 +253,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_strain_t_direct(Shot<R> S, Adj<R
 // dt * ls_I * q_{VP[I],J}; NP > 0 accumulates their projections on the
 // per-material rows proj[m][p][0..35].
 template <typename R, int NP>
-__global__ __launch_bounds__(kColBlock) void k_adj_grad_direct(Shot<R> S, Adj<R> A)
+__global__ __launch_bounds__(kColBlock) void k_adj_grad_direct(Shot<R> S, Adj<R> A,
+                                                              const R *__restrict__ g)
 {
     constexpr int K = NP == 0 ? 36 : NP;
     const int nz = S.nz, ny = S.ny, nx = S.nx, sy = nx, sz = nx * ny;
@@ -261,7 +262,6 @@ __global__ __launch_bounds__(kColBlock) void k_adj_grad_direct(Shot<R> S, Adj<R>
     if (col >= sz) return;
     const int ix = col % nx, iy = col / nx;
     const size_t n = (size_t)sz * nz;
-    const R *g = A.work;
     const double dt = (double)S.dt;
     double acc[K];
 #pragma unroll
@@ -479,22 +479,29 @@ hipError_t adjoint_run(const pe_shot *p, const pe_adjoint *a, int32_t n_hi, int3
                                pres ? res : (const R *)nullptr, p->n_src, p->src_idx,
                                (const R *)p->src_w, want_gw ? gw : (double *)nullptr);
         }
+        const bool gh = a->hist_kind == PE_HIST_STRAIN;
+        const R *slot = nullptr;
         if (a->run_acc) {
-            const R *vn = (const R *)vhist + (size_t)(n - n_lo) * 3 * ncell;
-            for (int k = 0; k < 3; ++k) Sv.v[k] = (R *)(vn + (size_t)k * ncell);
+            slot = (const R *)vhist + (size_t)(n - n_lo) * (gh ? 6 : 3) * ncell;
+            if (!gh)
+                for (int k = 0; k < 3; ++k) Sv.v[k] = (R *)(slot + (size_t)k * ncell);
         }
         if (a->run_acc && tiled && a->n_proj > 0) {
-            e = adjoint_step_tiled<R>(S, Sv, A, 1, 0, st);
+            e = adjoint_step_tiled<R>(S, Sv, A, 1, 0, st, gh ? slot : nullptr);
             if (e != hipSuccess) break;
         } else if (a->run_acc) {
-            e = launch_strain_direct<R>(Sv, st);
-            if (e != hipSuccess) break;
+            const R *g = gh ? slot : A.work;
+            if (!gh) {
+                e = launch_strain_direct<R>(Sv, st);
+                if (e != hipSuccess) break;
+            }
+            const dim3 gb(blocks_of(ncol, kColBlock)), tb(kColBlock);
             switch (a->n_proj) {
-            case 0: hipLaunchKernelGGL((k_adj_grad_direct<R, 0>), dim3(blocks_of(ncol, kColBlock)), dim3(kColBlock), 0, st, S, A); break;
-            case 1: hipLaunchKernelGGL((k_adj_grad_direct<R, 1>), dim3(blocks_of(ncol, kColBlock)), dim3(kColBlock), 0, st, S, A); break;
-            case 2: hipLaunchKernelGGL((k_adj_grad_direct<R, 2>), dim3(blocks_of(ncol, kColBlock)), dim3(kColBlock), 0, st, S, A); break;
-            case 3: hipLaunchKernelGGL((k_adj_grad_direct<R, 3>), dim3(blocks_of(ncol, kColBlock)), dim3(kColBlock), 0, st, S, A); break;
-            default: hipLaunchKernelGGL((k_adj_grad_direct<R, 4>), dim3(blocks_of(ncol, kColBlock)), dim3(kColBlock), 0, st, S, A); break;
+            case 0: hipLaunchKernelGGL((k_adj_grad_direct<R, 0>), gb, tb, 0, st, S, A, g); break;
+            case 1: hipLaunchKernelGGL((k_adj_grad_direct<R, 1>), gb, tb, 0, st, S, A, g); break;
+            case 2: hipLaunchKernelGGL((k_adj_grad_direct<R, 2>), gb, tb, 0, st, S, A, g); break;
+            case 3: hipLaunchKernelGGL((k_adj_grad_direct<R, 3>), gb, tb, 0, st, S, A, g); break;
+            default: hipLaunchKernelGGL((k_adj_grad_direct<R, 4>), gb, tb, 0, st, S, A, g); break;
             }
         }
         if (tiled) {

@@ -902,6 +902,138 @@ __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
     }
 }
 
+// Stiffness gradient from a stored strain history (PE_HIST_STRAIN): the
+// strains of v^n come from memory, so only their moves are formed, from a
+// 3-plane LDS ring of the tile and its one-cell ring (the layout of the
+// stress kernel's strain ring); per-run sums as in k_adj_grad_tiled.
+template <typename R, int TX, int TY>
+struct GradGTile {
+    static constexpr int GX = TX + 2, GY = TY + 2, NG = GX * GY;
+    static constexpr int NT = (NG + 63) / 64 * 64;
+    static constexpr size_t LDS = sizeof(R) * 6 * 3 * NG;
+};
+
+template <typename R, int TX, int TY, int NP>
+__global__ __launch_bounds__((GradGTile<R, TX, TY>::NT)) void k_adj_grad_g_tiled(
+    Shot<R> S, Adj<R> A, const R *__restrict__ gh, int tiles_x, int tiles_y, int zc_len)
+{
+    using T = GradGTile<R, TX, TY>;
+    constexpr int GX = T::GX, NG = T::NG;
+    extern __shared__ unsigned char lds_raw[];
+    R *gt = reinterpret_cast<R *>(lds_raw);                       // [6][3][NG]
+
+    const int nz = S.nz, ny = S.ny, nx = S.nx;
+    const int sy = nx, sz = nx * ny;
+    const size_t ncell = (size_t)sz * nz;
+    const int nchunk = (nz + zc_len - 1) / zc_len;
+    const int nb = tiles_x * tiles_y * nchunk;
+    const int t = xcd_remap(blockIdx.x, nb);
+    const int tx = t % tiles_x, ty = (t / tiles_x) % tiles_y, chunk = t / (tiles_x * tiles_y);
+    const int x0 = tx * TX, y0 = ty * TY;
+    const int z0 = chunk * zc_len, z1 = min(nz, z0 + zc_len);
+    const bool inner_xy = x0 >= 1 && x0 + TX <= nx - 1 && y0 >= 1 && y0 + TY <= ny - 1;
+
+    const int tid = threadIdx.x;
+    const bool gact = tid < NG;
+    const int gx = gact ? tid % GX : 0, gy = gact ? tid / GX : 0;
+    const int ix = x0 - 1 + gx, iy = y0 - 1 + gy;
+    const bool core = gact && gx >= 1 && gx <= TX && gy >= 1 && gy <= TY && ix < nx && iy < ny;
+    const int col = clampi(iy, 0, ny - 1) * sy + clampi(ix, 0, nx - 1);
+    const int c = gact ? tid : NG - 1;
+    auto at = [&](int z) { return (size_t)clampi(z, 0, nz - 1) * sz + col; };
+
+    R ng[6];
+    auto load_g = [&](int z) {
+        const size_t i = at(z);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) ng[k] = gh[k * ncell + i];
+    };
+    auto put = [&](int z) {
+        const int s = (z + 3) % 3;
+        if (gact) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) gt[(k * 3 + s) * NG + c] = ng[k];
+        }
+    };
+    load_g(z0 - 1);
+    put(z0 - 1);
+    load_g(z0);
+    put(z0);
+    load_g(z0 + 1);
+    R nl[6];
+    int nm;
+    auto load_out = [&](int z) {
+        const size_t i = at(z);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) nl[k] = A.ls[k][i];
+        nm = S.mats[i];
+    };
+    load_out(z0);
+
+    int run = core ? A.run_ptr[(size_t)col * nchunk + chunk] : 0;
+    int prev = -1;
+    constexpr bool kRunSums = sizeof(R) == 4;
+    constexpr int NG36 = kRunSums ? 36 : 1;
+    R g[NG36];
+    double acc[NP];
+#pragma unroll
+    for (int k = 0; k < NG36; ++k) g[k] = 0;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) acc[p] = 0.0;
+    const double dt = (double)S.dt;
+
+    for (int z = z0; z < z1; ++z) {
+        put(z + 1);
+        load_g(z + 2);
+        R l[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) l[k] = nl[k];
+        const int m = nm;
+        load_out(z + 1);
+        lds_barrier();
+        if (core) {
+            if (prev >= 0 && m != prev) {
+                if constexpr (kRunSums) {
+                    grad_flush<NP>(A.proj, prev, dt, g, A.run_acc + (size_t)run * NP);
+                } else {
+#pragma unroll
+                    for (int p = 0; p < NP; ++p) {
+                        A.run_acc[(size_t)run * NP + p] += acc[p];
+                        acc[p] = 0.0;
+                    }
+                }
+                ++run;
+            }
+            prev = m;
+            const int sl[3] = {(z + 2) % 3, (z + 3) % 3, (z + 4) % 3};   // planes z-1, z, z+1
+            const bool fast = inner_xy && z >= 1 && z <= nz - 2;
+            if constexpr (kRunSums) {
+                if (fast)
+                    grad_cell<false, R, GX, NG>(S, gt, sl, c, z, iy, ix, l, g);
+                else
+                    grad_cell<true, R, GX, NG>(S, gt, sl, c, z, iy, ix, l, g);
+            } else {
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+                    const R *pr = A.proj + ((size_t)m * NP + p) * 36;
+                    const R tp = fast ? proj_cell<false, R, GX, NG>(S, gt, sl, c, z, iy, ix, pr, l)
+                                      : proj_cell<true, R, GX, NG>(S, gt, sl, c, z, iy, ix, pr, l);
+                    acc[p] += dt * (double)tp;
+                }
+            }
+        }
+        lds_barrier();
+    }
+    if (core && prev >= 0) {
+        if constexpr (kRunSums) {
+            grad_flush<NP>(A.proj, prev, dt, g, A.run_acc + (size_t)run * NP);
+        } else {
+#pragma unroll
+            for (int p = 0; p < NP; ++p) A.run_acc[(size_t)run * NP + p] += acc[p];
+        }
+    }
+}
+
 inline unsigned blocks_of(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 template <typename R, int TX, int TY>
@@ -960,9 +1092,24 @@ hipError_t adj_grad_launch(const Shot<R> &s, const Adj<R> &a, hipStream_t st)
     return hipGetLastError();
 }
 
-template <typename R, int NP>
-hipError_t grad_tiled(const Shot<R> &s, const Adj<R> &a, hipStream_t st)
+template <typename R, int TX, int TY, int NP>
+hipError_t adj_grad_g_launch(const Shot<R> &s, const Adj<R> &a, const R *gh, hipStream_t st)
 {
+    using T = GradGTile<R, TX, TY>;
+    const int tiles_x = (s.nx + TX - 1) / TX, tiles_y = (s.ny + TY - 1) / TY;
+    const int zl = a.zbreak > 0 ? (a.zbreak < s.nz ? a.zbreak : s.nz) : s.nz;
+    const int nb = tiles_x * tiles_y * ((s.nz + zl - 1) / zl);
+    hipError_t e = allow_lds(k_adj_grad_g_tiled<R, TX, TY, NP>, T::LDS);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_adj_grad_g_tiled<R, TX, TY, NP>), dim3(nb), dim3(T::NT), T::LDS, st, s,
+                       a, gh, tiles_x, tiles_y, zl);
+    return hipGetLastError();
+}
+
+template <typename R, int NP>
+hipError_t grad_tiled(const Shot<R> &s, const Adj<R> &a, hipStream_t st, const R *gh)
+{
+    if (gh) return adj_grad_g_launch<R, 32, 8, NP>(s, a, gh, st);
     if constexpr (sizeof(R) == 4)
         return adj_grad_launch<R, 32, 8, 24, NP>(s, a, st);
     else
@@ -985,7 +1132,7 @@ hipError_t adjoint_face_weights(const pe_shot *p, const pe_adjoint *a, hipStream
 // gradient (when accumulating), M^T C^T, G^T.
 template <typename R>
 hipError_t adjoint_step_tiled(const Shot<R> &S, const Shot<R> &Sv, const Adj<R> &A, int stage,
-                              int damp_ls, hipStream_t st)
+                              int damp_ls, hipStream_t st, const R *gh)
 {
     constexpr bool f32 = sizeof(R) == 4;
     if (stage == 0) {
@@ -994,10 +1141,10 @@ hipError_t adjoint_step_tiled(const Shot<R> &S, const Shot<R> &Sv, const Adj<R> 
     }
     if (stage == 1) {
         switch (A.n_proj) {
-        case 1: return grad_tiled<R, 1>(Sv, A, st);
-        case 2: return grad_tiled<R, 2>(Sv, A, st);
-        case 3: return grad_tiled<R, 3>(Sv, A, st);
-        default: return grad_tiled<R, 4>(Sv, A, st);
+        case 1: return grad_tiled<R, 1>(Sv, A, st, gh);
+        case 2: return grad_tiled<R, 2>(Sv, A, st, gh);
+        case 3: return grad_tiled<R, 3>(Sv, A, st, gh);
+        default: return grad_tiled<R, 4>(Sv, A, st, gh);
         }
     }
     hipError_t e = f32 ? adj_gbar_launch<R, 32, 8>(S, A, st) : adj_gbar_launch<R, 32, 4>(S, A, st);
@@ -1018,9 +1165,11 @@ hipError_t adjoint_scale_ls(const pe_shot *p, const pe_adjoint *a, hipStream_t s
 template hipError_t adjoint_face_weights<float>(const pe_shot *, const pe_adjoint *, hipStream_t);
 template hipError_t adjoint_face_weights<double>(const pe_shot *, const pe_adjoint *, hipStream_t);
 template hipError_t adjoint_step_tiled<float>(const Shot<float> &, const Shot<float> &,
-                                              const Adj<float> &, int, int, hipStream_t);
+                                              const Adj<float> &, int, int, hipStream_t,
+                                              const float *);
 template hipError_t adjoint_step_tiled<double>(const Shot<double> &, const Shot<double> &,
-                                               const Adj<double> &, int, int, hipStream_t);
+                                               const Adj<double> &, int, int, hipStream_t,
+                                               const double *);
 template hipError_t adjoint_scale_ls<float>(const pe_shot *, const pe_adjoint *, hipStream_t);
 template hipError_t adjoint_scale_ls<double>(const pe_shot *, const pe_adjoint *, hipStream_t);
 

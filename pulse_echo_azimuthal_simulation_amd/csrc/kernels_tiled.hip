@@ -28,8 +28,10 @@ namespace {
 using namespace tiled;
 
 // Body of the stress kernel for logical tile t (tile x fastest, then tile
-// y, then z-chunk) of shot S.
-template <typename R, int TX, int TY, int PAD>
+// y, then z-chunk) of shot S.  STORE_G: also write the strains g1..g6 of the
+// tile's own columns into S.g (6 x ncell, the history the adjoint's
+// stiffness gradient reads), each plane once.
+template <typename R, int TX, int TY, int PAD, bool STORE_G = false>
 __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x, int tiles_y,
                                                   int zc_len, int t)
 {
@@ -211,6 +213,13 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
             R *gs = gt + slot * NT + tid;
 #pragma unroll
             for (int k = 0; k < 6; ++k) gs[k * 3 * NT] = a[k];
+            if constexpr (STORE_G) {
+                if (core && zc >= z0 && zc < z1) {
+                    const size_t n = (size_t)nz * sz, ig = (size_t)zc * sz + col;
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) S.g[k * n + ig] = a[k];
+                }
+            }
         }
         lds_barrier();
 
@@ -450,13 +459,14 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
 
 // ------------------------------------------------------------- launchers
 // Single-shot kernels.
-template <typename R, int TX, int TY, int PAD>
+template <typename R, int TX, int TY, int PAD, bool STORE_G = false>
 __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
                              (MinWaves<R, StressTile<R, TX, TY, PAD>::NT>::v)) void k_stress_tiled(
     Shot<R> S, int tiles_x, int tiles_y, int zc_len)
 {
     const int nb = tiles_x * tiles_y * ((S.nz + zc_len - 1) / zc_len);
-    stress_tiled_body<R, TX, TY, PAD>(S, tiles_x, tiles_y, zc_len, xcd_remap(blockIdx.x, nb));
+    stress_tiled_body<R, TX, TY, PAD, STORE_G>(S, tiles_x, tiles_y, zc_len,
+                                               xcd_remap(blockIdx.x, nb));
 }
 
 template <typename R, int TX, int TY>
@@ -500,12 +510,19 @@ __global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_velocity_
 }
 
 template <typename R, int TX, int TY, int PAD>
-hipError_t stress_launch(const Shot<R> &s, int zchunk, hipStream_t st)
+hipError_t stress_launch(const Shot<R> &s, int zchunk, hipStream_t st, bool store_g = false)
 {
     using T = StressTile<R, TX, TY, PAD>;
     const int tiles_x = (s.nx + TX - 1) / TX, tiles_y = (s.ny + TY - 1) / TY;
     const int zl = chunk_len(s.nz, tiles_x * tiles_y, zchunk);
     const int nb = tiles_x * tiles_y * ((s.nz + zl - 1) / zl);
+    if (store_g) {
+        hipError_t e = allow_lds(k_stress_tiled<R, TX, TY, PAD, true>, T::LDS);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_stress_tiled<R, TX, TY, PAD, true>), dim3(nb), dim3(T::NT), T::LDS,
+                           st, s, tiles_x, tiles_y, zl);
+        return hipGetLastError();
+    }
     hipError_t e = allow_lds(k_stress_tiled<R, TX, TY, PAD>, T::LDS);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_stress_tiled<R, TX, TY, PAD>), dim3(nb), dim3(T::NT), T::LDS, st, s,
@@ -647,27 +664,29 @@ hipError_t launch_velocity_batched<double>(int v, const BatchEntry<double> *tab,
 }
 
 template <>
-hipError_t launch_stress_tiled<float>(const Shot<float> &s, int v, int zc, hipStream_t st)
+hipError_t launch_stress_tiled<float>(const Shot<float> &s, int v, int zc, hipStream_t st,
+                                      bool g)
 {
     switch (v) {
-    case 1: return stress_launch<float, 62, 6, 0>(s, zc, st);
-    case 2: return stress_launch<float, 30, 14, 24>(s, zc, st);
-    case 3: return stress_launch<float, 32, 16, 24>(s, zc, st);
-    case 4: return stress_launch<float, 32, 8, 24>(s, zc, st);
-    case 5: return stress_launch<float, 16, 16, 0>(s, zc, st);
-    case 6: return stress_launch<float, 32, 16, 0>(s, zc, st);
-    default: return stress_launch<float, 32, 13, 24>(s, zc, st);
+    case 1: return stress_launch<float, 62, 6, 0>(s, zc, st, g);
+    case 2: return stress_launch<float, 30, 14, 24>(s, zc, st, g);
+    case 3: return stress_launch<float, 32, 16, 24>(s, zc, st, g);
+    case 4: return stress_launch<float, 32, 8, 24>(s, zc, st, g);
+    case 5: return stress_launch<float, 16, 16, 0>(s, zc, st, g);
+    case 6: return stress_launch<float, 32, 16, 0>(s, zc, st, g);
+    default: return stress_launch<float, 32, 13, 24>(s, zc, st, g);
     }
 }
 
 template <>
-hipError_t launch_stress_tiled<double>(const Shot<double> &s, int v, int zc, hipStream_t st)
+hipError_t launch_stress_tiled<double>(const Shot<double> &s, int v, int zc, hipStream_t st,
+                                       bool g)
 {
     switch (v) {
-    case 1: return stress_launch<double, 32, 4, 0>(s, zc, st);
-    case 2: return stress_launch<double, 16, 8, 0>(s, zc, st);
-    case 3: return stress_launch<double, 32, 6, 24>(s, zc, st);
-    default: return stress_launch<double, 32, 6, 0>(s, zc, st);
+    case 1: return stress_launch<double, 32, 4, 0>(s, zc, st, g);
+    case 2: return stress_launch<double, 16, 8, 0>(s, zc, st, g);
+    case 3: return stress_launch<double, 32, 6, 24>(s, zc, st, g);
+    default: return stress_launch<double, 32, 6, 0>(s, zc, st, g);
     }
 }
 

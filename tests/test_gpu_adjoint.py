@@ -279,3 +279,33 @@ def test_angle_gradient_matches_central_differences(torch_gpu):
     assert scale > 0
     rel = np.abs(fd - g) / np.maximum(np.abs(g), 1e-3 * scale)
     assert rel.max() <= 1e-4, (rel.max(), fd, g)
+
+
+@pytest.mark.parametrize("precision,kernel", [("f32", "tiled"), ("f64", "tiled"), ("f32", "direct")])
+@pytest.mark.parametrize("n_proj", [0, 2])
+def test_strain_history_equals_velocity_history(torch_gpu, precision, kernel, n_proj):
+    """Storing the strains the forward computes (6 fields) instead of v (3)
+    and skipping their recompute gives bitwise the same gradient."""
+    _case("adjoint strain history %s %s %d" % (precision, kernel, n_proj))
+    from pulse_echo_azimuthal_simulation_amd import adjoint, fabric
+    inp = H.random_medium((26, 30, 70), n_grain=12, seed=21, sponge=4)
+    theta, phi = fabric.angles_from_axes(inp["axes"])
+    P = fabric.angle_projections(theta, phi) if n_proj else None
+    nt = 36
+    rng = np.random.default_rng(2)
+    out = {}
+    res_np = None
+    for hist in ("velocity", "strain"):
+        sh = _device_shot(inp, precision, kernel)
+        if res_np is None:
+            res_np = rng.normal(size=(nt, sh.n_rec))
+        res = torch_gpu.as_tensor(res_np, device=sh.device).to(sh.real)
+        adj = adjoint.DeviceAdjoint(sh, n_proj=n_proj, proj=P, hist=hist)
+        _, g, _, gw = adjoint.misfit_gradient(sh, np.resize(inp["wavelet"], nt), nt,
+                                              lambda tr: (0.0, res), checkpoint_every=10,
+                                              with_wavelet=True, adjoint=adj)
+        torch_gpu.cuda.synchronize()
+        out[hist] = (g.cpu().numpy(), gw.cpu().numpy())
+    assert np.abs(out["velocity"][0]).max() > 0
+    assert np.array_equal(out["velocity"][0], out["strain"][0])
+    assert np.array_equal(out["velocity"][1], out["strain"][1])
