@@ -5,6 +5,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "common.hpp"
@@ -235,10 +236,12 @@ int batch_step_impl(const pe_batch *b, const BatchLayout &L, int32_t steps, hipS
     const auto *blk_s = reinterpret_cast<const int32_t *>((const char *)b->table + L.blk_s_off);
     const auto *blk_v = reinterpret_cast<const int32_t *>((const char *)b->table + L.blk_v_off);
     const int vs = b->shots[0].tile[0], vv = b->shots[0].tile[1];
+    int max_mat = 1;
+    for (int32_t i = 0; i < b->n_shot; ++i) max_mat = std::max(max_mat, (int)b->shots[i].n_mat);
     for (int32_t k = 0; k < steps; ++k) {
         hipError_t e = pe::launch_stress_batched<R>(vs, tab, blk_s, (int)L.nb[0], b->step, st);
         if (e == hipSuccess) e = pe::launch_source_record_batched<R>(tab, b->n_shot, b->step, 0, st);
-        if (e == hipSuccess) e = pe::launch_velocity_batched<R>(vv, tab, blk_v, (int)L.nb[1], b->step, st);
+        if (e == hipSuccess) e = pe::launch_velocity_batched<R>(vv, tab, blk_v, (int)L.nb[1], b->step, st, max_mat);
         if (e == hipSuccess) e = pe::launch_source_record_batched<R>(tab, b->n_shot, b->step, 1, st);
         if (e != hipSuccess) return hip_fail(e, "batched step launch");
     }

@@ -20,7 +20,7 @@ constexpr int kPos[4][3] = {{0, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
 // Kernel-side view of a pe_shot, passed by value.
 template <typename R>
 struct Shot {
-    int nz, ny, nx, nc;
+    int nz, ny, nx, nc, n_mat;
     R c[4];
     R inv_h, dt, dt_h;
     uint64_t act;
@@ -41,6 +41,7 @@ inline Shot<R> make_shot(const pe_shot *p)
     s.ny = p->ny;
     s.nx = p->nx;
     s.nc = p->nc;
+    s.n_mat = p->n_mat;
     for (int j = 0; j < 4; ++j) s.c[j] = j < p->nc ? (R)p->coeffs[j] : (R)0;
     s.inv_h = (R)p->inv_h;
     s.dt = (R)p->dt;
@@ -137,6 +138,10 @@ template <typename R> hipError_t launch_stress_direct(const Shot<R> &s, hipStrea
 template <typename R> hipError_t launch_velocity_direct(const Shot<R> &s, hipStream_t st);
 // Strains of s.v into the 6*ncell scratch s.g (first kernel of the direct stress half-step).
 template <typename R> hipError_t launch_strain_direct(const Shot<R> &s, hipStream_t st);
+// Densities up to this many materials are staged in LDS by the tiled
+// velocity kernel (face-density lookups then wait on LDS only).
+constexpr int kRhoLdsBytes = 16384;
+
 // Tiled launchers: `variant` picks a tile shape (see tile_variants),
 // `zchunk` the z planes per workgroup (0 = automatic).
 // store_g: also write the strains of every cell into s.g (6 x ncell).
@@ -157,7 +162,7 @@ hipError_t launch_stress_batched(int variant, const BatchEntry<R> *tab, const in
                                  const int32_t *step, hipStream_t st);
 template <typename R>
 hipError_t launch_velocity_batched(int variant, const BatchEntry<R> *tab, const int32_t *blk,
-                                   int nb, const int32_t *step, hipStream_t st);
+                                   int nb, const int32_t *step, hipStream_t st, int max_mat);
 // Source / record of every active shot (phase 0, after the stress
 // half-step) and velocity-mode record plus step advance (phase 1, after the
 // velocity half-step); one workgroup per shot.

@@ -287,7 +287,11 @@ __device__ __forceinline__ void accel_col(const Shot<R> &S, const R *p1, const R
     }
 }
 
-template <typename R, int TX, int TY>
+// RHO_LDS: the density table sits in LDS after the five plane tiles, so the
+// face-density lookups of the output stage wait on LDS only.  A global
+// lookup there would make the wave wait for every prefetch issued before it
+// (vector memory counters retire in order).
+template <typename R, int TX, int TY, bool RHO_LDS = false>
 __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_x, int tiles_y,
                                                     int zc_len, int t)
 {
@@ -300,6 +304,12 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
     R *t4 = st + 2 * NV;
     R *t5 = st + 3 * NV;
     R *t6 = st + 4 * NV;
+    const R *rho_tab = S.rho;
+    if constexpr (RHO_LDS) {
+        R *rl = st + 5 * NV;
+        for (int k = threadIdx.x; k < S.n_mat; k += NT) rl[k] = S.rho[k];
+        rho_tab = rl;                   // ordered by the loop's first barrier
+    }
 
     const int nz = S.nz, ny = S.ny, nx = S.nx;
     const int sy = nx, sz = nx * ny;
@@ -444,7 +454,7 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
             else
                 accel_col<true, R, VX>(S, t1 + vc, t2 + vc, t4 + vc, t5 + vc, t6 + vc, r3, r4, r5,
                                        ix, iy, z, ax, ay, az);
-            const R *__restrict__ rho = S.rho;
+            const R *rho = rho_tab;
             const R r0 = rho[m0];
             const R rvx = ix < nx - 1 ? (R)0.5 * (r0 + rho[mxp]) : r0;
             const R rvy = iy < ny - 1 ? (R)0.5 * (r0 + rho[myp]) : r0;
@@ -469,12 +479,13 @@ __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
                                                xcd_remap(blockIdx.x, nb));
 }
 
-template <typename R, int TX, int TY>
+template <typename R, int TX, int TY, bool RHO_LDS = false>
 __global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_velocity_tiled(
     Shot<R> S, int tiles_x, int tiles_y, int zc_len)
 {
     const int nb = tiles_x * tiles_y * ((S.nz + zc_len - 1) / zc_len);
-    velocity_tiled_body<R, TX, TY>(S, tiles_x, tiles_y, zc_len, xcd_remap(blockIdx.x, nb));
+    velocity_tiled_body<R, TX, TY, RHO_LDS>(S, tiles_x, tiles_y, zc_len,
+                                             xcd_remap(blockIdx.x, nb));
 }
 
 // Multi-shot kernels: the workgroups of every shot of a batch in one launch.
@@ -495,7 +506,7 @@ __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
                                       g - e->blk_off[0]);
 }
 
-template <typename R, int TX, int TY>
+template <typename R, int TX, int TY, bool RHO_LDS = false>
 __global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_velocity_batched(
     const BatchEntry<R> *__restrict__ tab, const int32_t *__restrict__ blk_shot, int nb_total,
     const int32_t *__restrict__ step)
@@ -505,8 +516,8 @@ __global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_velocity_
     const BatchEntry<R> *e = tab + s;
     if (step[s] >= e->nt) return;
     const Shot<R> S = e->S;
-    velocity_tiled_body<R, TX, TY>(S, e->tiles_x[1], e->tiles_y[1], e->zc_len[1],
-                                   g - e->blk_off[1]);
+    velocity_tiled_body<R, TX, TY, RHO_LDS>(S, e->tiles_x[1], e->tiles_y[1], e->zc_len[1],
+                                            g - e->blk_off[1]);
 }
 
 template <typename R, int TX, int TY, int PAD>
@@ -537,6 +548,15 @@ hipError_t velocity_launch(const Shot<R> &s, int zchunk, hipStream_t st)
     const int tiles_x = (s.nx + TX - 1) / TX, tiles_y = (s.ny + TY - 1) / TY;
     const int zl = chunk_len(s.nz, tiles_x * tiles_y, zchunk);
     const int nb = tiles_x * tiles_y * ((s.nz + zl - 1) / zl);
+    const size_t rho_bytes = sizeof(R) * (size_t)s.n_mat;
+    if (rho_bytes <= (size_t)kRhoLdsBytes) {
+        const size_t lds = T::LDS + rho_bytes;
+        hipError_t e = allow_lds(k_velocity_tiled<R, TX, TY, true>, lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_velocity_tiled<R, TX, TY, true>), dim3(nb), dim3(T::NT), lds, st, s,
+                           tiles_x, tiles_y, zl);
+        return hipGetLastError();
+    }
     hipError_t e = allow_lds(k_velocity_tiled<R, TX, TY>, T::LDS);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_velocity_tiled<R, TX, TY>), dim3(nb), dim3(T::NT), T::LDS, st, s,
@@ -558,9 +578,18 @@ hipError_t stress_batched_launch(const BatchEntry<R> *tab, const int32_t *blk, i
 
 template <typename R, int TX, int TY>
 hipError_t velocity_batched_launch(const BatchEntry<R> *tab, const int32_t *blk, int nb,
-                                   const int32_t *step, hipStream_t st)
+                                   const int32_t *step, hipStream_t st, int max_mat)
 {
     using T = VelTile<R, TX, TY>;
+    const size_t rho_bytes = sizeof(R) * (size_t)max_mat;
+    if (rho_bytes <= (size_t)kRhoLdsBytes) {
+        const size_t lds = T::LDS + rho_bytes;
+        hipError_t e = allow_lds(k_velocity_batched<R, TX, TY, true>, lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_velocity_batched<R, TX, TY, true>), dim3(nb), dim3(T::NT), lds, st,
+                           tab, blk, nb, step);
+        return hipGetLastError();
+    }
     hipError_t e = allow_lds(k_velocity_batched<R, TX, TY>, T::LDS);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_velocity_batched<R, TX, TY>), dim3(nb), dim3(T::NT), T::LDS, st, tab,
@@ -638,28 +667,29 @@ hipError_t launch_stress_batched<double>(int v, const BatchEntry<double> *tab, c
 
 template <>
 hipError_t launch_velocity_batched<float>(int v, const BatchEntry<float> *tab, const int32_t *blk,
-                                          int nb, const int32_t *step, hipStream_t st)
+                                          int nb, const int32_t *step, hipStream_t st,
+                                          int max_mat)
 {
     switch (v) {
-    case 1: return velocity_batched_launch<float, 32, 16>(tab, blk, nb, step, st);
-    case 2: return velocity_batched_launch<float, 32, 8>(tab, blk, nb, step, st);
-    case 3: return velocity_batched_launch<float, 64, 4>(tab, blk, nb, step, st);
-    case 4: return velocity_batched_launch<float, 128, 4>(tab, blk, nb, step, st);
-    case 5: return velocity_batched_launch<float, 32, 32>(tab, blk, nb, step, st);
-    default: return velocity_batched_launch<float, 64, 8>(tab, blk, nb, step, st);
+    case 1: return velocity_batched_launch<float, 32, 16>(tab, blk, nb, step, st, max_mat);
+    case 2: return velocity_batched_launch<float, 32, 8>(tab, blk, nb, step, st, max_mat);
+    case 3: return velocity_batched_launch<float, 64, 4>(tab, blk, nb, step, st, max_mat);
+    case 4: return velocity_batched_launch<float, 128, 4>(tab, blk, nb, step, st, max_mat);
+    case 5: return velocity_batched_launch<float, 32, 32>(tab, blk, nb, step, st, max_mat);
+    default: return velocity_batched_launch<float, 64, 8>(tab, blk, nb, step, st, max_mat);
     }
 }
 
 template <>
 hipError_t launch_velocity_batched<double>(int v, const BatchEntry<double> *tab,
                                            const int32_t *blk, int nb, const int32_t *step,
-                                           hipStream_t st)
+                                           hipStream_t st, int max_mat)
 {
     switch (v) {
-    case 1: return velocity_batched_launch<double, 64, 4>(tab, blk, nb, step, st);
-    case 2: return velocity_batched_launch<double, 32, 16>(tab, blk, nb, step, st);
-    case 3: return velocity_batched_launch<double, 32, 4>(tab, blk, nb, step, st);
-    default: return velocity_batched_launch<double, 32, 8>(tab, blk, nb, step, st);
+    case 1: return velocity_batched_launch<double, 64, 4>(tab, blk, nb, step, st, max_mat);
+    case 2: return velocity_batched_launch<double, 32, 16>(tab, blk, nb, step, st, max_mat);
+    case 3: return velocity_batched_launch<double, 32, 4>(tab, blk, nb, step, st, max_mat);
+    default: return velocity_batched_launch<double, 32, 8>(tab, blk, nb, step, st, max_mat);
     }
 }
 
