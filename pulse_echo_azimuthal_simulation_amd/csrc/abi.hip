@@ -287,6 +287,13 @@ int pe_check_shot(const pe_shot *p)
         return fail(PE_EINVAL, "precision %d is neither PE_F32 nor PE_F64", p->precision);
     if (p->kernel != PE_KERNEL_DIRECT && p->kernel != PE_KERNEL_TILED)
         return fail(PE_EINVAL, "unknown kernel family %d", p->kernel);
+    // the tiled kernels address a field by a 32-bit byte offset
+    const int64_t rbytes = p->precision == PE_F64 ? 8 : 4;
+    if (p->kernel == PE_KERNEL_TILED && ncell * rbytes >= ((int64_t)1 << 32))
+        return fail(PE_ERANGE,
+                    "grid of %lld cells: the tiled kernels need each field under 4 GiB "
+                    "(%lld bytes); use PE_KERNEL_DIRECT",
+                    (long long)ncell, (long long)(ncell * rbytes));
     if (p->n_mat < 1) return fail(PE_EINVAL, "n_mat = %d must be >= 1", p->n_mat);
     if (!p->mats || !p->Ct || !p->rho_t) return fail(PE_EINVAL, "mats, Ct and rho_t are required");
     for (int k = 0; k < 3; ++k)
@@ -300,8 +307,8 @@ int pe_check_shot(const pe_shot *p)
     if (p->n_rec < 0 || (p->n_rec > 0 && (!p->rec_ptr || !p->rec_idx || !p->rec_w)))
         return fail(PE_EINVAL, "receiver arrays missing");
     for (int h = 0; h < 2; ++h) {
-        int tx, ty, pad;
-        const int n = pe::tile_variants(p->precision, h, 0, &tx, &ty, &pad);
+        int tx, ty, pad, fr;
+        const int n = pe::tile_variants(p->precision, h, 0, &tx, &ty, &pad, &fr);
         if (p->tile[h] < 0 || p->tile[h] >= n)
             return fail(PE_EINVAL, "tile variant %d of half-step %d out of range [0, %d)",
                         p->tile[h], h, n);
@@ -393,13 +400,13 @@ int pe_tile_info(int32_t precision, int32_t half, int32_t variant, int32_t *info
     if (precision != PE_F32 && precision != PE_F64)
         return fail(PE_EINVAL, "precision %d is neither PE_F32 nor PE_F64", precision);
     if (half != 0 && half != 1) return fail(PE_EINVAL, "half must be 0 (stress) or 1 (velocity)");
-    int tx = 0, ty = 0, pad = 0;
-    const int n = pe::tile_variants(precision, half, variant, &tx, &ty, &pad);
+    int tx = 0, ty = 0, pad = 0, fr = 0;
+    const int n = pe::tile_variants(precision, half, variant, &tx, &ty, &pad, &fr);
     if (variant < 0 || variant >= n) return fail(PE_ERANGE, "variant %d >= %d", variant, n);
     if (info3) {
         info3[0] = tx;
         info3[1] = ty;
-        info3[2] = pad;
+        info3[2] = pad | (fr << 8);
     }
     return n;
 }

@@ -152,7 +152,7 @@ template <typename R>
 hipError_t launch_velocity_tiled(const Shot<R> &s, int variant, int zchunk, hipStream_t st);
 // Number of tile variants of a half-step (0 stress, 1 velocity) at a
 // precision; fills (tx, ty, pad) of `variant` when it exists.
-int tile_variants(int precision, int half, int variant, int *tx, int *ty, int *pad);
+int tile_variants(int precision, int half, int variant, int *tx, int *ty, int *pad, int *fr);
 // Tiling of one half-step of a grid; returns the workgroup count (-1: no
 // such variant).
 int tiled_geometry(int precision, int half, int variant, int zchunk, int nz, int ny, int nx,

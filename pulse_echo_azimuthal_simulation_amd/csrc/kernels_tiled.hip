@@ -30,17 +30,18 @@ using namespace tiled;
 // Body of the stress kernel for logical tile t (tile x fastest, then tile
 // y, then z-chunk) of shot S.  STORE_G: also write the strains g1..g6 of the
 // tile's own columns into S.g (6 x ncell, the history the adjoint's
-// stiffness gradient reads), each plane once.
-template <typename R, int TX, int TY, int PAD, bool STORE_G = false>
+// stiffness gradient reads), each plane once.  FR: see StressTile.
+template <typename R, int TX, int TY, int PAD, bool STORE_G = false, bool FR = false>
 __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x, int tiles_y,
                                                   int zc_len, int t)
 {
-    using T = StressTile<R, TX, TY, PAD>;
+    using T = StressTile<R, TX, TY, PAD, FR>;
     constexpr int GX = T::GX, GY = T::GY, NG = T::NG, VX = T::VX, VY = T::VY, NV = T::NV;
-    constexpr int NT = T::NT, NF = (T::NFRAME + NT - 1) / NT, WX = T::WX;
+    constexpr int NT = T::NT, NF = T::NF, WX = T::WX;
     extern __shared__ unsigned char lds_raw[];
     R *vt = reinterpret_cast<R *>(lds_raw);       // [3][VY][VX]
     R *gt = vt + 3 * NV;                           // [6][3][NT]: strains by thread
+    R *fr = gt + 6 * 3 * NT;                       // FR: [4][NF][3][NT] frame ring
 
     const int nz = S.nz, ny = S.ny, nx = S.nx;
     const int sy = nx, sz = nx * ny;
@@ -95,44 +96,69 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
     // z rings: at the top of iteration zc, r?[k] holds plane zc - 5 + k and
     // p? holds plane zc + 4 (loaded one iteration earlier); the shift at the
     // top makes r?[k] = plane zc - 4 + k.
-    // Frame delay line: fq[d] holds plane zc + d of this thread's frame
-    // entries.  Frame planes are fetched FD planes ahead so that they are
-    // read within about one step of the neighbouring tiles streaming the
-    // same plane into their rings, while the lines are still in L2.
+    // Frame: without FR, fq holds plane zc of this thread's frame entries
+    // (fetched one step ahead).  With FR, fq holds plane zc + 4, fetched in
+    // the same step as the ring's plane zc + 4 and so together with the
+    // neighbouring tiles' ring loads of that plane (an L2 hit for one of
+    // them); at the top of step zc it goes to ring slot zc & 3 right after
+    // plane zc is read from that slot.  Each thread reads back only what it
+    // wrote itself, so the ring needs no barrier.
     constexpr int FD = 1;
     R rx[9], ry[9], rz[9], px, py, pz, fq[FD][NF][3];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const int i = clampi(z0 - 6 + k, 0, nz - 1) * sz + col;
-        rx[k] = vx[i];
-        ry[k] = vy[i];
-        rz[k] = vz[i];
+        rx[k] = ldg(vx, i);
+        ry[k] = ldg(vy, i);
+        rz[k] = ldg(vz, i);
     }
     {
         const int i = clampi(z0 + 3, 0, nz - 1) * sz + col;
-        px = vx[i];
-        py = vy[i];
-        pz = vz[i];
+        px = ldg(vx, i);
+        py = ldg(vy, i);
+        pz = ldg(vz, i);
     }
-    // output-plane operands of the next step: s, label, damping
-    R ns[6], nd;
-    int nm;
-    {
+    // output-plane operands of the next step: s, label, damping (only the
+    // tile's own columns write a stress)
+    R ns[6] = {0, 0, 0, 0, 0, 0}, nd = 1;
+    int nm = 0;
+    if (core) {
         const int io1 = clampi(z0 - 2, 0, nz - 1) * sz + col;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) ns[k] = S.s[k][io1];
-        nm = S.mats[io1];
-        nd = has_damp ? S.damp[io1] : (R)1;
+        for (int k = 0; k < 6; ++k) ns[k] = ldg(S.s[k], io1);
+        nm = ldg(S.mats, io1);
+        nd = has_damp ? ldg(S.damp, io1) : (R)1;
     }
+    if constexpr (FR) {
+        // planes z0-1 .. z0+2 into their ring slots, plane z0+3 into fq
 #pragma unroll
-    for (int d = 0; d < FD; ++d)
+        for (int d = 0; d < 5; ++d)
 #pragma unroll
-        for (int k = 0; k < NF; ++k) {
-            const int i = clampi(z0 - 1 + d, 0, nz - 1) * sz + f_xy[k];
-            fq[d][k][0] = vx[i];
-            fq[d][k][1] = vy[i];
-            fq[d][k][2] = vz[i];
-        }
+            for (int k = 0; k < NF; ++k) {
+                const int i = clampi(z0 - 1 + d, 0, nz - 1) * sz + f_xy[k];
+                const R a = ldg(vx, i), b = ldg(vy, i), c = ldg(vz, i);
+                if (d < 4) {
+                    R *sl = fr + ((((z0 - 1 + d) & 3) * NF + k) * 3) * NT + tid;
+                    sl[0] = a;
+                    sl[NT] = b;
+                    sl[2 * NT] = c;
+                } else {
+                    fq[0][k][0] = a;
+                    fq[0][k][1] = b;
+                    fq[0][k][2] = c;
+                }
+            }
+    } else {
+#pragma unroll
+        for (int d = 0; d < FD; ++d)
+#pragma unroll
+            for (int k = 0; k < NF; ++k) {
+                const int i = clampi(z0 - 1 + d, 0, nz - 1) * sz + f_xy[k];
+                fq[d][k][0] = ldg(vx, i);
+                fq[d][k][1] = ldg(vy, i);
+                fq[d][k][2] = ldg(vz, i);
+            }
+    }
 
     for (int zc = z0 - 1; zc <= z1; ++zc) {
         const bool plane_in = zc >= 0 && zc < nz;
@@ -150,11 +176,23 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
         vt[0 * NV + vc] = rx[4];
         vt[1 * NV + vc] = ry[4];
         vt[2 * NV + vc] = rz[4];
+        if constexpr (FR) {
 #pragma unroll
-        for (int k = 0; k < NF; ++k) {
-            vt[0 * NV + f_lds[k]] = fq[0][k][0];
-            vt[1 * NV + f_lds[k]] = fq[0][k][1];
-            vt[2 * NV + f_lds[k]] = fq[0][k][2];
+            for (int k = 0; k < NF; ++k) {
+                R *sl = fr + (((zc & 3) * NF + k) * 3) * NT + tid;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    vt[c * NV + f_lds[k]] = sl[c * NT];
+                    sl[c * NT] = fq[0][k][c];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < NF; ++k) {
+                vt[0 * NV + f_lds[k]] = fq[0][k][0];
+                vt[1 * NV + f_lds[k]] = fq[0][k][1];
+                vt[2 * NV + f_lds[k]] = fq[0][k][2];
+            }
         }
 #pragma unroll
         for (int d = 0; d + 1 < FD; ++d)
@@ -164,10 +202,10 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
                 fq[d][k][1] = fq[d + 1][k][1];
                 fq[d][k][2] = fq[d + 1][k][2];
             }
-        // loads in the order they are needed: this iteration's output-plane
-        // operands, then the frame of plane zc+FD, then the ring's plane zc+5
-        // this step's output-plane operands (loaded one step ahead), then
-        // the next step's
+        // this step's output-plane operands (loaded one step ahead); then the
+        // loads for later steps in the order they are needed: the next
+        // step's operands, the frame (plane zc+1, or zc+5 with FR), the
+        // ring's plane zc+5
         const int zo = zc - 1;
         const bool out = core && zo >= z0 && zo < z1;
         const int io = clampi(zo, 0, nz - 1) * sz + col;
@@ -176,27 +214,27 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
         for (int k = 0; k < 6; ++k) sv[k] = ns[k];
         const int m = nm;
         const R dmp = nd;
-        {
+        if (core) {
             const int io1 = clampi(zo + 1, 0, nz - 1) * sz + col;
 #pragma unroll
-            for (int k = 0; k < 6; ++k) ns[k] = S.s[k][io1];
-            nm = S.mats[io1];
-            nd = has_damp ? S.damp[io1] : (R)1;
+            for (int k = 0; k < 6; ++k) ns[k] = ldg(S.s[k], io1);
+            nm = ldg(S.mats, io1);
+            nd = has_damp ? ldg(S.damp, io1) : (R)1;
         }
         {
-            const int zf = clampi(zc + FD, 0, nz - 1) * sz;
+            const int zf = clampi(zc + (FR ? 5 : FD), 0, nz - 1) * sz;
 #pragma unroll
             for (int k = 0; k < NF; ++k) {
-                fq[FD - 1][k][0] = vx[zf + f_xy[k]];
-                fq[FD - 1][k][1] = vy[zf + f_xy[k]];
-                fq[FD - 1][k][2] = vz[zf + f_xy[k]];
+                fq[FD - 1][k][0] = ldg(vx, zf + f_xy[k]);
+                fq[FD - 1][k][1] = ldg(vy, zf + f_xy[k]);
+                fq[FD - 1][k][2] = ldg(vz, zf + f_xy[k]);
             }
         }
         {
             const int i = clampi(zc + 5, 0, nz - 1) * sz + col;
-            px = vx[i];
-            py = vy[i];
-            pz = vz[i];
+            px = ldg(vx, i);
+            py = ldg(vy, i);
+            pz = ldg(vz, i);
         }
         lds_barrier();
 
@@ -240,7 +278,7 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
                                                     S.Ct + (size_t)m * 36, dmp, sv);
             }
 #pragma unroll
-            for (int k = 0; k < 6; ++k) S.s[k][io] = sv[k];
+            for (int k = 0; k < 6; ++k) stg(S.s[k], io, sv[k]);
         }
     }
 }
@@ -370,15 +408,15 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const int i = clampi(z0 - 5 + k, 0, nz - 1) * sz + col;
-        r3[k] = s3[i];
-        r4[k] = s4[i];
-        r5[k] = s5[i];
+        r3[k] = ldg(s3, i);
+        r4[k] = ldg(s4, i);
+        r5[k] = ldg(s5, i);
     }
     {
         const int i = clampi(z0 + 4, 0, nz - 1) * sz + col;
-        p3 = s3[i];
-        p4 = s4[i];
-        p5 = s5[i];
+        p3 = ldg(s3, i);
+        p4 = ldg(s4, i);
+        p5 = ldg(s5, i);
     }
     const bool has_damp = S.damp != nullptr;
     auto load_frame = [&](int z) {
@@ -388,24 +426,24 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
             const int gi = zo + f_xy[k];
             fv[k][0] = (f_x[k] ? s1 : s2)[gi];
             fv[k][1] = (f_x[k] ? s5 : s4)[gi];
-            fv[k][2] = s6[gi];
+            fv[k][2] = ldg(s6, gi);
         }
     };
     R c1, c2, c6, cvx, cvy, cvz, cdmp;
     int cm0, cmx, cmy, cmz;
     auto load_col = [&](int z) {
         const int i = clampi(z, 0, nz - 1) * sz + col;
-        c1 = s1[i];
-        c2 = s2[i];
-        c6 = s6[i];
-        cvx = S.v[0][i];
-        cvy = S.v[1][i];
-        cvz = S.v[2][i];
-        cdmp = has_damp ? S.damp[i] : (R)1;
-        cm0 = S.mats[i];
-        cmx = S.mats[ix < nx - 1 ? i + 1 : i];
-        cmy = S.mats[iy < ny - 1 ? i + sy : i];
-        cmz = S.mats[z < nz - 1 ? i + sz : i];
+        c1 = ldg(s1, i);
+        c2 = ldg(s2, i);
+        c6 = ldg(s6, i);
+        cvx = ldg(S.v[0], i);
+        cvy = ldg(S.v[1], i);
+        cvz = ldg(S.v[2], i);
+        cdmp = has_damp ? ldg(S.damp, i) : (R)1;
+        cm0 = ldg(S.mats, i);
+        cmx = ldg(S.mats, ix < nx - 1 ? i + 1 : i);
+        cmy = ldg(S.mats, iy < ny - 1 ? i + sy : i);
+        cmz = ldg(S.mats, z < nz - 1 ? i + sz : i);
     };
     load_frame(z0);
     load_col(z0);
@@ -440,9 +478,9 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
         load_col(z + 1);
         {
             const int i5 = clampi(z + 5, 0, nz - 1) * sz + col;
-            p3 = s3[i5];
-            p4 = s4[i5];
-            p5 = s5[i5];
+            p3 = ldg(s3, i5);
+            p4 = ldg(s4, i5);
+            p5 = ldg(s5, i5);
         }
         lds_barrier();
 
@@ -459,9 +497,9 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
             const R rvx = ix < nx - 1 ? (R)0.5 * (r0 + rho[mxp]) : r0;
             const R rvy = iy < ny - 1 ? (R)0.5 * (r0 + rho[myp]) : r0;
             const R rvz = z < nz - 1 ? (R)0.5 * (r0 + rho[mzp]) : r0;
-            S.v[0][i] = (vxo + S.dt_h * ax / rvx) * dmp;
-            S.v[1][i] = (vyo + S.dt_h * ay / rvy) * dmp;
-            S.v[2][i] = (vzo + S.dt_h * az / rvz) * dmp;
+            stg(S.v[0], i, (vxo + S.dt_h * ax / rvx) * dmp);
+            stg(S.v[1], i, (vyo + S.dt_h * ay / rvy) * dmp);
+            stg(S.v[2], i, (vzo + S.dt_h * az / rvz) * dmp);
         }
         lds_barrier();
     }
@@ -469,14 +507,14 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
 
 // ------------------------------------------------------------- launchers
 // Single-shot kernels.
-template <typename R, int TX, int TY, int PAD, bool STORE_G = false>
+template <typename R, int TX, int TY, int PAD, bool STORE_G = false, bool FR = false>
 __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
                              (MinWaves<R, StressTile<R, TX, TY, PAD>::NT>::v)) void k_stress_tiled(
     Shot<R> S, int tiles_x, int tiles_y, int zc_len)
 {
     const int nb = tiles_x * tiles_y * ((S.nz + zc_len - 1) / zc_len);
-    stress_tiled_body<R, TX, TY, PAD, STORE_G>(S, tiles_x, tiles_y, zc_len,
-                                               xcd_remap(blockIdx.x, nb));
+    stress_tiled_body<R, TX, TY, PAD, STORE_G, FR>(S, tiles_x, tiles_y, zc_len,
+                                                   xcd_remap(blockIdx.x, nb));
 }
 
 template <typename R, int TX, int TY, bool RHO_LDS = false>
@@ -491,7 +529,7 @@ __global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_velocity_
 // Multi-shot kernels: the workgroups of every shot of a batch in one launch.
 // blk_shot maps a (remapped) workgroup id to its shot; a shot whose step
 // counter has reached its step count exits at once.
-template <typename R, int TX, int TY, int PAD>
+template <typename R, int TX, int TY, int PAD, bool FR = false>
 __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
                              (MinWaves<R, StressTile<R, TX, TY, PAD>::NT>::v)) void k_stress_batched(
     const BatchEntry<R> *__restrict__ tab, const int32_t *__restrict__ blk_shot, int nb_total,
@@ -502,8 +540,8 @@ __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
     const BatchEntry<R> *e = tab + s;
     if (step[s] >= e->nt) return;
     const Shot<R> S = e->S;
-    stress_tiled_body<R, TX, TY, PAD>(S, e->tiles_x[0], e->tiles_y[0], e->zc_len[0],
-                                      g - e->blk_off[0]);
+    stress_tiled_body<R, TX, TY, PAD, false, FR>(S, e->tiles_x[0], e->tiles_y[0], e->zc_len[0],
+                                                 g - e->blk_off[0]);
 }
 
 template <typename R, int TX, int TY, bool RHO_LDS = false>
@@ -520,24 +558,24 @@ __global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_velocity_
                                             g - e->blk_off[1]);
 }
 
-template <typename R, int TX, int TY, int PAD>
+template <typename R, int TX, int TY, int PAD, bool FR = false>
 hipError_t stress_launch(const Shot<R> &s, int zchunk, hipStream_t st, bool store_g = false)
 {
-    using T = StressTile<R, TX, TY, PAD>;
+    using T = StressTile<R, TX, TY, PAD, FR>;
     const int tiles_x = (s.nx + TX - 1) / TX, tiles_y = (s.ny + TY - 1) / TY;
     const int zl = chunk_len(s.nz, tiles_x * tiles_y, zchunk);
     const int nb = tiles_x * tiles_y * ((s.nz + zl - 1) / zl);
     if (store_g) {
-        hipError_t e = allow_lds(k_stress_tiled<R, TX, TY, PAD, true>, T::LDS);
+        hipError_t e = allow_lds(k_stress_tiled<R, TX, TY, PAD, true, FR>, T::LDS);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_stress_tiled<R, TX, TY, PAD, true>), dim3(nb), dim3(T::NT), T::LDS,
-                           st, s, tiles_x, tiles_y, zl);
+        hipLaunchKernelGGL((k_stress_tiled<R, TX, TY, PAD, true, FR>), dim3(nb), dim3(T::NT),
+                           T::LDS, st, s, tiles_x, tiles_y, zl);
         return hipGetLastError();
     }
-    hipError_t e = allow_lds(k_stress_tiled<R, TX, TY, PAD>, T::LDS);
+    hipError_t e = allow_lds(k_stress_tiled<R, TX, TY, PAD, false, FR>, T::LDS);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_stress_tiled<R, TX, TY, PAD>), dim3(nb), dim3(T::NT), T::LDS, st, s,
-                       tiles_x, tiles_y, zl);
+    hipLaunchKernelGGL((k_stress_tiled<R, TX, TY, PAD, false, FR>), dim3(nb), dim3(T::NT), T::LDS,
+                       st, s, tiles_x, tiles_y, zl);
     return hipGetLastError();
 }
 
@@ -564,15 +602,15 @@ hipError_t velocity_launch(const Shot<R> &s, int zchunk, hipStream_t st)
     return hipGetLastError();
 }
 
-template <typename R, int TX, int TY, int PAD>
+template <typename R, int TX, int TY, int PAD, bool FR = false>
 hipError_t stress_batched_launch(const BatchEntry<R> *tab, const int32_t *blk, int nb,
                                  const int32_t *step, hipStream_t st)
 {
-    using T = StressTile<R, TX, TY, PAD>;
-    hipError_t e = allow_lds(k_stress_batched<R, TX, TY, PAD>, T::LDS);
+    using T = StressTile<R, TX, TY, PAD, FR>;
+    hipError_t e = allow_lds(k_stress_batched<R, TX, TY, PAD, FR>, T::LDS);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_stress_batched<R, TX, TY, PAD>), dim3(nb), dim3(T::NT), T::LDS, st, tab,
-                       blk, nb, step);
+    hipLaunchKernelGGL((k_stress_batched<R, TX, TY, PAD, FR>), dim3(nb), dim3(T::NT), T::LDS, st,
+                       tab, blk, nb, step);
     return hipGetLastError();
 }
 
@@ -598,19 +636,22 @@ hipError_t velocity_batched_launch(const BatchEntry<R> *tab, const int32_t *blk,
 }
 
 // Tile variants (TX, TY, LDS row padding); variant 0 is the default.
+// fr: the stress kernel's frame ring (StressTile).
 struct TileInfo {
-    int tx, ty, pad;
+    int tx, ty, pad, fr;
 };
-constexpr TileInfo kStressF32[] = {{32, 13, 24}, {62, 6, 0}, {30, 14, 24}, {32, 16, 24},
-                                   {32, 8, 24},  {16, 16, 0}, {32, 16, 0}};
-constexpr TileInfo kStressF64[] = {{32, 6, 0}, {32, 4, 0}, {16, 8, 0}, {32, 6, 24}};
-constexpr TileInfo kVelF32[] = {{64, 8, 0}, {32, 16, 0}, {32, 8, 0}, {64, 4, 0}, {128, 4, 0},
-                                {32, 32, 0}};
-constexpr TileInfo kVelF64[] = {{32, 8, 0}, {64, 4, 0}, {32, 16, 0}, {32, 4, 0}};
+constexpr TileInfo kStressF32[] = {{32, 13, 24, 0}, {62, 6, 0, 0},  {30, 14, 24, 0},
+                                   {32, 16, 24, 0}, {32, 8, 24, 0}, {16, 16, 0, 0},
+                                   {32, 16, 0, 0},  {30, 14, 24, 1}, {32, 13, 24, 1},
+                                   {30, 14, 0, 1},  {62, 14, 0, 0}, {62, 14, 0, 1}};
+constexpr TileInfo kStressF64[] = {{32, 6, 0, 0}, {32, 4, 0, 0}, {16, 8, 0, 0}, {32, 6, 24, 0}};
+constexpr TileInfo kVelF32[] = {{64, 8, 0, 0},  {32, 16, 0, 0}, {32, 8, 0, 0},
+                                {64, 4, 0, 0},  {128, 4, 0, 0}, {32, 32, 0, 0}};
+constexpr TileInfo kVelF64[] = {{32, 8, 0, 0}, {64, 4, 0, 0}, {32, 16, 0, 0}, {32, 4, 0, 0}};
 
 }  // namespace
 
-int tile_variants(int precision, int half, int variant, int *tx, int *ty, int *pad)
+int tile_variants(int precision, int half, int variant, int *tx, int *ty, int *pad, int *fr)
 {
     const TileInfo *t = nullptr;
     int n = 0;
@@ -622,6 +663,7 @@ int tile_variants(int precision, int half, int variant, int *tx, int *ty, int *p
         *tx = t[variant].tx;
         *ty = t[variant].ty;
         *pad = t[variant].pad;
+        *fr = t[variant].fr;
     }
     return n;
 }
@@ -629,8 +671,8 @@ int tile_variants(int precision, int half, int variant, int *tx, int *ty, int *p
 int tiled_geometry(int precision, int half, int variant, int zchunk, int nz, int ny, int nx,
                    int *tiles_x, int *tiles_y, int *zc_len)
 {
-    int tx = 0, ty = 0, pad = 0;
-    const int n = tile_variants(precision, half, variant, &tx, &ty, &pad);
+    int tx = 0, ty = 0, pad = 0, fr = 0;
+    const int n = tile_variants(precision, half, variant, &tx, &ty, &pad, &fr);
     if (variant < 0 || variant >= n) return -1;
     *tiles_x = (nx + tx - 1) / tx;
     *tiles_y = (ny + ty - 1) / ty;
@@ -649,6 +691,11 @@ hipError_t launch_stress_batched<float>(int v, const BatchEntry<float> *tab, con
     case 4: return stress_batched_launch<float, 32, 8, 24>(tab, blk, nb, step, st);
     case 5: return stress_batched_launch<float, 16, 16, 0>(tab, blk, nb, step, st);
     case 6: return stress_batched_launch<float, 32, 16, 0>(tab, blk, nb, step, st);
+    case 7: return stress_batched_launch<float, 30, 14, 24, true>(tab, blk, nb, step, st);
+    case 8: return stress_batched_launch<float, 32, 13, 24, true>(tab, blk, nb, step, st);
+    case 9: return stress_batched_launch<float, 30, 14, 0, true>(tab, blk, nb, step, st);
+    case 10: return stress_batched_launch<float, 62, 14, 0>(tab, blk, nb, step, st);
+    case 11: return stress_batched_launch<float, 62, 14, 0, true>(tab, blk, nb, step, st);
     default: return stress_batched_launch<float, 32, 13, 24>(tab, blk, nb, step, st);
     }
 }
@@ -704,6 +751,11 @@ hipError_t launch_stress_tiled<float>(const Shot<float> &s, int v, int zc, hipSt
     case 4: return stress_launch<float, 32, 8, 24>(s, zc, st, g);
     case 5: return stress_launch<float, 16, 16, 0>(s, zc, st, g);
     case 6: return stress_launch<float, 32, 16, 0>(s, zc, st, g);
+    case 7: return stress_launch<float, 30, 14, 24, true>(s, zc, st, g);
+    case 8: return stress_launch<float, 32, 13, 24, true>(s, zc, st, g);
+    case 9: return stress_launch<float, 30, 14, 0, true>(s, zc, st, g);
+    case 10: return stress_launch<float, 62, 14, 0>(s, zc, st, g);
+    case 11: return stress_launch<float, 62, 14, 0, true>(s, zc, st, g);
     default: return stress_launch<float, 32, 13, 24>(s, zc, st, g);
     }
 }

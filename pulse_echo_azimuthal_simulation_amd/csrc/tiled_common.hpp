@@ -10,6 +10,23 @@ namespace tiled {
 
 constexpr int kXcd = 8;
 
+// Element i of a field addressed by a 32-bit unsigned byte offset from the
+// (wave-uniform) field base: the access takes the base in SGPRs and one
+// offset VGPR, with no 64-bit address arithmetic per lane.  Valid while the
+// field is smaller than 4 GiB (pe_check_shot enforces it for the tiled
+// kernels).
+template <typename T>
+__device__ __forceinline__ T ldg(const T *base, int i)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) +
+                                        (uint32_t)i * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void stg(T *base, int i, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (uint32_t)i * (uint32_t)sizeof(T)) = v;
+}
+
 // Remap a workgroup id so consecutive logical tiles land on the same XCD.
 __device__ __forceinline__ int xcd_remap(int bid, int nb)
 {
@@ -25,7 +42,10 @@ __device__ __forceinline__ R ld(const R *p, int i, bool ok)
 }
 
 // ------------------------------------------------------------------ stress
-template <typename R, int TX, int TY, int PAD>
+// FR: the frame of each v plane is fetched in the same step as the z rings'
+// plane (5 planes before it is used) and parked in a per-thread LDS ring of
+// 4 planes; without FR it is fetched one step before use.
+template <typename R, int TX, int TY, int PAD, bool FR = false>
 struct StressTile {
     static constexpr int GX = TX + 2, GY = TY + 2, NG = GX * GY;   // strain columns
     static constexpr int NT = (NG + 63) / 64 * 64;                 // threads
@@ -33,7 +53,9 @@ struct StressTile {
     // onto disjoint LDS banks
     static constexpr int WX = GX + 8, VX = WX + PAD, VY = GY + 8, NV = VX * VY;
     static constexpr int NFRAME = 8 * (WX + GY);                   // v tile outside G
-    static constexpr size_t LDS = sizeof(R) * (3 * NV + 6 * 3 * NT);
+    static constexpr int NF = (NFRAME + NT - 1) / NT;              // frame entries per thread
+    static constexpr int NRING = FR ? 4 * NF * 3 * NT : 0;         // frame ring words
+    static constexpr size_t LDS = sizeof(R) * (3 * NV + 6 * 3 * NT + NRING);
 };
 
 // Moved strain q_{P,J} read from the LDS strain ring (see moved_direct in
@@ -208,10 +230,13 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? l
 // the values a clamped address returns are never used (the edge guards of
 // the generic path exclude them; the fast path only runs where none occur).
 // Waves per SIMD the register allocator must leave room for so that two
-// workgroups of NT threads fit on a CU (float); double keeps the compiler's
-// choice (its larger register footprint would spill).
+// workgroups of NT threads fit on a CU (float; one workgroup from 1024
+// threads up); double keeps the compiler's choice (its larger register
+// footprint would spill).
 template <typename R, int NT> struct MinWaves { static constexpr int v = 1; };
-template <int NT> struct MinWaves<float, NT> { static constexpr int v = (2 * (NT / 64) + 3) / 4; };
+template <int NT> struct MinWaves<float, NT> {
+    static constexpr int v = NT >= 1024 ? NT / 256 : (2 * (NT / 64) + 3) / 4;
+};
 
 // ------------------------------------------------------------- launchers
 // z planes per workgroup when the caller does not fix it: about a dozen
