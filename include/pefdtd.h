@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PE_ABI_VERSION 7
+#define PE_ABI_VERSION 8
 
 enum { PE_OK = 0, PE_EINVAL = -1, PE_EHIP = -2, PE_ERANGE = -3 };
 enum { PE_F32 = 0, PE_F64 = 1 };
@@ -72,6 +72,24 @@ typedef struct pe_shot {
     int32_t kernel;              /* PE_KERNEL_* */
     int32_t tile[2];             /* PE_KERNEL_TILED tile variant: [0] stress, [1] velocity */
     int32_t zchunk[2];           /* z planes per workgroup, 0 = automatic */
+    /* CPML absorbing boundary (no upstream counterpart: the upstream absorber
+     * is the sponge folded into damp).  cpml_width L = 0: none.  Otherwise
+     * axis a (0 z, 1 y, 2 x, length n >= 2L+2) has a band of L+1 cells at
+     * each end (i <= L or i >= n-1-L, band index i or i-(n-2L-2)); cells in
+     * any band replace each derivative dd along a band axis by
+     * fma(dd, 1/kappa, psi'), psi' = fma(b, psi, a*dd).
+     *   cpml_prof[a]: real[2][3][n]: [stagger: 0 integer points, 1 half
+     *                 points][b, a, 1/kappa][index]
+     *   cpml_psi[a]:  real[2][6][band volume], zero-initialised; band volume
+     *                 (2L+2)*ny*nx (z), nz*(2L+2)*nx (y), nz*ny*(2L+2) (x);
+     *                 components 0..2 belong to the stress half-step, 3..5
+     *                 to the velocity half-step
+     * Step n reads buffer n & 1 and writes the other; pe_run takes n from
+     * its loop, pe_stress / pe_velocity from cpml_step.  Forward only. */
+    int32_t cpml_width;
+    int32_t cpml_step;
+    const void *cpml_prof[3];
+    void *cpml_psi[3];
 } pe_shot;
 
 /* ABI version (PE_ABI_VERSION) and the last error message of this thread. */

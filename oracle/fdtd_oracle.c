@@ -58,7 +58,41 @@ typedef struct {
     const int64_t *rec_idx;
     const void *rec_w;          /* real */
     int32_t record_mode;        /* 0 pressure, 1 vx, 2 vy, 3 vz */
+    /* CPML (no upstream counterpart; cpml_width 0 = none).  Axis a (0 z,
+     * 1 y, 2 x) of length n: band cells i <= L or i >= n-1-L, band index
+     * i or i - (n - 2L - 2); prof[a] = real[2 staggers][b, a, 1/kappa][n];
+     * psi[a] = real[2 buffers][6][band volume], read from buffer
+     * cpml_parity, written to the other. */
+    int32_t cpml_width;
+    int32_t cpml_parity;
+    const void *cpml_prof[3];
+    void *cpml_psi[3];
 } orc_shot;
+
+/* CPML band geometry of axis a: is index i in the band, its band index, and
+ * the flat index of cell (iz, iy, ix) within one psi component. */
+static inline int pml_in(int i, int n, int L) { return i <= L || i >= n - 1 - L; }
+static inline int pml_bi(int i, int n, int L) { return i <= L ? i : i - (n - 2 * L - 2); }
+static inline int64_t pml_vol(const orc_shot *S, int a)
+{
+    const int64_t W = 2 * (int64_t)S->cpml_width + 2;
+    if (a == 0) return W * S->ny * S->nx;
+    if (a == 1) return (int64_t)S->nz * W * S->nx;
+    return (int64_t)S->nz * S->ny * W;
+}
+static inline int64_t pml_flat(const orc_shot *S, int a, int iz, int iy, int ix)
+{
+    const int L = S->cpml_width;
+    const int64_t W = 2 * (int64_t)L + 2;
+    if (a == 0) return ((int64_t)pml_bi(iz, S->nz, L) * S->ny + iy) * S->nx + ix;
+    if (a == 1) return ((int64_t)iz * W + pml_bi(iy, S->ny, L)) * S->nx + ix;
+    return ((int64_t)iz * S->ny + iy) * W + pml_bi(ix, S->nx, L);
+}
+static inline int pml_cell(const orc_shot *S, int iz, int iy, int ix)
+{
+    const int L = S->cpml_width;
+    return L > 0 && (pml_in(iz, S->nz, L) || pml_in(iy, S->ny, L) || pml_in(ix, S->nx, L));
+}
 
 /* Position of each strain component J: 0 = centre, 1 = yz, 2 = xz, 3 = xy. */
 static const int VP[6] = {0, 0, 0, 1, 2, 3};

@@ -11,9 +11,33 @@
 #define CAT_(a, b) a##b
 #define CAT(a, b) CAT_(a, b)
 
+/* CPML derivative of psi component k along axis a (no upstream
+ * counterpart): dd is the plain derivative (sum_j c_j * difference / h) at
+ * stagger st (0 integer point, 1 half point) of axis index ia.  Outside the
+ * band of axis a it is dd itself; inside,
+ *   psi' = fma(b, psi, a_coef * dd),  D = fma(dd, 1/kappa, psi'),
+ * psi read from buffer cpml_parity and psi' written to the other. */
+static inline REAL CAT(pml_d, SUF)(const orc_shot *S, int a, int k, int st, int iz, int iy,
+                                   int ix, REAL dd)
+{
+    const int dims[3] = {S->nz, S->ny, S->nx}, idx[3] = {iz, iy, ix};
+    const int n = dims[a], ia = idx[a];
+    if (!pml_in(ia, n, S->cpml_width)) return dd;
+    const REAL *pr = (const REAL *)S->cpml_prof[a] + (int64_t)st * 3 * n;
+    const int64_t vol = pml_vol(S, a), f = (int64_t)k * vol + pml_flat(S, a, iz, iy, ix);
+    REAL *psi = (REAL *)S->cpml_psi[a];
+    const REAL p = FMA(pr[ia], psi[(int64_t)S->cpml_parity * 6 * vol + f], pr[n + ia] * dd);
+    psi[(int64_t)(1 - S->cpml_parity) * 6 * vol + f] = p;
+    return FMA(dd, pr[2 * n + ia], p);
+}
+
 /* Strains g1..g6 at every cell, from v (sim/core/fdtd.py:1136-1164).
  * Each g is sum_j c_j * (difference) accumulated j-ascending with per-term
- * edge guards, then scaled by 1/h. */
+ * edge guards, then scaled by 1/h.  CPML band cells (no upstream
+ * counterpart) accumulate each of the nine derivatives on its own, scale it
+ * by 1/h, pass it through pml_d and add the two derivatives of a shear
+ * strain in the order of the plain sums (g4: y then z, g5: x then z, g6: x
+ * then y). */
 static void CAT(strains, SUF)(const orc_shot *S)
 {
     const int nz = S->nz, ny = S->ny, nx = S->nx, nc = S->nc;
@@ -32,6 +56,41 @@ static void CAT(strains, SUF)(const orc_shot *S)
             for (int iy = 0; iy < ny; ++iy)
                 for (int ix = 0; ix < nx; ++ix) {
                     const int64_t i = ((int64_t)iz * ny + iy) * nx + ix;
+                    if (pml_cell(S, iz, iy, ix)) {
+                        REAL d[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                        for (int j = 0; j < nc; ++j) {
+                            const REAL cj = c[j];
+                            if (ix >= j + 1 && ix < nx - j)
+                                d[0] = FMA(cj, vx[i + j] - vx[i - (j + 1)], d[0]);
+                            if (iy >= j + 1 && iy < ny - j)
+                                d[1] = FMA(cj, vy[i + j * sy] - vy[i - (j + 1) * sy], d[1]);
+                            if (iz >= j + 1 && iz < nz - j)
+                                d[2] = FMA(cj, vz[i + j * sz] - vz[i - (j + 1) * sz], d[2]);
+                            if (iy >= j && iy < ny - j - 1)
+                                d[3] = FMA(cj, vz[i + (j + 1) * sy] - vz[i - j * sy], d[3]);
+                            if (iz >= j && iz < nz - j - 1)
+                                d[4] = FMA(cj, vy[i + (j + 1) * sz] - vy[i - j * sz], d[4]);
+                            if (ix >= j && ix < nx - j - 1)
+                                d[5] = FMA(cj, vz[i + (j + 1)] - vz[i - j], d[5]);
+                            if (iz >= j && iz < nz - j - 1)
+                                d[6] = FMA(cj, vx[i + (j + 1) * sz] - vx[i - j * sz], d[6]);
+                            if (ix >= j && ix < nx - j - 1)
+                                d[7] = FMA(cj, vy[i + (j + 1)] - vy[i - j], d[7]);
+                            if (iy >= j && iy < ny - j - 1)
+                                d[8] = FMA(cj, vx[i + (j + 1) * sy] - vx[i - j * sy], d[8]);
+                        }
+                        for (int t = 0; t < 9; ++t) d[t] = d[t] * inv_h;
+                        g[0 * ncell + i] = CAT(pml_d, SUF)(S, 2, 0, 0, iz, iy, ix, d[0]);
+                        g[1 * ncell + i] = CAT(pml_d, SUF)(S, 1, 0, 0, iz, iy, ix, d[1]);
+                        g[2 * ncell + i] = CAT(pml_d, SUF)(S, 0, 0, 0, iz, iy, ix, d[2]);
+                        g[3 * ncell + i] = CAT(pml_d, SUF)(S, 1, 1, 1, iz, iy, ix, d[3]) +
+                                           CAT(pml_d, SUF)(S, 0, 1, 1, iz, iy, ix, d[4]);
+                        g[4 * ncell + i] = CAT(pml_d, SUF)(S, 2, 1, 1, iz, iy, ix, d[5]) +
+                                           CAT(pml_d, SUF)(S, 0, 2, 1, iz, iy, ix, d[6]);
+                        g[5 * ncell + i] = CAT(pml_d, SUF)(S, 2, 2, 1, iz, iy, ix, d[7]) +
+                                           CAT(pml_d, SUF)(S, 1, 2, 1, iz, iy, ix, d[8]);
+                        continue;
+                    }
                     REAL a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0, a6 = 0;
                     for (int j = 0; j < nc; ++j) {
                         const REAL cj = c[j];
@@ -147,7 +206,10 @@ static void CAT(source, SUF)(const orc_shot *S, REAL wn)
 
 /* Velocity half-step (sim/core/fdtd.py:1211-1244): j-interleaved sums in
  * x, y, z tap order; face density 0.5 (rho[m] + rho[m(+1)]) or rho[m] on the
- * last slice; v <- (v + dt/h * a / rho_face) * damp. */
+ * last slice; v <- (v + dt/h * a / rho_face) * damp.  CPML band cells (no
+ * upstream counterpart): each of the nine stress derivatives accumulated on
+ * its own, scaled by 1/h, through pml_d (psi components 3..5), summed x, y, z
+ * per component; v <- (v + dt * a / rho_face) * damp. */
 static void CAT(velocity, SUF)(const orc_shot *S)
 {
     const int nz = S->nz, ny = S->ny, nx = S->nx, nc = S->nc;
@@ -171,6 +233,50 @@ static void CAT(velocity, SUF)(const orc_shot *S)
             for (int iy = 0; iy < ny; ++iy)
                 for (int ix = 0; ix < nx; ++ix) {
                     const int64_t i = ((int64_t)iz * ny + iy) * nx + ix;
+                    if (pml_cell(S, iz, iy, ix)) {
+                        REAL e[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                        for (int j = 0; j < nc; ++j) {
+                            const REAL cj = c[j];
+                            if (ix >= j && ix < nx - j - 1)
+                                e[0] = FMA(cj, s1[i + (j + 1)] - s1[i - j], e[0]);
+                            if (ix >= j + 1 && ix < nx - j) {
+                                e[3] = FMA(cj, s6[i + j] - s6[i - (j + 1)], e[3]);
+                                e[6] = FMA(cj, s5[i + j] - s5[i - (j + 1)], e[6]);
+                            }
+                            if (iy >= j && iy < ny - j - 1)
+                                e[4] = FMA(cj, s2[i + (j + 1) * sy] - s2[i - j * sy], e[4]);
+                            if (iy >= j + 1 && iy < ny - j) {
+                                e[1] = FMA(cj, s6[i + j * sy] - s6[i - (j + 1) * sy], e[1]);
+                                e[7] = FMA(cj, s4[i + j * sy] - s4[i - (j + 1) * sy], e[7]);
+                            }
+                            if (iz >= j && iz < nz - j - 1)
+                                e[8] = FMA(cj, s3[i + (j + 1) * sz] - s3[i - j * sz], e[8]);
+                            if (iz >= j + 1 && iz < nz - j) {
+                                e[2] = FMA(cj, s5[i + j * sz] - s5[i - (j + 1) * sz], e[2]);
+                                e[5] = FMA(cj, s4[i + j * sz] - s4[i - (j + 1) * sz], e[5]);
+                            }
+                        }
+                        const REAL inv_h = (REAL)S->inv_h, dt = (REAL)S->dt;
+                        for (int t = 0; t < 9; ++t) e[t] = e[t] * inv_h;
+                        const REAL ax = CAT(pml_d, SUF)(S, 2, 3, 1, iz, iy, ix, e[0]) +
+                                        CAT(pml_d, SUF)(S, 1, 3, 0, iz, iy, ix, e[1]) +
+                                        CAT(pml_d, SUF)(S, 0, 3, 0, iz, iy, ix, e[2]);
+                        const REAL ay = CAT(pml_d, SUF)(S, 2, 4, 0, iz, iy, ix, e[3]) +
+                                        CAT(pml_d, SUF)(S, 1, 4, 1, iz, iy, ix, e[4]) +
+                                        CAT(pml_d, SUF)(S, 0, 4, 0, iz, iy, ix, e[5]);
+                        const REAL az = CAT(pml_d, SUF)(S, 2, 5, 0, iz, iy, ix, e[6]) +
+                                        CAT(pml_d, SUF)(S, 1, 5, 0, iz, iy, ix, e[7]) +
+                                        CAT(pml_d, SUF)(S, 0, 5, 1, iz, iy, ix, e[8]);
+                        const REAL r0 = rho[mats[i]];
+                        const REAL rvx = ix < nx - 1 ? half * (r0 + rho[mats[i + 1]]) : r0;
+                        const REAL rvy = iy < ny - 1 ? half * (r0 + rho[mats[i + sy]]) : r0;
+                        const REAL rvz = iz < nz - 1 ? half * (r0 + rho[mats[i + sz]]) : r0;
+                        const REAL dmp = damp ? damp[i] : (REAL)1;
+                        vx[i] = (vx[i] + dt * ax / rvx) * dmp;
+                        vy[i] = (vy[i] + dt * ay / rvy) * dmp;
+                        vz[i] = (vz[i] + dt * az / rvz) * dmp;
+                        continue;
+                    }
                     REAL ax = 0, ay = 0, az = 0;
                     for (int j = 0; j < nc; ++j) {
                         const REAL cj = c[j];
@@ -232,16 +338,19 @@ static void CAT(record, SUF)(const orc_shot *S, REAL *row)
 }
 
 /* Steps n0 .. n1-1 of the hot loop (sim/core/fdtd.py:1786-1815):
- * stress, source, velocity, record.  traces is (nt, n_rec), row n written. */
+ * stress, source, velocity, record.  traces is (nt, n_rec), row n written.
+ * Step n reads the CPML memory variables from buffer n & 1. */
 void CAT(orc_run, SUF)(const orc_shot *S, int n0, int n1, const double *wavelet,
                        REAL *traces)
 {
+    orc_shot Sn = *S;
     for (int n = n0; n < n1; ++n) {
-        CAT(strains, SUF)(S);
-        CAT(stress, SUF)(S);
-        CAT(source, SUF)(S, (REAL)wavelet[n]);
-        CAT(velocity, SUF)(S);
-        if (traces) CAT(record, SUF)(S, traces + (int64_t)n * S->n_rec);
+        Sn.cpml_parity = n & 1;
+        CAT(strains, SUF)(&Sn);
+        CAT(stress, SUF)(&Sn);
+        CAT(source, SUF)(&Sn, (REAL)wavelet[n]);
+        CAT(velocity, SUF)(&Sn);
+        if (traces) CAT(record, SUF)(&Sn, traces + (int64_t)n * Sn.n_rec);
     }
 }
 

@@ -19,6 +19,11 @@ Parity pins (each checked in tests/):
   stress, stress symmetry under mirror of the grid, energy decay with damping;
 * a brute-force per-cell Python step on tiny grids (tests/test_oracle_small.py)
   that restates the stencils from their definitions independently of this C.
+
+The CPML boundary (no upstream counterpart) is pinned by its reduction to
+the plain scheme when every profile is (b, a, 1/kappa) = (1, 0, 1), by an
+independent per-cell Python restatement of the band path on tiny grids, and
+by a wide-domain reflection test (tests/test_oracle_cpml.py).
 """
 from __future__ import annotations
 
@@ -47,6 +52,8 @@ class _Shot(C.Structure):
         ("n_rec", C.c_int32), ("rec_ptr", C.c_void_p), ("rec_idx", C.c_void_p),
         ("rec_w", C.c_void_p),
         ("record_mode", C.c_int32),
+        ("cpml_width", C.c_int32), ("cpml_parity", C.c_int32),
+        ("cpml_prof", C.c_void_p * 3), ("cpml_psi", C.c_void_p * 3),
     ]
 
 
@@ -89,7 +96,7 @@ class OracleShot:
     """
 
     def __init__(self, mats, Ct, rho_t, damp, coeffs, h, dt, src, rec,
-                 record="pressure", precision="f32", act_mask=None):
+                 record="pressure", precision="f32", act_mask=None, cpml=None):
         self.real = np.float32 if precision == "f32" else np.float64
         self.suf = precision
         r = self.real
@@ -138,6 +145,20 @@ class OracleShot:
         st.rec_ptr = self.rec_ptr.ctypes.data
         st.rec_idx = self.rec_idx.ctypes.data
         st.rec_w = self.rec_w.ctypes.data
+        # CPML: (width, [prof_z, prof_y, prof_x]) with prof real (2, 3, n);
+        # memory variables (2 buffers, 6 components, band volume) per axis
+        self.psi = []
+        self.prof = []
+        if cpml is not None:
+            width, prof = cpml
+            W = 2 * int(width) + 2
+            vols = (W * ny * nx, nz * W * nx, nz * ny * W)
+            st.cpml_width = int(width)
+            for a in range(3):
+                self.prof.append(np.ascontiguousarray(prof[a], r))
+                self.psi.append(np.zeros((2, 6, vols[a]), r))
+                st.cpml_prof[a] = self.prof[a].ctypes.data
+                st.cpml_psi[a] = self.psi[a].ctypes.data
         st.record_mode = RECORD_MODES[record]
         self._st = st
 
@@ -151,6 +172,10 @@ class OracleShot:
         getattr(lib, "orc_run_" + self.suf)(C.byref(self._st), int(n0), int(n1),
                                             wav.ctypes.data, traces.ctypes.data)
         return traces
+
+    def set_step(self, n):
+        """Parity of step n picks the CPML buffer stress()/velocity() read."""
+        self._st.cpml_parity = int(n) & 1
 
     def stress(self):
         getattr(_load(), "orc_stress_" + self.suf)(C.byref(self._st))

@@ -16,7 +16,7 @@ PE_KERNEL_DIRECT, PE_KERNEL_TILED = 0, 1
 PE_HIST_VELOCITY, PE_HIST_STRAIN = 0, 1
 RECORD_MODES = {"pressure": 0, "vx": 1, "vy": 2, "vz": 3}
 KERNELS = {"direct": PE_KERNEL_DIRECT, "tiled": PE_KERNEL_TILED}
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 EXPORTS = ("pe_abi_version", "pe_last_error", "pe_check_shot", "pe_stress", "pe_source",
            "pe_velocity", "pe_record", "pe_run", "pe_run_timed", "pe_tile_info", "pe_run_store",
@@ -46,6 +46,10 @@ class PeShot(C.Structure):
         ("kernel", C.c_int32),
         ("tile", C.c_int32 * 2),
         ("zchunk", C.c_int32 * 2),
+        ("cpml_width", C.c_int32),
+        ("cpml_step", C.c_int32),
+        ("cpml_prof", C.c_void_p * 3),
+        ("cpml_psi", C.c_void_p * 3),
     ]
 
 

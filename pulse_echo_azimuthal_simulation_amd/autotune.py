@@ -39,7 +39,7 @@ def tune(shot, reps=5, zchunks=ZCHUNKS, verbose=False):
     """Pick tile variants and z-chunk lengths for ``shot`` (a DeviceShot with
     zero fields); sets them in its descriptor and returns
     {half: (variant, zchunk, ms)} plus the full timing table."""
-    key = (shot.shape, shot.desc.precision)
+    key = (shot.shape, shot.desc.precision, shot.desc.cpml_width)
     if key in _CACHE:
         best = _CACHE[key]
         for h in (0, 1):

@@ -62,13 +62,14 @@ template <typename R>
 int run_impl(const pe_shot *p, int32_t n0, int32_t n1, const double *wavelet, void *traces,
              hipStream_t st, void *vhist = nullptr, int hist_kind = 0)
 {
-    const pe::Shot<R> s = pe::make_shot<R>(p);
+    pe::Shot<R> s = pe::make_shot<R>(p);
     const bool direct = p->kernel == PE_KERNEL_DIRECT;
     // Pressure is read from s, which the velocity half-step does not touch,
     // so source and record share one launch before the velocity update.
     const bool rec_before = p->record_mode == PE_RECORD_PRESSURE;
     const size_t ncell = (size_t)p->nz * p->ny * p->nx;
     for (int32_t n = n0; n < n1; ++n) {
+        s.pml_par = n & 1;
         hipError_t e;
         if (vhist && hist_kind == 0) {
             e = pe::store_velocity<R>(s, (R *)vhist + (size_t)(n - n0) * 3 * ncell, st);
@@ -105,7 +106,7 @@ int run_timed_impl(const pe_shot *p, int32_t n0, int32_t n1, const double *wavel
                    void *traces, hipStream_t st, double *ms)
 {
     constexpr int kBatch = 128;
-    const pe::Shot<R> s = pe::make_shot<R>(p);
+    pe::Shot<R> s = pe::make_shot<R>(p);
     const bool direct = p->kernel == PE_KERNEL_DIRECT;
     const bool rec_before = p->record_mode == PE_RECORD_PRESSURE;
     hipEvent_t ev[kBatch][4];
@@ -118,6 +119,7 @@ int run_timed_impl(const pe_shot *p, int32_t n0, int32_t n1, const double *wavel
         const int nb = (n1 - b0) < kBatch ? (n1 - b0) : kBatch;
         for (int b = 0; b < nb && rc == PE_OK; ++b) {
             const int32_t n = b0 + b;
+            s.pml_par = n & 1;
             (void)hipEventRecord(ev[b][0], st);
             hipError_t e = direct ? pe::launch_stress_direct<R>(s, st) : pe::launch_stress_tiled<R>(s, p->tile[0], p->zchunk[0], st);
             (void)hipEventRecord(ev[b][1], st);
@@ -168,6 +170,8 @@ int batch_layout(const pe_shot *shots, int32_t n, BatchLayout *L)
         if (p->precision != prec) return fail(PE_EINVAL, "shot %d: precision differs from shot 0", i);
         if (p->kernel != PE_KERNEL_TILED)
             return fail(PE_EINVAL, "shot %d: batches run the tiled kernels only", i);
+        if (p->cpml_width > 0)
+            return fail(PE_EINVAL, "shot %d: batches run the sponge boundary only (no CPML)", i);
         for (int h = 0; h < 2; ++h) {
             int tx, ty, zl;
             const int nb = pe::tiled_geometry(prec, h, shots[0].tile[h], p->zchunk[h], p->nz, p->ny,
@@ -316,6 +320,17 @@ int pe_check_shot(const pe_shot *p)
     }
     if (p->record_mode < PE_RECORD_PRESSURE || p->record_mode > PE_RECORD_VZ)
         return fail(PE_EINVAL, "record mode %d unknown", p->record_mode);
+    if (p->cpml_width < 0) return fail(PE_EINVAL, "cpml_width %d < 0", p->cpml_width);
+    if (p->cpml_width > 0) {
+        const int dims[3] = {p->nz, p->ny, p->nx};
+        for (int a = 0; a < 3; ++a) {
+            if (dims[a] < 2 * p->cpml_width + 2)
+                return fail(PE_EINVAL, "CPML width %d needs every axis >= %d cells (axis %d has %d)",
+                            p->cpml_width, 2 * p->cpml_width + 2, a, dims[a]);
+            if (!p->cpml_prof[a] || !p->cpml_psi[a])
+                return fail(PE_EINVAL, "CPML profile or memory variables of axis %d are NULL", a);
+        }
+    }
     return PE_OK;
 }
 
@@ -390,6 +405,9 @@ int pe_run_store_strains(const pe_shot *p, int32_t n0, int32_t n1, const double 
     if (n0 < 0 || n1 < n0) return fail(PE_EINVAL, "bad step range [%d, %d)", n0, n1);
     if (!wavelet && p->n_src > 0) return fail(PE_EINVAL, "wavelet is NULL");
     if (!ghist) return fail(PE_EINVAL, "ghist is NULL");
+    if (p->cpml_width > 0)
+        return fail(PE_EINVAL, "the strain history serves the adjoint, which runs the sponge "
+                               "boundary only (no CPML)");
     return p->precision == PE_F32
                ? run_impl<float>(p, n0, n1, wavelet, traces, (hipStream_t)stream, ghist, 1)
                : run_impl<double>(p, n0, n1, wavelet, traces, (hipStream_t)stream, ghist, 1);
@@ -428,6 +446,8 @@ int pe_check_adjoint(const pe_shot *p, const pe_adjoint *a)
 {
     int rc = pe_check_shot(p);
     if (rc) return rc;
+    if (p->cpml_width > 0)
+        return fail(PE_EINVAL, "the adjoint runs the sponge boundary only (CPML is forward-only)");
     if (!a) return fail(PE_EINVAL, "adjoint descriptor is NULL");
     for (int k = 0; k < 3; ++k)
         if (!a->lv[k]) return fail(PE_EINVAL, "velocity adjoint %d is NULL", k);

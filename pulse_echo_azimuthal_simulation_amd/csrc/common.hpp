@@ -31,6 +31,13 @@ struct Shot {
     R *v[3];
     R *s[6];
     R *g;
+    // CPML (no upstream counterpart): band half-width (0 = none), the psi
+    // buffer read this step (the other one is written), per-axis (z, y, x)
+    // profiles real[2 staggers][b, a, 1/kappa][n] and memory variables
+    // real[2 buffers][6][band volume].
+    int pml, pml_par;
+    const R *prof[3];
+    R *psi[3];
 };
 
 template <typename R>
@@ -54,11 +61,64 @@ inline Shot<R> make_shot(const pe_shot *p)
     for (int k = 0; k < 3; ++k) s.v[k] = (R *)p->v[k];
     for (int k = 0; k < 6; ++k) s.s[k] = (R *)p->s[k];
     s.g = (R *)p->g;
+    s.pml = p->cpml_width;
+    s.pml_par = p->cpml_step & 1;
+    for (int a = 0; a < 3; ++a) {
+        s.prof[a] = (const R *)p->cpml_prof[a];
+        s.psi[a] = (R *)p->cpml_psi[a];
+    }
     return s;
 }
 
 __device__ __forceinline__ float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma_r(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// ------------------------------------------------------------------ CPML
+// Convolutional PML (no upstream counterpart).  Axis A (0 z, 1 y, 2 x) of
+// length n has a band of L+1 cells at each end: i <= L or i >= n-1-L, band
+// index i or i - (n - 2L - 2).  A cell in the band of any axis takes the
+// band path of the stencils: each derivative accumulated on its own, scaled
+// by 1/h, and along a band axis replaced by
+//   psi' = fma(b, psi, a * dd),  D = fma(dd, 1/kappa, psi')
+// with the profile of its stagger ST (0 integer point, 1 half point) and
+// memory variable K (0..2 strains, 3..5 accelerations).  psi is read from
+// buffer pml_par and written to the other, so a value read twice in one
+// step (the ring columns of neighbouring stress tiles) is the same.
+__device__ __forceinline__ bool pml_in(int i, int n, int L) { return i <= L || i >= n - 1 - L; }
+
+template <typename R>
+__device__ __forceinline__ bool pml_cell(const Shot<R> &S, int iz, int iy, int ix)
+{
+    const int L = S.pml;
+    return L > 0 && (pml_in(iz, S.nz, L) || pml_in(iy, S.ny, L) || pml_in(ix, S.nx, L));
+}
+
+template <int A, int K, int ST, typename R>
+__device__ __forceinline__ R pml_d(const Shot<R> &S, int iz, int iy, int ix, R dd, bool write)
+{
+    const int L = S.pml, W = 2 * L + 2;
+    const int n = A == 0 ? S.nz : (A == 1 ? S.ny : S.nx);
+    const int ia = A == 0 ? iz : (A == 1 ? iy : ix);
+    if (!pml_in(ia, n, L)) return dd;
+    const int bi = ia <= L ? ia : ia - (n - W);
+    size_t vol, f;
+    if constexpr (A == 0) {
+        vol = (size_t)W * S.ny * S.nx;
+        f = ((size_t)bi * S.ny + iy) * S.nx + ix;
+    } else if constexpr (A == 1) {
+        vol = (size_t)S.nz * W * S.nx;
+        f = ((size_t)iz * W + bi) * S.nx + ix;
+    } else {
+        vol = (size_t)S.nz * S.ny * W;
+        f = ((size_t)iz * S.ny + iy) * W + bi;
+    }
+    const R *pr = S.prof[A] + (size_t)ST * 3 * n;
+    f += (size_t)K * vol;
+    R *psi = S.psi[A];
+    const R p = fma_r(pr[ia], psi[(size_t)S.pml_par * 6 * vol + f], pr[n + ia] * dd);
+    if (write) psi[(size_t)(1 - S.pml_par) * 6 * vol + f] = p;
+    return fma_r(dd, pr[2 * n + ia], p);
+}
 
 // Workgroup barrier for LDS hand-offs only.  __syncthreads() also fences
 // global memory, which makes every wave wait for its outstanding global

@@ -30,8 +30,10 @@ using namespace tiled;
 // Body of the stress kernel for logical tile t (tile x fastest, then tile
 // y, then z-chunk) of shot S.  STORE_G: also write the strains g1..g6 of the
 // tile's own columns into S.g (6 x ncell, the history the adjoint's
-// stiffness gradient reads), each plane once.  FR: see StressTile.
-template <typename R, int TX, int TY, int PAD, bool STORE_G = false, bool FR = false>
+// stiffness gradient reads), each plane once.  FR: see StressTile.  PML: the
+// shot has a CPML boundary (band columns take strains_col_pml).
+template <typename R, int TX, int TY, int PAD, bool STORE_G = false, bool FR = false,
+          bool PML = false>
 __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x, int tiles_y,
                                                   int zc_len, int t)
 {
@@ -58,6 +60,7 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
     const int gx = gact ? tid % GX : 0, gy = gact ? tid / GX : 0;
     const int ix = x0 - 1 + gx, iy = y0 - 1 + gy;                  // column of this thread
     const bool col_in = gact && ix >= 0 && ix < nx && iy >= 0 && iy < ny;
+    const bool pml_xy = PML && (pml_in(ix, nx, S.pml) || pml_in(iy, ny, S.pml));
     const bool core = col_in && gx >= 1 && gx <= TX && gy >= 1 && gy <= TY;
     const int col = clampi(iy, 0, ny - 1) * sy + clampi(ix, 0, nx - 1);
     // LDS slot of this column in each v plane tile; idle threads write the
@@ -242,7 +245,17 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
         const int slot = (zc + 3) % 3;
         {
             R a[6] = {0, 0, 0, 0, 0, 0};
-            if (inner_xy && zc >= 4 && zc <= nz - 5)
+            bool band = false;
+            if constexpr (PML) band = pml_xy || pml_in(zc, nz, S.pml);
+            if (band) {
+                // CPML band: only the owner of the cell advances its memory
+                // variables (ring columns and chunk-edge planes are also
+                // computed by neighbouring workgroups)
+                if constexpr (PML)
+                    if (col_in && plane_in)
+                        strains_col_pml<R, VX>(S, vt + vc, vt + NV + vc, vt + 2 * NV + vc, rx, ry,
+                                               rz, ix, iy, zc, core && zc >= z0 && zc < z1, a);
+            } else if (inner_xy && zc >= 4 && zc <= nz - 5)
                 strains_col<false, R, VX>(S, vt + vc, vt + NV + vc, vt + 2 * NV + vc, rx, ry, rz,
                                           ix, iy, zc, a);
             else if (col_in && plane_in)
@@ -325,11 +338,54 @@ __device__ __forceinline__ void accel_col(const Shot<R> &S, const R *p1, const R
     }
 }
 
+// Accelerations of one column at a CPML band cell (see pml_d): the taps of
+// accel_col per derivative, scaled by 1/h, through pml_d (memory variables
+// 3..5), summed x, y, z per component.  Every velocity cell has one owner.
+template <typename R, int VX>
+__device__ __forceinline__ void accel_col_pml(const Shot<R> &S, const R *p1,
+                                                       const R *p2, const R *p4, const R *p5,
+                                                       const R *p6, const R (&r3)[9],
+                                                       const R (&r4)[9], const R (&r5)[9],
+                                                       int ix, int iy, int z, R &ax, R &ay,
+                                                       R &az)
+{
+    const int nz = S.nz, ny = S.ny, nx = S.nx;
+    R e[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (j >= S.nc) break;
+        const R cj = S.c[j];
+        if (ix >= j && ix < nx - j - 1) e[0] = fma_r(cj, p1[j + 1] - p1[-j], e[0]);
+        if (ix >= j + 1 && ix < nx - j) {
+            e[3] = fma_r(cj, p6[j] - p6[-(j + 1)], e[3]);
+            e[6] = fma_r(cj, p5[j] - p5[-(j + 1)], e[6]);
+        }
+        if (iy >= j && iy < ny - j - 1) e[4] = fma_r(cj, p2[(j + 1) * VX] - p2[-j * VX], e[4]);
+        if (iy >= j + 1 && iy < ny - j) {
+            e[1] = fma_r(cj, p6[j * VX] - p6[-(j + 1) * VX], e[1]);
+            e[7] = fma_r(cj, p4[j * VX] - p4[-(j + 1) * VX], e[7]);
+        }
+        if (z >= j && z < nz - j - 1) e[8] = fma_r(cj, r3[5 + j] - r3[4 - j], e[8]);
+        if (z >= j + 1 && z < nz - j) {
+            e[2] = fma_r(cj, r5[4 + j] - r5[3 - j], e[2]);
+            e[5] = fma_r(cj, r4[4 + j] - r4[3 - j], e[5]);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 9; ++t) e[t] = e[t] * S.inv_h;
+    ax = pml_d<2, 3, 1>(S, z, iy, ix, e[0], true) + pml_d<1, 3, 0>(S, z, iy, ix, e[1], true) +
+         pml_d<0, 3, 0>(S, z, iy, ix, e[2], true);
+    ay = pml_d<2, 4, 0>(S, z, iy, ix, e[3], true) + pml_d<1, 4, 1>(S, z, iy, ix, e[4], true) +
+         pml_d<0, 4, 0>(S, z, iy, ix, e[5], true);
+    az = pml_d<2, 5, 0>(S, z, iy, ix, e[6], true) + pml_d<1, 5, 0>(S, z, iy, ix, e[7], true) +
+         pml_d<0, 5, 1>(S, z, iy, ix, e[8], true);
+}
+
 // RHO_LDS: the density table sits in LDS after the five plane tiles, so the
 // face-density lookups of the output stage wait on LDS only.  A global
 // lookup there would make the wave wait for every prefetch issued before it
 // (vector memory counters retire in order).
-template <typename R, int TX, int TY, bool RHO_LDS = false>
+template <typename R, int TX, int TY, bool RHO_LDS = false, bool PML = false>
 __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_x, int tiles_y,
                                                     int zc_len, int t)
 {
@@ -360,6 +416,7 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
     const int lx = tid % TX, ly = tid / TX;
     const int ix = x0 + lx, iy = y0 + ly;
     const bool col_in = ix < nx && iy < ny;
+    const bool pml_xy = PML && (pml_in(ix, nx, S.pml) || pml_in(iy, ny, S.pml));
     const int col = clampi(iy, 0, ny - 1) * sy + clampi(ix, 0, nx - 1);
     const int vc = (ly + 4) * VX + (lx + 4);
 
@@ -485,8 +542,18 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
         lds_barrier();
 
         if (col_in) {
-            R ax, ay, az;
-            if (inner_xy && z >= 4 && z <= nz - 5)
+            R ax = 0, ay = 0, az = 0;
+            // a CPML band cell's accelerations are in units of 1/h already
+            R tstep = S.dt_h;
+            bool band = false;
+            if constexpr (PML) band = pml_xy || pml_in(z, nz, S.pml);
+            if (band) {
+                if constexpr (PML) {
+                    accel_col_pml<R, VX>(S, t1 + vc, t2 + vc, t4 + vc, t5 + vc, t6 + vc, r3, r4,
+                                         r5, ix, iy, z, ax, ay, az);
+                    tstep = S.dt;
+                }
+            } else if (inner_xy && z >= 4 && z <= nz - 5)
                 accel_col<false, R, VX>(S, t1 + vc, t2 + vc, t4 + vc, t5 + vc, t6 + vc, r3, r4, r5,
                                         ix, iy, z, ax, ay, az);
             else
@@ -497,9 +564,9 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
             const R rvx = ix < nx - 1 ? (R)0.5 * (r0 + rho[mxp]) : r0;
             const R rvy = iy < ny - 1 ? (R)0.5 * (r0 + rho[myp]) : r0;
             const R rvz = z < nz - 1 ? (R)0.5 * (r0 + rho[mzp]) : r0;
-            stg(S.v[0], i, (vxo + S.dt_h * ax / rvx) * dmp);
-            stg(S.v[1], i, (vyo + S.dt_h * ay / rvy) * dmp);
-            stg(S.v[2], i, (vzo + S.dt_h * az / rvz) * dmp);
+            stg(S.v[0], i, (vxo + tstep * ax / rvx) * dmp);
+            stg(S.v[1], i, (vyo + tstep * ay / rvy) * dmp);
+            stg(S.v[2], i, (vzo + tstep * az / rvz) * dmp);
         }
         lds_barrier();
     }
@@ -507,23 +574,24 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
 
 // ------------------------------------------------------------- launchers
 // Single-shot kernels.
-template <typename R, int TX, int TY, int PAD, bool STORE_G = false, bool FR = false>
+template <typename R, int TX, int TY, int PAD, bool STORE_G = false, bool FR = false,
+          bool PML = false>
 __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
                              (MinWaves<R, StressTile<R, TX, TY, PAD>::NT>::v)) void k_stress_tiled(
     Shot<R> S, int tiles_x, int tiles_y, int zc_len)
 {
     const int nb = tiles_x * tiles_y * ((S.nz + zc_len - 1) / zc_len);
-    stress_tiled_body<R, TX, TY, PAD, STORE_G, FR>(S, tiles_x, tiles_y, zc_len,
-                                                   xcd_remap(blockIdx.x, nb));
+    stress_tiled_body<R, TX, TY, PAD, STORE_G, FR, PML>(S, tiles_x, tiles_y, zc_len,
+                                                        xcd_remap(blockIdx.x, nb));
 }
 
-template <typename R, int TX, int TY, bool RHO_LDS = false>
+template <typename R, int TX, int TY, bool RHO_LDS = false, bool PML = false>
 __global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_velocity_tiled(
     Shot<R> S, int tiles_x, int tiles_y, int zc_len)
 {
     const int nb = tiles_x * tiles_y * ((S.nz + zc_len - 1) / zc_len);
-    velocity_tiled_body<R, TX, TY, RHO_LDS>(S, tiles_x, tiles_y, zc_len,
-                                             xcd_remap(blockIdx.x, nb));
+    velocity_tiled_body<R, TX, TY, RHO_LDS, PML>(S, tiles_x, tiles_y, zc_len,
+                                                  xcd_remap(blockIdx.x, nb));
 }
 
 // Multi-shot kernels: the workgroups of every shot of a batch in one launch.
@@ -558,13 +626,21 @@ __global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_velocity_
                                             g - e->blk_off[1]);
 }
 
-template <typename R, int TX, int TY, int PAD, bool FR = false>
+template <typename R, int TX, int TY, int PAD, bool FR = false, bool PML = false>
 hipError_t stress_launch(const Shot<R> &s, int zchunk, hipStream_t st, bool store_g = false)
 {
     using T = StressTile<R, TX, TY, PAD, FR>;
     const int tiles_x = (s.nx + TX - 1) / TX, tiles_y = (s.ny + TY - 1) / TY;
     const int zl = chunk_len(s.nz, tiles_x * tiles_y, zchunk);
     const int nb = tiles_x * tiles_y * ((s.nz + zl - 1) / zl);
+    if constexpr (PML) {
+        if (store_g) return hipErrorInvalidValue;   // the strain history serves the adjoint only
+        hipError_t e = allow_lds(k_stress_tiled<R, TX, TY, PAD, false, FR, true>, T::LDS);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_stress_tiled<R, TX, TY, PAD, false, FR, true>), dim3(nb),
+                           dim3(T::NT), T::LDS, st, s, tiles_x, tiles_y, zl);
+        return hipGetLastError();
+    }
     if (store_g) {
         hipError_t e = allow_lds(k_stress_tiled<R, TX, TY, PAD, true, FR>, T::LDS);
         if (e != hipSuccess) return e;
@@ -579,7 +655,7 @@ hipError_t stress_launch(const Shot<R> &s, int zchunk, hipStream_t st, bool stor
     return hipGetLastError();
 }
 
-template <typename R, int TX, int TY>
+template <typename R, int TX, int TY, bool PML = false>
 hipError_t velocity_launch(const Shot<R> &s, int zchunk, hipStream_t st)
 {
     using T = VelTile<R, TX, TY>;
@@ -589,16 +665,16 @@ hipError_t velocity_launch(const Shot<R> &s, int zchunk, hipStream_t st)
     const size_t rho_bytes = sizeof(R) * (size_t)s.n_mat;
     if (rho_bytes <= (size_t)kRhoLdsBytes) {
         const size_t lds = T::LDS + rho_bytes;
-        hipError_t e = allow_lds(k_velocity_tiled<R, TX, TY, true>, lds);
+        hipError_t e = allow_lds(k_velocity_tiled<R, TX, TY, true, PML>, lds);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_velocity_tiled<R, TX, TY, true>), dim3(nb), dim3(T::NT), lds, st, s,
-                           tiles_x, tiles_y, zl);
+        hipLaunchKernelGGL((k_velocity_tiled<R, TX, TY, true, PML>), dim3(nb), dim3(T::NT), lds,
+                           st, s, tiles_x, tiles_y, zl);
         return hipGetLastError();
     }
-    hipError_t e = allow_lds(k_velocity_tiled<R, TX, TY>, T::LDS);
+    hipError_t e = allow_lds(k_velocity_tiled<R, TX, TY, false, PML>, T::LDS);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_velocity_tiled<R, TX, TY>), dim3(nb), dim3(T::NT), T::LDS, st, s,
-                       tiles_x, tiles_y, zl);
+    hipLaunchKernelGGL((k_velocity_tiled<R, TX, TY, false, PML>), dim3(nb), dim3(T::NT), T::LDS,
+                       st, s, tiles_x, tiles_y, zl);
     return hipGetLastError();
 }
 
@@ -744,6 +820,8 @@ template <>
 hipError_t launch_stress_tiled<float>(const Shot<float> &s, int v, int zc, hipStream_t st,
                                       bool g)
 {
+    // CPML shots run one tile shape (the band path is compiled once)
+    if (s.pml > 0) return stress_launch<float, 32, 13, 24, false, true>(s, zc, st, g);
     switch (v) {
     case 1: return stress_launch<float, 62, 6, 0>(s, zc, st, g);
     case 2: return stress_launch<float, 30, 14, 24>(s, zc, st, g);
@@ -764,6 +842,7 @@ template <>
 hipError_t launch_stress_tiled<double>(const Shot<double> &s, int v, int zc, hipStream_t st,
                                        bool g)
 {
+    if (s.pml > 0) return stress_launch<double, 32, 6, 0, false, true>(s, zc, st, g);
     switch (v) {
     case 1: return stress_launch<double, 32, 4, 0>(s, zc, st, g);
     case 2: return stress_launch<double, 16, 8, 0>(s, zc, st, g);
@@ -775,6 +854,7 @@ hipError_t launch_stress_tiled<double>(const Shot<double> &s, int v, int zc, hip
 template <>
 hipError_t launch_velocity_tiled<float>(const Shot<float> &s, int v, int zc, hipStream_t st)
 {
+    if (s.pml > 0) return velocity_launch<float, 64, 4, true>(s, zc, st);
     switch (v) {
     case 1: return velocity_launch<float, 32, 16>(s, zc, st);
     case 2: return velocity_launch<float, 32, 8>(s, zc, st);
@@ -788,6 +868,7 @@ hipError_t launch_velocity_tiled<float>(const Shot<float> &s, int v, int zc, hip
 template <>
 hipError_t launch_velocity_tiled<double>(const Shot<double> &s, int v, int zc, hipStream_t st)
 {
+    if (s.pml > 0) return velocity_launch<double, 32, 8, true>(s, zc, st);
     switch (v) {
     case 1: return velocity_launch<double, 64, 4>(s, zc, st);
     case 2: return velocity_launch<double, 32, 16>(s, zc, st);

@@ -198,6 +198,42 @@ __device__ __forceinline__ void strains_col(const Shot<R> &S, const R *tx_, cons
     a[5] = a6 * S.inv_h;
 }
 
+// Strains of one column at a CPML band cell (see pml_d): the edge-guarded
+// taps of strains_col accumulated per derivative, scaled by 1/h, through
+// pml_d; write: this thread owns the cell's memory variables.
+template <typename R, int VX>
+__device__ __forceinline__ void strains_col_pml(const Shot<R> &S, const R *tx_,
+                                                         const R *ty_, const R *tz_,
+                                                         const R (&rx)[9], const R (&ry)[9],
+                                                         const R (&rz)[9], int ix, int iy,
+                                                         int zc, bool write, R (&a)[6])
+{
+    const int nz = S.nz, ny = S.ny, nx = S.nx;
+    R d[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (j >= S.nc) break;
+        const R cj = S.c[j];
+        if (ix >= j + 1 && ix < nx - j) d[0] = fma_r(cj, tx_[j] - tx_[-(j + 1)], d[0]);
+        if (iy >= j + 1 && iy < ny - j) d[1] = fma_r(cj, ty_[j * VX] - ty_[-(j + 1) * VX], d[1]);
+        if (zc >= j + 1 && zc < nz - j) d[2] = fma_r(cj, rz[4 + j] - rz[3 - j], d[2]);
+        if (iy >= j && iy < ny - j - 1) d[3] = fma_r(cj, tz_[(j + 1) * VX] - tz_[-j * VX], d[3]);
+        if (zc >= j && zc < nz - j - 1) d[4] = fma_r(cj, ry[5 + j] - ry[4 - j], d[4]);
+        if (ix >= j && ix < nx - j - 1) d[5] = fma_r(cj, tz_[j + 1] - tz_[-j], d[5]);
+        if (zc >= j && zc < nz - j - 1) d[6] = fma_r(cj, rx[5 + j] - rx[4 - j], d[6]);
+        if (ix >= j && ix < nx - j - 1) d[7] = fma_r(cj, ty_[j + 1] - ty_[-j], d[7]);
+        if (iy >= j && iy < ny - j - 1) d[8] = fma_r(cj, tx_[(j + 1) * VX] - tx_[-j * VX], d[8]);
+    }
+#pragma unroll
+    for (int t = 0; t < 9; ++t) d[t] = d[t] * S.inv_h;
+    a[0] = pml_d<2, 0, 0>(S, zc, iy, ix, d[0], write);
+    a[1] = pml_d<1, 0, 0>(S, zc, iy, ix, d[1], write);
+    a[2] = pml_d<0, 0, 0>(S, zc, iy, ix, d[2], write);
+    a[3] = pml_d<1, 1, 1>(S, zc, iy, ix, d[3], write) + pml_d<0, 1, 1>(S, zc, iy, ix, d[4], write);
+    a[4] = pml_d<2, 1, 1>(S, zc, iy, ix, d[5], write) + pml_d<0, 2, 1>(S, zc, iy, ix, d[6], write);
+    a[5] = pml_d<2, 2, 1>(S, zc, iy, ix, d[7], write) + pml_d<1, 2, 1>(S, zc, iy, ix, d[8], write);
+}
+
 // Stiffness rows are read in place.  A wave whose active lanes share one
 // material (the common case: grains span thousands of cells) reads its row
 // through a wave-uniform pointer in the constant address space, which the

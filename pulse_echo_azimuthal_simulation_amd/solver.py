@@ -34,11 +34,14 @@ class DeviceShot:
     All buffers are torch tensors on the current HIP device; the descriptor
     holds their addresses.  Fields start at zero.  ``mats`` and ``damp`` may
     be given as device tensors (prepared on the GPU), else host arrays.
+    ``cpml``: a ``cpml.Cpml`` for a CPML boundary (profiles on the device,
+    memory variables allocated here, zero), or None.
     """
 
     def __init__(self, mats, Ct, rho_t, damp, coeffs, h, dt, src_idx, src_w,
                  rec_ptr, rec_idx, rec_w, record="pressure", precision="f32",
-                 kernel="tiled", act_mask=None, device=None, tile=(0, 0), zchunk=(0, 0)):
+                 kernel="tiled", act_mask=None, device=None, tile=(0, 0), zchunk=(0, 0),
+                 cpml=None):
         torch = _torch()
         self.torch = torch
         dev = torch.device("cuda", torch.cuda.current_device() if device is None else device)
@@ -113,6 +116,17 @@ class DeviceShot:
         d.kernel = _abi.KERNELS[kernel]
         d.tile[0], d.tile[1] = int(tile[0]), int(tile[1])
         d.zchunk[0], d.zchunk[1] = int(zchunk[0]), int(zchunk[1])
+        self.cpml = cpml
+        self.cpml_prof, self.psi = [], []
+        if cpml is not None:
+            if tuple(cpml.shape) != self.shape:
+                raise ValueError(f"CPML built for {cpml.shape}, grid is {self.shape}")
+            d.cpml_width = int(cpml.width)
+            for a in range(3):
+                self.cpml_prof.append(dev_real(cpml.prof[a]))
+                self.psi.append(torch.zeros((2, 6, int(cpml.vol[a])), dtype=self.real, device=dev))
+                d.cpml_prof[a] = self.cpml_prof[a].data_ptr()
+                d.cpml_psi[a] = self.psi[a].data_ptr()
         self.desc = d
         self.lib = _abi.load()
         _abi.check(self.lib.pe_check_shot(C.byref(d)), "pe_check_shot")
@@ -120,6 +134,18 @@ class DeviceShot:
     # ---------------------------------------------------------------- steps
     def _stream(self):
         return self.torch.cuda.current_stream(self.device).cuda_stream
+
+    def reset(self):
+        """Zero the fields (and the CPML memory variables)."""
+        for f in self.v + self.s + self.psi:
+            f.zero_()
+        if self.g is not None:
+            self.g.zero_()
+
+    def set_step(self, n):
+        """Step index whose parity picks the CPML memory-variable buffer read by
+        ``stress()`` / ``velocity()`` (``run`` takes it from its own loop)."""
+        self.desc.cpml_step = int(n)
 
     def stress(self):
         _abi.check(self.lib.pe_stress(C.byref(self.desc), self._stream()), "pe_stress")
@@ -243,7 +269,9 @@ def forward_fused_labels(lab, axes, h, dt, nt, src_pts, wavelet, rec_groups,
                          c_couplant=1500.0, rho_couplant=300.0,
                          record="pressure", slab=32, stress_smem=False,
                          progress=None, mat_tables=None, *, precision="f32",
-                         kernel="tiled", compat_uint8=False, progress_every=200, tune=False):
+                         kernel="tiled", compat_uint8=False, progress_every=200, tune=False,
+                         boundary="sponge", cpml_width=20, cpml_R=1e-4, cpml_f0=None,
+                         cpml_kappa=1.0):
     """Label-indexed anisotropic elastic FDTD, drop-in for the upstream
     ``fdtd.forward_fused_labels`` (sim/core/fdtd.py:1618-1816).
 
@@ -253,9 +281,19 @@ def forward_fused_labels(lab, axes, h, dt, nt, src_pts, wavelet, rec_groups,
     ``progress(frac)`` is called on the host every ``progress_every`` steps
     (at n = 0, 200, ...) and may raise to abort, as upstream.  ``tune=True``
     times the tiled kernels' launch shapes on this grid first (see autotune).
+
+    ``boundary="cpml"`` (no upstream counterpart) replaces the sponge by a
+    convolutional PML of ``cpml_width`` cells (see cpml.py): reflection
+    coefficient ``cpml_R``, frequency shift pi * ``cpml_f0`` (default: the
+    wavelet's spectral peak), ``cpml_kappa`` the largest stretch.  The sponge
+    must then be off (sponge_width 0); ``damp_mask`` still applies.
     """
     if record not in _abi.RECORD_MODES:
         raise ValueError(f"record must be one of {sorted(_abi.RECORD_MODES)}")
+    if boundary not in ("sponge", "cpml"):
+        raise ValueError("boundary must be 'sponge' or 'cpml'")
+    if boundary == "cpml" and sponge_width > 0:
+        raise ValueError("boundary='cpml' replaces the sponge: pass sponge_width=0")
     prep = prepare_shot(lab, axes, h, dt, src_pts, rec_groups, order, coeffs, sponge_width,
                         sponge_strength, damp_mask, c_couplant, rho_couplant, mat_tables,
                         compat_uint8)
@@ -263,10 +301,16 @@ def forward_fused_labels(lab, axes, h, dt, nt, src_pts, wavelet, rec_groups,
     wavelet = np.asarray(wavelet, float)
     if len(wavelet) < nt:
         raise ValueError(f"wavelet has {len(wavelet)} samples, nt = {nt}")
+    cp = None
+    if boundary == "cpml":
+        from . import cpml as _cpml
+        f0 = _cpml.dominant_frequency(wavelet[:nt], dt) if cpml_f0 is None else cpml_f0
+        cp = _cpml.Cpml(prep["mats"].shape, cpml_width, h, dt,
+                        _cpml.vmax_table(prep["Ct"], prep["rho_t"]), f0, cpml_R, cpml_kappa)
     shot = DeviceShot(prep["mats"], prep["Ct"], prep["rho_t"], prep["damp"], prep["coeffs"],
                       h, dt, prep["src_idx"], prep["src_w"], prep["rec_ptr"],
                       prep["rec_idx"], prep["rec_w"], record=record, precision=precision,
-                      kernel=kernel)
+                      kernel=kernel, cpml=cp)
     if tune and kernel == "tiled":
         from . import autotune
         autotune.tune(shot)

@@ -64,6 +64,10 @@ def test_check_shot_accepts_valid():
     (dict(record_mode=9), _abi.PE_EINVAL, "record mode"),
     (dict(n_src=3), _abi.PE_EINVAL, "source"),
     (dict(mats=None), _abi.PE_EINVAL, "required"),
+    (dict(nz=1100, ny=1000, nx=1000), _abi.PE_ERANGE, "under 4 GiB"),
+    (dict(cpml_width=-1), _abi.PE_EINVAL, "cpml_width"),
+    (dict(cpml_width=2), _abi.PE_EINVAL, "needs every axis >= 6"),
+    (dict(cpml_width=1), _abi.PE_EINVAL, "CPML profile or memory variables"),
 ])
 def test_check_shot_rejects(kw, code, needle):
     lib = _abi.load()
@@ -126,3 +130,24 @@ def test_csr_layout():
     assert list(p["rec_ptr"]) == [0, 2, 3]
     assert list(p["rec_idx"]) == [3 * 56 + 3 * 8 + 4, 3 * 56 + 4 * 8 + 4, 2 * 56 + 2 * 8 + 2]
     assert list(p["src_idx"]) == [3 * 56 + 3 * 8 + 4]
+
+
+def test_cpml_is_refused_by_adjoint_and_history():
+    """The adjoint and its strain history run the sponge boundary only."""
+    lib = _abi.load()
+    fake = 0x1000
+    d = _desc(cpml_width=1)
+    for a in range(3):
+        d.cpml_prof[a] = fake
+        d.cpml_psi[a] = fake
+    assert lib.pe_check_shot(C.byref(d)) == _abi.PE_OK
+    adj = _abi.PeAdjoint()
+    for k in range(3):
+        adj.lv[k] = fake
+    for k in range(6):
+        adj.ls[k] = fake
+    adj.work = fake
+    assert lib.pe_check_adjoint(C.byref(d), C.byref(adj)) == _abi.PE_EINVAL
+    assert "CPML" in lib.pe_last_error().decode()
+    assert lib.pe_run_store_strains(C.byref(d), 0, 1, fake, None, fake, None) == _abi.PE_EINVAL
+    assert "CPML" in lib.pe_last_error().decode()
