@@ -3,7 +3,9 @@
 Workload (default): one single-shot production run -- a 400 x 400 x 200
 FP32 grid (32M cells) holding a 100 mm x 35 mm ice disk of 3000 Voronoi
 grains with a Watson girdle fabric (kappa = -3), a 2.25 MHz Ricker from a
-6 mm piston on the top face, 20-cell sponge absorber, 12000 steps.
+6 mm piston on the top face, 20-cell CPML absorber (the configuration's
+boundary; --boundary sponge runs the upstream sponge instead, and the
+default run also times the sponge shot for comparison), 12000 steps.
 
 A "step" is one FDTD time step over the whole grid (stress half-step,
 source, velocity half-step, receiver record), i.e. nz*ny*nx cell-updates.
@@ -106,6 +108,11 @@ def main():
                     help="adjoint mode: HBM budget for velocity history kept from the first "
                          "pass (segments that fit are not recomputed)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--boundary", default="cpml", choices=["cpml", "sponge"],
+                    help="absorber of the forward run (cpml: 20 cells + couplant mask)")
+    ap.add_argument("--sponge-steps", type=int, default=2000,
+                    help="forward cpml runs: also time this many steps with the 20-cell "
+                         "sponge (0 = skip)")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--launch", default=None,
                     help="fixed launch shapes 'stress_variant,stress_zchunk,vel_variant,vel_zchunk' "
@@ -136,19 +143,32 @@ def main():
         cfg = specimen.config_c1(nt=nt)
         workload = ("single-shot forward, 400x400x200 FP32 grid (32M cells), 3000-grain Voronoi "
                     "ice disk 100x35 mm, Watson girdle kappa=-3, 2.25 MHz Ricker, 6 mm top "
-                    "piston (pulse-echo), 20-cell sponge + couplant damping mask, %d steps" % nt)
+                    "piston (pulse-echo), 20-cell %s + couplant damping mask, %d steps"
+                    % ("CPML" if args.boundary == "cpml" else "sponge", nt))
     else:
         cfg = specimen.config_c0(nt=nt)
         workload = ("plain forward, 96x96x48 disk raster (+23-cell pads), 200 grains, Watson "
-                    "kappa=5, 1 MHz 3-cycle Hann, 6 mm top piston, 20-cell sponge, %d steps" % nt)
-    prep = solver.prepare_shot(cfg["lab"], cfg["axes"], cfg["h"], cfg["dt"], cfg["src_pts"],
-                               cfg["rec_groups"], coeffs=cfg["coeffs"],
-                               sponge_width=cfg["sponge_width"], damp_mask=cfg["damp_mask"],
-                               mat_tables=cfg["mat_tables"])
-    shot = solver.DeviceShot(prep["mats"], prep["Ct"], prep["rho_t"], prep["damp"],
-                             prep["coeffs"], cfg["h"], cfg["dt"], prep["src_idx"], prep["src_w"],
-                             prep["rec_ptr"], prep["rec_idx"], prep["rec_w"],
-                             precision=args.precision, kernel=args.kernel)
+                    "kappa=5, 1 MHz 3-cycle Hann, 6 mm top piston, 20-cell %s, %d steps"
+                    % ("CPML" if args.boundary == "cpml" else "sponge", nt))
+
+    def make_shot(boundary):
+        cpml_w = cfg["sponge_width"] if boundary == "cpml" else 0
+        prep = solver.prepare_shot(cfg["lab"], cfg["axes"], cfg["h"], cfg["dt"], cfg["src_pts"],
+                                   cfg["rec_groups"], coeffs=cfg["coeffs"],
+                                   sponge_width=0 if boundary == "cpml" else cfg["sponge_width"],
+                                   damp_mask=cfg["damp_mask"], mat_tables=cfg["mat_tables"])
+        cp = None
+        if cpml_w:
+            from pulse_echo_azimuthal_simulation_amd import cpml
+            cp = cpml.Cpml(prep["mats"].shape, cpml_w, cfg["h"], cfg["dt"],
+                           cpml.vmax_table(prep["Ct"], prep["rho_t"]),
+                           cpml.dominant_frequency(cfg["wavelet"], cfg["dt"]))
+        return solver.DeviceShot(prep["mats"], prep["Ct"], prep["rho_t"], prep["damp"],
+                                 prep["coeffs"], cfg["h"], cfg["dt"], prep["src_idx"],
+                                 prep["src_w"], prep["rec_ptr"], prep["rec_idx"], prep["rec_w"],
+                                 precision=args.precision, kernel=args.kernel, cpml=cp)
+
+    shot = make_shot(args.boundary)
     tuned = None
     if args.launch:
         from pulse_echo_azimuthal_simulation_amd import _abi
@@ -175,8 +195,7 @@ def main():
     if args.warmup > 0:
         shot.run(wav, 0, args.warmup, traces)
     torch.cuda.synchronize()
-    for a in shot.v + shot.s:
-        a.zero_()
+    shot.reset()
     traces.zero_()
     torch.cuda.synchronize()
     log("warm-up %d steps done; timing %d steps" % (args.warmup, args.steps))
@@ -219,7 +238,8 @@ def main():
         "dtype": p,
         "data": "synthetic",
         "config": {"workload": workload, "grid_zyx": [nz, ny, nx], "cells": ncell,
-                   "grains": int(len(cfg["axes"])), "boundary": "sponge20+fluid-mask",
+                   "grains": int(len(cfg["axes"])),
+                   "boundary": ("cpml20" if args.boundary == "cpml" else "sponge20") + "+fluid-mask",
                    "kernel": args.kernel, "mode": "forward",
                    "launch": None if tuned is None else
                    {"stress": tuned[0], "velocity": tuned[1]}},
@@ -232,6 +252,29 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "traces_finite": finite,
     }
+    if args.boundary == "cpml" and args.sponge_steps > 0:
+        # the same workload with the upstream absorber (20-cell sponge)
+        del shot, traces
+        torch.cuda.empty_cache()
+        sp = make_shot("sponge")
+        if args.kernel == "tiled" and not args.no_tune:
+            from pulse_echo_azimuthal_simulation_amd import autotune
+            autotune.tune(sp)
+        tr2 = sp.new_traces(args.sponge_steps)
+        sp.run(wav, 0, min(args.warmup, args.sponge_steps), tr2)
+        torch.cuda.synchronize()
+        sp.reset()
+        t1 = time.perf_counter()
+        ms2 = sp.run_timed(wav, 0, args.sponge_steps, tr2)
+        torch.cuda.synchronize()
+        w2 = time.perf_counter() - t1
+        out["sponge20"] = {"value": round(ncell * args.sponge_steps / w2 / 1e9, 4),
+                           "steps": args.sponge_steps,
+                           "kernels_ms_per_step": {"stress": round(ms2[0] / args.sponge_steps, 5),
+                                                   "velocity": round(ms2[1] / args.sponge_steps, 5)}}
+        log("sponge20: %.2f Gcell-updates/s over %d steps" % (out["sponge20"]["value"],
+                                                               args.sponge_steps))
+        del sp, tr2
     if args.calibrate_copy > 0:
         n = int(args.calibrate_copy * 2 ** 30 / 4)
         a = torch.ones(n, dtype=torch.float32, device="cuda")

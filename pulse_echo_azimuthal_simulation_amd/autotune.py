@@ -50,7 +50,8 @@ def tune(shot, reps=5, zchunks=ZCHUNKS, verbose=False):
     best = {}
     for h in (0, 1):
         variants = _abi.tile_variants(prec, h)
-        for v in range(len(variants)):
+        # CPML shots run the variant-0 tile shape whatever the variant
+        for v in range(1 if shot.desc.cpml_width > 0 else len(variants)):
             for zc in zchunks:
                 shot.desc.tile[h] = v
                 shot.desc.zchunk[h] = zc

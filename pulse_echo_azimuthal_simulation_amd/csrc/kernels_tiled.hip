@@ -44,6 +44,8 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
     R *vt = reinterpret_cast<R *>(lds_raw);       // [3][VY][VX]
     R *gt = vt + 3 * NV;                           // [6][3][NT]: strains by thread
     R *fr = gt + 6 * 3 * NT;                       // FR: [4][NF][3][NT] frame ring
+    R *bp = fr + T::NRING;                         // PML: band profiles (pml_profiles_to_lds)
+    if constexpr (PML) pml_profiles_to_lds(S, bp, NT);
 
     const int nz = S.nz, ny = S.ny, nx = S.nx;
     const int sy = nx, sz = nx * ny;
@@ -174,6 +176,14 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
         rx[8] = px;
         ry[8] = py;
         rz[8] = pz;
+        // CPML band: this plane's memory variables, loaded ahead of the
+        // step's prefetches
+        R ps[9];
+        bool band = false;
+        if constexpr (PML) {
+            band = pml_xy || pml_in(zc, nz, S.pml);
+            if (band && col_in && plane_in) pml_psi_load(S, zc, iy, ix, 0, ps);
+        }
         // (1) plane zc of v into LDS: own column from the ring, frame from
         //     the delay line
         vt[0 * NV + vc] = rx[4];
@@ -245,16 +255,15 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
         const int slot = (zc + 3) % 3;
         {
             R a[6] = {0, 0, 0, 0, 0, 0};
-            bool band = false;
-            if constexpr (PML) band = pml_xy || pml_in(zc, nz, S.pml);
             if (band) {
                 // CPML band: only the owner of the cell advances its memory
                 // variables (ring columns and chunk-edge planes are also
                 // computed by neighbouring workgroups)
                 if constexpr (PML)
                     if (col_in && plane_in)
-                        strains_col_pml<R, VX>(S, vt + vc, vt + NV + vc, vt + 2 * NV + vc, rx, ry,
-                                               rz, ix, iy, zc, core && zc >= z0 && zc < z1, a);
+                        strains_col_pml<R, VX>(S, bp, ps, vt + vc, vt + NV + vc, vt + 2 * NV + vc,
+                                               rx, ry, rz, ix, iy, zc,
+                                               core && zc >= z0 && zc < z1, a);
             } else if (inner_xy && zc >= 4 && zc <= nz - 5)
                 strains_col<false, R, VX>(S, vt + vc, vt + NV + vc, vt + 2 * NV + vc, rx, ry, rz,
                                           ix, iy, zc, a);
@@ -339,15 +348,15 @@ __device__ __forceinline__ void accel_col(const Shot<R> &S, const R *p1, const R
 }
 
 // Accelerations of one column at a CPML band cell (see pml_d): the taps of
-// accel_col per derivative, scaled by 1/h, through pml_d (memory variables
-// 3..5), summed x, y, z per component.  Every velocity cell has one owner.
+// accel_col per derivative, scaled by 1/h, through pml_t (memory variables
+// 3..5 in ps); summed x, y, z per component.  Every velocity cell has one
+// owner.
 template <typename R, int VX>
-__device__ __forceinline__ void accel_col_pml(const Shot<R> &S, const R *p1,
-                                                       const R *p2, const R *p4, const R *p5,
-                                                       const R *p6, const R (&r3)[9],
-                                                       const R (&r4)[9], const R (&r5)[9],
-                                                       int ix, int iy, int z, R &ax, R &ay,
-                                                       R &az)
+__device__ __forceinline__ void accel_col_pml(const Shot<R> &S, const R *bp, const R (&ps)[9],
+                                              const R *p1, const R *p2, const R *p4,
+                                              const R *p5, const R *p6, const R (&r3)[9],
+                                              const R (&r4)[9], const R (&r5)[9], int ix, int iy,
+                                              int z, R &ax, R &ay, R &az)
 {
     const int nz = S.nz, ny = S.ny, nx = S.nx;
     R e[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -373,12 +382,15 @@ __device__ __forceinline__ void accel_col_pml(const Shot<R> &S, const R *p1,
     }
 #pragma unroll
     for (int t = 0; t < 9; ++t) e[t] = e[t] * S.inv_h;
-    ax = pml_d<2, 3, 1>(S, z, iy, ix, e[0], true) + pml_d<1, 3, 0>(S, z, iy, ix, e[1], true) +
-         pml_d<0, 3, 0>(S, z, iy, ix, e[2], true);
-    ay = pml_d<2, 4, 0>(S, z, iy, ix, e[3], true) + pml_d<1, 4, 1>(S, z, iy, ix, e[4], true) +
-         pml_d<0, 4, 0>(S, z, iy, ix, e[5], true);
-    az = pml_d<2, 5, 0>(S, z, iy, ix, e[6], true) + pml_d<1, 5, 0>(S, z, iy, ix, e[7], true) +
-         pml_d<0, 5, 1>(S, z, iy, ix, e[8], true);
+    ax = pml_t<2, 3, 1>(S, bp, z, iy, ix, e[0], ps, true) +
+         pml_t<1, 3, 0>(S, bp, z, iy, ix, e[1], ps, true) +
+         pml_t<0, 3, 0>(S, bp, z, iy, ix, e[2], ps, true);
+    ay = pml_t<2, 4, 0>(S, bp, z, iy, ix, e[3], ps, true) +
+         pml_t<1, 4, 1>(S, bp, z, iy, ix, e[4], ps, true) +
+         pml_t<0, 4, 0>(S, bp, z, iy, ix, e[5], ps, true);
+    az = pml_t<2, 5, 0>(S, bp, z, iy, ix, e[6], ps, true) +
+         pml_t<1, 5, 0>(S, bp, z, iy, ix, e[7], ps, true) +
+         pml_t<0, 5, 1>(S, bp, z, iy, ix, e[8], ps, true);
 }
 
 // RHO_LDS: the density table sits in LDS after the five plane tiles, so the
@@ -404,6 +416,9 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
         for (int k = threadIdx.x; k < S.n_mat; k += NT) rl[k] = S.rho[k];
         rho_tab = rl;                   // ordered by the loop's first barrier
     }
+    // PML: band profiles after the density table
+    R *bp = st + 5 * NV + (RHO_LDS ? S.n_mat : 0);
+    if constexpr (PML) pml_profiles_to_lds(S, bp, NT);
 
     const int nz = S.nz, ny = S.ny, nx = S.nx;
     const int sy = nx, sz = nx * ny;
@@ -516,6 +531,14 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
         r3[8] = p3;
         r4[8] = p4;
         r5[8] = p5;
+        // CPML band: this plane's memory variables, loaded ahead of the
+        // step's prefetches
+        R ps[9];
+        bool band = false;
+        if constexpr (PML) {
+            band = pml_xy || pml_in(z, nz, S.pml);
+            if (band && col_in) pml_psi_load(S, z, iy, ix, 3, ps);
+        }
         // (1) plane z of s1, s2, s4, s5, s6 into LDS
         t1[vc] = c1;
         t2[vc] = c2;
@@ -545,12 +568,10 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
             R ax = 0, ay = 0, az = 0;
             // a CPML band cell's accelerations are in units of 1/h already
             R tstep = S.dt_h;
-            bool band = false;
-            if constexpr (PML) band = pml_xy || pml_in(z, nz, S.pml);
             if (band) {
                 if constexpr (PML) {
-                    accel_col_pml<R, VX>(S, t1 + vc, t2 + vc, t4 + vc, t5 + vc, t6 + vc, r3, r4,
-                                         r5, ix, iy, z, ax, ay, az);
+                    accel_col_pml<R, VX>(S, bp, ps, t1 + vc, t2 + vc, t4 + vc, t5 + vc, t6 + vc,
+                                         r3, r4, r5, ix, iy, z, ax, ay, az);
                     tstep = S.dt;
                 }
             } else if (inner_xy && z >= 4 && z <= nz - 5)
@@ -573,25 +594,54 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
 }
 
 // ------------------------------------------------------------- launchers
-// Single-shot kernels.
+// Single-shot kernels.  With a CPML boundary the tiles are split over two
+// launches of the same geometry: the plain kernel (PML false) takes the
+// tiles clear of every band, the PML kernel the others.  The PML kernel may
+// use twice the registers (its band path needs them; the plain tiles keep
+// their occupancy).
+template <typename R, int TX, int TY>
+__device__ __forceinline__ bool stress_band_tile(const Shot<R> &S, int t, int tiles_x,
+                                                 int tiles_y, int zc_len)
+{
+    // strain columns x0-1 .. x0+TX, y0-1 .. y0+TY; planes z0-1 .. z1
+    const int L = S.pml;
+    const int x0 = (t % tiles_x) * TX, y0 = ((t / tiles_x) % tiles_y) * TY;
+    const int z0 = (t / (tiles_x * tiles_y)) * zc_len, z1 = min(S.nz, z0 + zc_len);
+    return x0 - 1 <= L || x0 + TX >= S.nx - 1 - L || y0 - 1 <= L || y0 + TY >= S.ny - 1 - L ||
+           z0 - 1 <= L || z1 >= S.nz - 1 - L;
+}
+
+template <typename R, int TX, int TY>
+__device__ __forceinline__ bool velocity_band_tile(const Shot<R> &S, int t, int tiles_x,
+                                                   int tiles_y, int zc_len)
+{
+    const int L = S.pml;
+    const int x0 = (t % tiles_x) * TX, y0 = ((t / tiles_x) % tiles_y) * TY;
+    const int z0 = (t / (tiles_x * tiles_y)) * zc_len, z1 = min(S.nz, z0 + zc_len);
+    return x0 <= L || x0 + TX - 1 >= S.nx - 1 - L || y0 <= L || y0 + TY - 1 >= S.ny - 1 - L ||
+           z0 <= L || z1 - 1 >= S.nz - 1 - L;
+}
+
 template <typename R, int TX, int TY, int PAD, bool STORE_G = false, bool FR = false,
           bool PML = false>
 __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
-                             (MinWaves<R, StressTile<R, TX, TY, PAD>::NT>::v)) void k_stress_tiled(
-    Shot<R> S, int tiles_x, int tiles_y, int zc_len)
+                             (PML ? 1 : MinWaves<R, StressTile<R, TX, TY, PAD>::NT>::v)) void
+k_stress_tiled(Shot<R> S, int tiles_x, int tiles_y, int zc_len)
 {
     const int nb = tiles_x * tiles_y * ((S.nz + zc_len - 1) / zc_len);
-    stress_tiled_body<R, TX, TY, PAD, STORE_G, FR, PML>(S, tiles_x, tiles_y, zc_len,
-                                                        xcd_remap(blockIdx.x, nb));
+    const int t = xcd_remap(blockIdx.x, nb);
+    if (S.pml > 0 && stress_band_tile<R, TX, TY>(S, t, tiles_x, tiles_y, zc_len) != PML) return;
+    stress_tiled_body<R, TX, TY, PAD, STORE_G, FR, PML>(S, tiles_x, tiles_y, zc_len, t);
 }
 
 template <typename R, int TX, int TY, bool RHO_LDS = false, bool PML = false>
-__global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_velocity_tiled(
+__global__ __launch_bounds__(TX *TY, (PML ? 1 : MinWaves<R, TX * TY>::v)) void k_velocity_tiled(
     Shot<R> S, int tiles_x, int tiles_y, int zc_len)
 {
     const int nb = tiles_x * tiles_y * ((S.nz + zc_len - 1) / zc_len);
-    velocity_tiled_body<R, TX, TY, RHO_LDS, PML>(S, tiles_x, tiles_y, zc_len,
-                                                  xcd_remap(blockIdx.x, nb));
+    const int t = xcd_remap(blockIdx.x, nb);
+    if (S.pml > 0 && velocity_band_tile<R, TX, TY>(S, t, tiles_x, tiles_y, zc_len) != PML) return;
+    velocity_tiled_body<R, TX, TY, RHO_LDS, PML>(S, tiles_x, tiles_y, zc_len, t);
 }
 
 // Multi-shot kernels: the workgroups of every shot of a batch in one launch.
@@ -635,10 +685,14 @@ hipError_t stress_launch(const Shot<R> &s, int zchunk, hipStream_t st, bool stor
     const int nb = tiles_x * tiles_y * ((s.nz + zl - 1) / zl);
     if constexpr (PML) {
         if (store_g) return hipErrorInvalidValue;   // the strain history serves the adjoint only
-        hipError_t e = allow_lds(k_stress_tiled<R, TX, TY, PAD, false, FR, true>, T::LDS);
+        const size_t lds_pml = T::LDS + sizeof(R) * 18 * (2 * (size_t)s.pml + 2);
+        hipError_t e = allow_lds(k_stress_tiled<R, TX, TY, PAD, false, FR, false>, T::LDS);
+        if (e == hipSuccess) e = allow_lds(k_stress_tiled<R, TX, TY, PAD, false, FR, true>, lds_pml);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_stress_tiled<R, TX, TY, PAD, false, FR, true>), dim3(nb),
+        hipLaunchKernelGGL((k_stress_tiled<R, TX, TY, PAD, false, FR, false>), dim3(nb),
                            dim3(T::NT), T::LDS, st, s, tiles_x, tiles_y, zl);
+        hipLaunchKernelGGL((k_stress_tiled<R, TX, TY, PAD, false, FR, true>), dim3(nb),
+                           dim3(T::NT), lds_pml, st, s, tiles_x, tiles_y, zl);
         return hipGetLastError();
     }
     if (store_g) {
@@ -665,16 +719,30 @@ hipError_t velocity_launch(const Shot<R> &s, int zchunk, hipStream_t st)
     const size_t rho_bytes = sizeof(R) * (size_t)s.n_mat;
     if (rho_bytes <= (size_t)kRhoLdsBytes) {
         const size_t lds = T::LDS + rho_bytes;
-        hipError_t e = allow_lds(k_velocity_tiled<R, TX, TY, true, PML>, lds);
+        hipError_t e = allow_lds(k_velocity_tiled<R, TX, TY, true>, lds);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_velocity_tiled<R, TX, TY, true, PML>), dim3(nb), dim3(T::NT), lds,
-                           st, s, tiles_x, tiles_y, zl);
+        hipLaunchKernelGGL((k_velocity_tiled<R, TX, TY, true>), dim3(nb), dim3(T::NT), lds, st, s,
+                           tiles_x, tiles_y, zl);
+        if constexpr (PML) {
+            const size_t lds_pml = lds + sizeof(R) * 18 * (2 * (size_t)s.pml + 2);
+            e = allow_lds(k_velocity_tiled<R, TX, TY, true, true>, lds_pml);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL((k_velocity_tiled<R, TX, TY, true, true>), dim3(nb), dim3(T::NT),
+                               lds_pml, st, s, tiles_x, tiles_y, zl);
+        }
         return hipGetLastError();
     }
-    hipError_t e = allow_lds(k_velocity_tiled<R, TX, TY, false, PML>, T::LDS);
+    hipError_t e = allow_lds(k_velocity_tiled<R, TX, TY>, T::LDS);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_velocity_tiled<R, TX, TY, false, PML>), dim3(nb), dim3(T::NT), T::LDS,
-                       st, s, tiles_x, tiles_y, zl);
+    hipLaunchKernelGGL((k_velocity_tiled<R, TX, TY>), dim3(nb), dim3(T::NT), T::LDS, st, s,
+                       tiles_x, tiles_y, zl);
+    if constexpr (PML) {
+        const size_t lds_pml = T::LDS + sizeof(R) * 18 * (2 * (size_t)s.pml + 2);
+        e = allow_lds(k_velocity_tiled<R, TX, TY, false, true>, lds_pml);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_velocity_tiled<R, TX, TY, false, true>), dim3(nb), dim3(T::NT),
+                           lds_pml, st, s, tiles_x, tiles_y, zl);
+    }
     return hipGetLastError();
 }
 
@@ -820,7 +888,8 @@ template <>
 hipError_t launch_stress_tiled<float>(const Shot<float> &s, int v, int zc, hipStream_t st,
                                       bool g)
 {
-    // CPML shots run one tile shape (the band path is compiled once)
+    // CPML shots run the variant-0 tile shape whatever the variant (the
+    // band path is compiled once per precision)
     if (s.pml > 0) return stress_launch<float, 32, 13, 24, false, true>(s, zc, st, g);
     switch (v) {
     case 1: return stress_launch<float, 62, 6, 0>(s, zc, st, g);
@@ -854,7 +923,7 @@ hipError_t launch_stress_tiled<double>(const Shot<double> &s, int v, int zc, hip
 template <>
 hipError_t launch_velocity_tiled<float>(const Shot<float> &s, int v, int zc, hipStream_t st)
 {
-    if (s.pml > 0) return velocity_launch<float, 64, 4, true>(s, zc, st);
+    if (s.pml > 0) return velocity_launch<float, 64, 8, true>(s, zc, st);
     switch (v) {
     case 1: return velocity_launch<float, 32, 16>(s, zc, st);
     case 2: return velocity_launch<float, 32, 8>(s, zc, st);

@@ -198,15 +198,112 @@ __device__ __forceinline__ void strains_col(const Shot<R> &S, const R *tx_, cons
     a[5] = a6 * S.inv_h;
 }
 
+// CPML in the tiled kernels (the recursion of pml_d).  The band part of the
+// profiles is copied to LDS at kernel start as bp[((a*2 + st)*3 + c)*W + bi]
+// (W = 2L+2 band indices per axis), and the memory variables of a cell are
+// loaded at the top of its plane's step, before that step's prefetches, so
+// the band path waits on no global load issued after them.
+template <typename R>
+__device__ __forceinline__ void pml_profiles_to_lds(const Shot<R> &S, R *bp, int nthreads)
+{
+    const int L = S.pml, W = 2 * L + 2;
+    for (int e = threadIdx.x; e < 18 * W; e += nthreads) {
+        const int bi = e % W, c = (e / W) % 3, st = (e / (3 * W)) % 2, a = e / (6 * W);
+        const int n = a == 0 ? S.nz : (a == 1 ? S.ny : S.nx);
+        const int i = bi <= L ? bi : bi + (n - W);
+        // select the axis without indexing the kernel argument (which would
+        // move it to scratch memory)
+        const R *pa = a == 0 ? S.prof[0] : (a == 1 ? S.prof[1] : S.prof[2]);
+        bp[e] = pa[(st * 3 + c) * n + i];
+    }
+}
+
+// Flat offset of cell (iz, iy, ix) in one psi component of axis A (band
+// index bi along A) and that component's volume.
+template <int A, typename R>
+__device__ __forceinline__ size_t pml_flat(const Shot<R> &S, int iz, int iy, int ix, int bi,
+                                           size_t &vol)
+{
+    const int W = 2 * S.pml + 2;
+    if constexpr (A == 0) {
+        vol = (size_t)W * S.ny * S.nx;
+        return ((size_t)bi * S.ny + iy) * S.nx + ix;
+    } else if constexpr (A == 1) {
+        vol = (size_t)S.nz * W * S.nx;
+        return ((size_t)iz * W + bi) * S.nx + ix;
+    } else {
+        vol = (size_t)S.nz * S.ny * W;
+        return ((size_t)iz * S.ny + iy) * W + bi;
+    }
+}
+
+template <int A, typename R>
+__device__ __forceinline__ int pml_bi(const Shot<R> &S, int iz, int iy, int ix)
+{
+    const int L = S.pml, W = 2 * L + 2;
+    const int n = A == 0 ? S.nz : (A == 1 ? S.ny : S.nx);
+    const int ia = A == 0 ? iz : (A == 1 ? iy : ix);
+    if (!pml_in(ia, n, L)) return -1;
+    return ia <= L ? ia : ia - (n - W);
+}
+
+// psi components K0..K0+2 of every band axis of a cell (read buffer):
+// ps[3A + k]; 0 along axes whose band the cell is not in.
+template <int A, typename R>
+__device__ __forceinline__ void pml_psi_load_axis(const Shot<R> &S, int iz, int iy, int ix, int K0,
+                                                  R (&ps)[9])
+{
+    const int bi = pml_bi<A>(S, iz, iy, ix);
+    if (bi < 0) {
+        ps[3 * A] = ps[3 * A + 1] = ps[3 * A + 2] = 0;
+        return;
+    }
+    size_t vol;
+    const size_t f = pml_flat<A>(S, iz, iy, ix, bi, vol);
+    const R *q = S.psi[A] + (size_t)S.pml_par * 6 * vol + (size_t)K0 * vol + f;
+    ps[3 * A] = q[0];
+    ps[3 * A + 1] = q[vol];
+    ps[3 * A + 2] = q[2 * vol];
+}
+
+template <typename R>
+__device__ __forceinline__ void pml_psi_load(const Shot<R> &S, int iz, int iy, int ix, int K0,
+                                             R (&ps)[9])
+{
+    pml_psi_load_axis<0>(S, iz, iy, ix, K0, ps);
+    pml_psi_load_axis<1>(S, iz, iy, ix, K0, ps);
+    pml_psi_load_axis<2>(S, iz, iy, ix, K0, ps);
+}
+
+// pml_d with the profile from the LDS band table and psi given (ps[3A +
+// K % 3]); the new psi goes to the write buffer when `write`.
+template <int A, int K, int ST, typename R>
+__device__ __forceinline__ R pml_t(const Shot<R> &S, const R *bp, int iz, int iy, int ix, R dd,
+                                   const R (&ps)[9], bool write)
+{
+    const int bi = pml_bi<A>(S, iz, iy, ix);
+    if (bi < 0) return dd;
+    const int W = 2 * S.pml + 2;
+    const R *pr = bp + ((A * 2 + ST) * 3) * W + bi;
+    const R p = fma_r(pr[0], ps[3 * A + K % 3], pr[W] * dd);
+    if (write) {
+        size_t vol;
+        const size_t f = pml_flat<A>(S, iz, iy, ix, bi, vol);
+        S.psi[A][(size_t)(1 - S.pml_par) * 6 * vol + (size_t)K * vol + f] = p;
+    }
+    return fma_r(dd, pr[2 * W], p);
+}
+
 // Strains of one column at a CPML band cell (see pml_d): the edge-guarded
 // taps of strains_col accumulated per derivative, scaled by 1/h, through
-// pml_d; write: this thread owns the cell's memory variables.
+// pml_t; ps: the cell's psi (components 0..2); write: this thread owns the
+// cell's memory variables.
 template <typename R, int VX>
-__device__ __forceinline__ void strains_col_pml(const Shot<R> &S, const R *tx_,
-                                                         const R *ty_, const R *tz_,
-                                                         const R (&rx)[9], const R (&ry)[9],
-                                                         const R (&rz)[9], int ix, int iy,
-                                                         int zc, bool write, R (&a)[6])
+__device__ __forceinline__ void strains_col_pml(const Shot<R> &S, const R *bp, const R (&ps)[9],
+                                                const R *tx_, const R *ty_, const R *tz_,
+                                                const R (&rx)[9], const R (&ry)[9],
+                                                const R (&rz)[9], int ix, int iy, int zc,
+                                                bool write, R (&a)[6])
 {
     const int nz = S.nz, ny = S.ny, nx = S.nx;
     R d[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -226,12 +323,15 @@ __device__ __forceinline__ void strains_col_pml(const Shot<R> &S, const R *tx_,
     }
 #pragma unroll
     for (int t = 0; t < 9; ++t) d[t] = d[t] * S.inv_h;
-    a[0] = pml_d<2, 0, 0>(S, zc, iy, ix, d[0], write);
-    a[1] = pml_d<1, 0, 0>(S, zc, iy, ix, d[1], write);
-    a[2] = pml_d<0, 0, 0>(S, zc, iy, ix, d[2], write);
-    a[3] = pml_d<1, 1, 1>(S, zc, iy, ix, d[3], write) + pml_d<0, 1, 1>(S, zc, iy, ix, d[4], write);
-    a[4] = pml_d<2, 1, 1>(S, zc, iy, ix, d[5], write) + pml_d<0, 2, 1>(S, zc, iy, ix, d[6], write);
-    a[5] = pml_d<2, 2, 1>(S, zc, iy, ix, d[7], write) + pml_d<1, 2, 1>(S, zc, iy, ix, d[8], write);
+    a[0] = pml_t<2, 0, 0>(S, bp, zc, iy, ix, d[0], ps, write);
+    a[1] = pml_t<1, 0, 0>(S, bp, zc, iy, ix, d[1], ps, write);
+    a[2] = pml_t<0, 0, 0>(S, bp, zc, iy, ix, d[2], ps, write);
+    a[3] = pml_t<1, 1, 1>(S, bp, zc, iy, ix, d[3], ps, write) +
+           pml_t<0, 1, 1>(S, bp, zc, iy, ix, d[4], ps, write);
+    a[4] = pml_t<2, 1, 1>(S, bp, zc, iy, ix, d[5], ps, write) +
+           pml_t<0, 2, 1>(S, bp, zc, iy, ix, d[6], ps, write);
+    a[5] = pml_t<2, 2, 1>(S, bp, zc, iy, ix, d[7], ps, write) +
+           pml_t<1, 2, 1>(S, bp, zc, iy, ix, d[8], ps, write);
 }
 
 // Stiffness rows are read in place.  A wave whose active lanes share one
