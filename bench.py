@@ -220,7 +220,8 @@ def main():
     dominant = "stress" if t_s >= t_v else "velocity"
     achieved = bw_s if dominant == "stress" else bw_v
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_%s_%s_%s.json" % (args.config, p, args.kernel))
+    tfile = os.path.join(ROOT, "profiles", "traffic_%s_%s_%s_%s.json"
+                         % (args.config, p, args.kernel, args.boundary))
     if os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = json.load(fh).get(dominant)
