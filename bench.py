@@ -93,7 +93,8 @@ def main():
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
     ap.add_argument("--config", default=None, choices=["c1", "c0", "c3"],
                     help="workload (default c1 forward; c3 for --mode adjoint)")
-    ap.add_argument("--mode", default="forward", choices=["forward", "adjoint", "sweep"],
+    ap.add_argument("--mode", default="forward",
+                    choices=["forward", "adjoint", "sweep", "inversion"],
                     help="forward: single-shot forward run; adjoint: misfit gradient w.r.t. "
                          "per-grain c-axis angles (forward + checkpointed reverse pass); sweep: "
                          "batched azimuth sweep of the 400x400x200 disk plus 8 smaller disks")
@@ -135,6 +136,8 @@ def main():
         return bench_adjoint(args)
     if args.mode == "sweep":
         return bench_sweep(args)
+    if args.mode == "inversion":
+        return bench_inversion(args)
     if args.config is None:
         args.config = "c1"
     t_setup = time.perf_counter()
@@ -360,7 +363,14 @@ def bench_sweep(args):
     torch.cuda.synchronize()
     t_tune = time.perf_counter() - t1
     t1 = time.perf_counter()
-    traces = batch.run(graph_steps=50)
+    t_prog = [time.perf_counter()]
+
+    def progress(done, total):
+        if time.perf_counter() - t_prog[0] > 20 or done >= total:
+            t_prog[0] = time.perf_counter()
+            log("batched run: %d / %d steps" % (done, total))
+
+    traces = batch.run(graph_steps=50, progress=progress, progress_every=250)
     torch.cuda.synchronize()
     t_run = time.perf_counter() - t1
     t_batched = time.perf_counter() - t0
@@ -417,6 +427,67 @@ def bench_sweep(args):
                                    "small_disk": [round(x, 3) for x in plain_small]},
         "speedup_vs_one_at_a_time": round(t_plain / t_batched, 3),
         "traces_match": match,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_inversion(args):
+    """Fabric inversion over an azimuthal sweep (reduced size of the
+    production inversion: a 9.6 mm x 4.8 mm disk of 200 grains at 0.1 mm,
+    12 azimuths 30 deg apart, 3 mm rim element, 2 MHz Ricker, the default
+    600 steps each).
+    Synthetic data from the Watson fabric alpha = 30 deg, kappa = 5; start
+    alpha = 70 deg, kappa = 1.5; L-BFGS-B on the float64 misfit with the
+    adjoint gradient (fabric_fit.py), at most 30 iterations.  value =
+    forward-equivalent cell-updates per second over the whole fit."""
+    import math
+
+    import numpy as np
+    import torch
+
+    from pulse_echo_azimuthal_simulation_amd import fabric_fit as F, host, sweep
+
+    nt = args.steps if args.steps != 12000 else 600
+    h = 1e-4
+    G = 200
+    build = sweep.disk_build(9.6e-3, 4.8e-3, h, G, seed=21)
+    lat = F.latents(G, 22)
+    co = host.optimised_coeffs(8, kh_max=2.0, multistart=True)
+    azs = [30.0 * k for k in range(12)]
+    t0 = time.perf_counter()
+    inv = F.FabricInversion(build, azs, h, co, 2e6, nt=nt, lat=lat, sponge=10,
+                            checkpoint_every=100, precision=args.precision, element_d_mm=3.0)
+    truth = (math.radians(30.0), 5.0)
+    data = [t.clone() for t in inv.traces(truth)]
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    p0 = (math.radians(70.0), 1.5)
+    t1 = time.perf_counter()
+    p, J, hist, res = inv.fit(p0, data, max_iter=30, log=log)
+    t_fit = time.perf_counter() - t1
+    cells = int(np.prod(inv.shots[0].shape))
+    evals = len(hist)
+    value = evals * len(azs) * cells * nt / t_fit / 1e9
+    a_fit = math.degrees(p[0]) % 180.0
+    out = {
+        "metric": METRIC, "value": round(value, 4),
+        "unit": "Gcell-updates/s (forward-equivalent, whole fit)",
+        "n_gpus": 1, "steps": nt, "warmup": 0,
+        "ms_per_step": round(t_fit / (evals * len(azs) * nt) * 1e3, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": args.precision, "data": "synthetic",
+        "config": {"workload": "fabric inversion: 9.6x4.8 mm disk, %d grains, grid %s, %d azimuths, "
+                               "%d steps, Watson truth alpha 30 deg kappa 5, start 70 deg / 1.5"
+                               % (G, list(inv.shots[0].shape), len(azs), nt),
+                   "mode": "inversion"},
+        "truth": {"alpha_deg": 30.0, "kappa": 5.0, "a1": round(F.watson_eigenvalue(5.0), 4)},
+        "recovered": {"alpha_deg": round(a_fit, 3), "kappa": round(float(p[1]), 4),
+                      "a1": round(F.watson_eigenvalue(float(p[1])), 4)},
+        "misfit": {"start": hist[0]["J"], "final": J}, "evaluations": evals,
+        "iterations": int(res.nit), "seconds": {"setup": round(t_setup, 2),
+                                                 "fit": round(t_fit, 2)},
+        "history": [[round(math.degrees(e["alpha"]), 3), round(e["kappa"], 4), e["J"]]
+                    for e in hist],
     }
     print(json.dumps(out), flush=True)
 

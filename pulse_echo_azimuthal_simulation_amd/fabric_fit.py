@@ -1,0 +1,209 @@
+"""Full-waveform fabric inversion over an azimuthal pulse-echo sweep.
+
+The upstream fits its fabric (mean c-axis azimuth and Watson concentration)
+by Nelder-Mead on trace observables through a fast forward model
+(sim/model/fit_fabric.py:224-320, fit_sweep.py:356-560).  Here the fabric
+parameters are fitted to the full traces of every azimuth by a
+quasi-Newton method whose gradient is exact: the discrete adjoint of the
+GPU solver gives dJ/dC for every table row, and the chain to the fabric
+parameters runs through a deterministic reparameterisation of the Watson
+sample.
+
+Reparameterisation.  Grain g keeps fixed latent numbers (u_g, psi_g) drawn
+once from U(0, 1) x U(0, 2 pi).  Its c-axis for the fabric (alpha, kappa)
+(cluster axis mu = (cos alpha, sin alpha, 0) in the disk plane,
+concentration kappa; kappa > 0 clusters, kappa < 0 girdles) is
+
+    a_g = t_g mu + sqrt(1 - t_g^2) (cos psi_g e_z + sin psi_g (mu x e_z))
+
+with t_g the u_g-quantile of |mu . a|, whose density on [0, 1] under the
+Watson law p(a) ~ exp(kappa (mu . a)^2) is exp(kappa t^2) / Z(kappa): so the
+a_g are an exact Watson sample for every (alpha, kappa), and smooth in both.
+The quantile is found by safeguarded Newton iteration on the CDF, itself
+evaluated by 64-point Gauss-Legendre quadrature (exact to rounding for
+these smooth integrands).  The eigenvalue reported beside kappa is
+E[(mu . a)^2].
+
+Per azimuth the solver grid, footprint, time step and step count are fixed
+(the time step is the stable step of a fabric whose every grain has its
+largest diagonal stiffness C33 along a grid axis, which bounds every
+fabric), so only the stiffness table changes with the fabric.  The misfit
+J = 1/2 sum over azimuths, steps and receivers of (y - d)^2 is accumulated
+in float64; dJ/d(alpha, kappa) = sum over azimuths of <dJ/dC_az, dC_az/dp>
+with dC_az/dp by central differences of the float64 host table map.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+from . import materials
+
+_GLX, _GLW = np.polynomial.legendre.leggauss(64)
+_GLX = 0.5 * (_GLX + 1.0)           # nodes on [0, 1]
+_GLW = 0.5 * _GLW
+
+
+def _partial(t, kappa):
+    """int_0^t exp(kappa s^2) ds for an array t in [0, 1]."""
+    t = np.asarray(t, float)
+    s = t[..., None] * _GLX
+    return t * (np.exp(kappa * s * s) @ _GLW)
+
+
+def watson_cos(u, kappa):
+    """t with P(|mu . a| <= t) = u under Watson(kappa), for an array u."""
+    u = np.clip(np.asarray(u, float), 0.0, 1.0)
+    Z = float(_partial(np.array([1.0]), kappa)[0])
+    lo = np.zeros_like(u)
+    hi = np.ones_like(u)
+    t = u.copy()
+    for _ in range(60):
+        F = _partial(t, kappa) / Z - u
+        lo = np.where(F < 0, t, lo)
+        hi = np.where(F >= 0, t, hi)
+        dF = np.exp(kappa * t * t) / Z
+        tn = t - F / dF
+        bad = (tn <= lo) | (tn >= hi) | ~np.isfinite(tn)
+        tn = np.where(bad, 0.5 * (lo + hi), tn)
+        if np.max(np.abs(tn - t)) < 1e-16:
+            t = tn
+            break
+        t = tn
+    return t
+
+
+def watson_eigenvalue(kappa) -> float:
+    """E[(mu . a)^2] under Watson(kappa)."""
+    Z = float(_partial(np.array([1.0]), kappa)[0])
+    return float((_GLX ** 2 * np.exp(kappa * _GLX ** 2)) @ _GLW / Z)
+
+
+def watson_axes(alpha, kappa, u, psi):
+    """(G, 3) c-axes of the reparameterised Watson sample (module docstring)."""
+    t = watson_cos(u, kappa)
+    st = np.sqrt(np.maximum(0.0, 1.0 - t * t))
+    mu = np.array([math.cos(alpha), math.sin(alpha), 0.0])
+    ez = np.array([0.0, 0.0, 1.0])
+    e2 = np.cross(mu, ez)
+    psi = np.asarray(psi, float)
+    return (t[:, None] * mu + st[:, None] * (np.cos(psi)[:, None] * ez
+                                              + np.sin(psi)[:, None] * e2))
+
+
+def latents(n_grains, seed):
+    """Fixed per-grain latent numbers (u, psi)."""
+    rng = np.random.default_rng(seed)
+    return rng.uniform(0.0, 1.0, n_grains), rng.uniform(0.0, 2.0 * math.pi, n_grains)
+
+
+def _rz(az_deg):
+    a = math.radians(az_deg)
+    ca, sa = math.cos(a), math.sin(a)
+    return np.array([[ca, -sa, 0.0], [sa, ca, 0.0], [0.0, 0.0, 1.0]])
+
+
+class FabricInversion:
+    """Forward and gradient of the sweep misfit in the fabric parameters.
+
+    ``build``: a disk in the upstream build layout (sweep.disk_build);
+    ``azimuths`` in degrees; the shots use the sweep preparation of
+    ``sweep.GpuAzimuthPrep`` (rim transducer, sponge, couplant mask).
+    """
+
+    def __init__(self, build, azimuths, h, coeffs, f0, nt, lat, sponge=10,
+                 checkpoint_every=100, precision="f32", element_d_mm=6.35):
+        from . import adjoint, sweep
+        self.azimuths = [float(a) for a in azimuths]
+        self.lat = lat
+        self.nt = int(nt)
+        self.checkpoint_every = int(checkpoint_every)
+        prep = sweep.GpuAzimuthPrep(build, h, coeffs, f0, sponge=sponge,
+                                    element_d_mm=element_d_mm)
+        G = len(lat[0])
+        zaxes = np.tile([0.0, 0.0, 1.0], (G, 1))
+        self.shots, self.wavs, self.rot, self.adj = [], [], [], []
+        for az in self.azimuths:
+            R = _rz(az)
+            bound = materials.material_tables(zaxes @ R)
+            sh, wav, _, _ = prep.shot(az, precision=precision, nt=self.nt, tables=bound)
+            # every stiffness term active: the bound tables have zeros that
+            # a general fabric fills
+            sh.desc.act_mask = (1 << 36) - 1
+            self.shots.append(sh)
+            self.wavs.append(wav)
+            self.rot.append(R)
+            self.adj.append(adjoint.DeviceAdjoint(sh, n_proj=0))
+        self.torch = self.shots[0].torch
+        self.evals = []
+
+    def tables64(self, p, k):
+        """float64 (n_mat, 36) table of azimuth k for fabric p = (alpha, kappa)."""
+        axes = watson_axes(p[0], p[1], *self.lat)
+        return materials.material_tables(axes @ self.rot[k], dtype=np.float64)[0]
+
+    def _set(self, k, C):
+        sh = self.shots[k]
+        sh.Ct.copy_(self.torch.as_tensor(np.asarray(C).reshape(sh.Ct.shape), device=sh.device))
+
+    def traces(self, p):
+        """Device traces of every azimuth for fabric p."""
+        out = []
+        for k, sh in enumerate(self.shots):
+            self._set(k, self.tables64(p, k))
+            sh.reset()
+            tr = sh.new_traces(self.nt)
+            sh.run(self.wavs[k], 0, self.nt, tr)
+            out.append(tr)
+        return out
+
+    def misfit_gradient(self, p, data, h_alpha=1e-6, h_kappa=1e-6):
+        """(J, dJ/d(alpha, kappa)) in float64 for the fabric p."""
+        from . import adjoint
+        torch = self.torch
+        J = 0.0
+        g = np.zeros(2)
+        steps = (np.array([h_alpha, 0.0]), np.array([0.0, h_kappa * max(1.0, abs(p[1]))]))
+        for k, sh in enumerate(self.shots):
+            self._set(k, self.tables64(p, k))
+            sh.reset()
+            d = data[k]
+
+            def resid(tr, d=d):
+                r = tr - d
+                return 0.5 * float((r.double() ** 2).sum()), r
+
+            Jk, gC, _ = adjoint.misfit_gradient(sh, self.wavs[k], self.nt, resid,
+                                                checkpoint_every=self.checkpoint_every,
+                                                adjoint=self.adj[k])
+            gC = gC.cpu().numpy()
+            for j, dp in enumerate(steps):
+                hp = np.asarray(p, float) + dp
+                hm = np.asarray(p, float) - dp
+                dC = (self.tables64(hp, k) - self.tables64(hm, k)) / (2.0 * dp[j])
+                g[j] += float(np.sum(gC * dC))
+            J += Jk
+        torch.cuda.synchronize()
+        return J, g
+
+    def fit(self, p0, data, max_iter=30, bounds=((None, None), (-20.0, 30.0)), log=None):
+        """L-BFGS-B from p0; returns (p, J, history)."""
+        from scipy.optimize import minimize
+        hist = []
+        t0 = time.perf_counter()
+
+        def fun(x):
+            J, g = self.misfit_gradient(x, data)
+            hist.append(dict(alpha=float(x[0]), kappa=float(x[1]), J=J,
+                             g=[float(g[0]), float(g[1])], t=time.perf_counter() - t0))
+            if log:
+                log("eval %d: alpha %.3f deg, kappa %.3f, J %.6e"
+                    % (len(hist), math.degrees(x[0]), x[1], J))
+            return J, g
+
+        res = minimize(fun, np.asarray(p0, float), jac=True, method="L-BFGS-B",
+                       bounds=bounds, options=dict(maxiter=int(max_iter), ftol=1e-14,
+                                                   gtol=1e-12))
+        return res.x, float(res.fun), hist, res

@@ -114,8 +114,10 @@ def rim_footprint(lab, nd, h, element_d_mm=6.35):
     Rc = nd // 2
     ixp = next(cx + k - 1 for k in range(Rc - 1, 0, -1) if lab[cz, cy, cx + k] >= 0)
     er = max(int(element_d_mm * 1e-3 / 2 / h), 1)
+    # cells beyond the grid (an element wider than the disk) are left out
     pts = [(cz + dz, cy + dy, ixp) for dy in range(-er, er + 1) for dz in range(-er, er + 1)
-           if dy * dy + dz * dz <= er * er and lab[cz + dz, cy + dy, ixp] >= 0]
+           if dy * dy + dz * dz <= er * er and 0 <= cz + dz < nz and 0 <= cy + dy < ny
+           and lab[cz + dz, cy + dy, ixp] >= 0]
     return pts, 1.0 / len(pts)
 
 
@@ -276,7 +278,8 @@ class GpuAzimuthPrep:
         plane = lab[:, :, ixp].cpu().numpy()
         er = max(int(self.element_d_mm * 1e-3 / 2 / self.h), 1)
         pts = [(cz + dz, cy + dy, ixp) for dy in range(-er, er + 1) for dz in range(-er, er + 1)
-               if dy * dy + dz * dz <= er * er and plane[cz + dz, cy + dy] >= 0]
+               if dy * dy + dz * dz <= er * er and 0 <= cz + dz < nz and 0 <= cy + dy < ny
+               and plane[cz + dz, cy + dy] >= 0]
         w = 1.0 / len(pts)
         shape = tuple(lab.shape)
         src_idx = np.array([np.ravel_multi_index(p, shape) for p in pts], np.int64)
@@ -340,8 +343,10 @@ class ShotBatch:
         _abi.check(self.lib.pe_batch_step(C.byref(self.desc), int(k), self._stream()),
                    "pe_batch_step")
 
-    def run(self, graph_steps=32):
-        """Run every shot to its step count; returns the per-shot traces."""
+    def run(self, graph_steps=32, progress=None, progress_every=1000):
+        """Run every shot to its step count; returns the per-shot traces.
+        ``progress(done, total)`` is called on the host about every
+        ``progress_every`` steps (after a stream synchronisation)."""
         torch = self.torch
         self.setup()
         total = max(self.nts) if self.nts else 0
@@ -356,10 +361,15 @@ class ShotBatch:
                                                C.byref(g)), "pe_batch_graph")
             self._graph, self._graph_steps = g, k
         done = 0
+        next_report = progress_every
         while done < total:
             _abi.check(self.lib.pe_graph_launch(self._graph, stream.cuda_stream),
                        "pe_graph_launch")
             done += k
+            if progress is not None and (done >= next_report or done >= total):
+                stream.synchronize()
+                progress(min(done, total), total)
+                next_report += progress_every
         cur.wait_stream(stream)
         return self.traces
 
