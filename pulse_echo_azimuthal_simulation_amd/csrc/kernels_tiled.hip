@@ -35,7 +35,7 @@ using namespace tiled;
 template <typename R, int TX, int TY, int PAD, bool STORE_G = false, bool FR = false,
           bool PML = false>
 __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x, int tiles_y,
-                                                  int zc_len, int t)
+                                                  int zc_len, int t, int zlo, int zhi)
 {
     using T = StressTile<R, TX, TY, PAD, FR>;
     constexpr int GX = T::GX, GY = T::GY, NG = T::NG, VX = T::VX, VY = T::VY, NV = T::NV;
@@ -51,7 +51,7 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
     const int sy = nx, sz = nx * ny;
     const int tx = t % tiles_x, ty = (t / tiles_x) % tiles_y, chunk = t / (tiles_x * tiles_y);
     const int x0 = tx * TX, y0 = ty * TY;
-    const int z0 = chunk * zc_len, z1 = min(nz, z0 + zc_len);
+    const int z0 = zlo + chunk * zc_len, z1 = min(zhi, z0 + zc_len);
     // every strain column at least 4 cells (every output 1 cell) from the x/y edges
     const bool inner_xy = x0 >= 5 && x0 + TX <= nx - 5 && y0 >= 5 && y0 + TY <= ny - 5;
     const bool full = S.act == 0xFFFFFFFFFull;
@@ -399,7 +399,7 @@ __device__ __forceinline__ void accel_col_pml(const Shot<R> &S, const R *bp, con
 // (vector memory counters retire in order).
 template <typename R, int TX, int TY, bool RHO_LDS = false, bool PML = false>
 __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_x, int tiles_y,
-                                                    int zc_len, int t)
+                                                    int zc_len, int t, int zlo, int zhi)
 {
     using T = VelTile<R, TX, TY>;
     constexpr int VX = T::VX, NV = T::NV, NT = T::NT, NF = (T::NFRAME + NT - 1) / NT;
@@ -424,7 +424,7 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
     const int sy = nx, sz = nx * ny;
     const int tx = t % tiles_x, ty = (t / tiles_x) % tiles_y, chunk = t / (tiles_x * tiles_y);
     const int x0 = tx * TX, y0 = ty * TY;
-    const int z0 = chunk * zc_len, z1 = min(nz, z0 + zc_len);
+    const int z0 = zlo + chunk * zc_len, z1 = min(zhi, z0 + zc_len);
     const bool inner_xy = x0 >= 4 && x0 + TX <= nx - 4 && y0 >= 4 && y0 + TY <= ny - 4;
 
     const int tid = threadIdx.x;
@@ -601,23 +601,23 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
 // their occupancy).
 template <typename R, int TX, int TY>
 __device__ __forceinline__ bool stress_band_tile(const Shot<R> &S, int t, int tiles_x,
-                                                 int tiles_y, int zc_len)
+                                                 int tiles_y, int zc_len, int zlo, int zhi)
 {
     // strain columns x0-1 .. x0+TX, y0-1 .. y0+TY; planes z0-1 .. z1
     const int L = S.pml;
     const int x0 = (t % tiles_x) * TX, y0 = ((t / tiles_x) % tiles_y) * TY;
-    const int z0 = (t / (tiles_x * tiles_y)) * zc_len, z1 = min(S.nz, z0 + zc_len);
+    const int z0 = zlo + (t / (tiles_x * tiles_y)) * zc_len, z1 = min(zhi, z0 + zc_len);
     return x0 - 1 <= L || x0 + TX >= S.nx - 1 - L || y0 - 1 <= L || y0 + TY >= S.ny - 1 - L ||
            z0 - 1 <= L || z1 >= S.nz - 1 - L;
 }
 
 template <typename R, int TX, int TY>
 __device__ __forceinline__ bool velocity_band_tile(const Shot<R> &S, int t, int tiles_x,
-                                                   int tiles_y, int zc_len)
+                                                   int tiles_y, int zc_len, int zlo, int zhi)
 {
     const int L = S.pml;
     const int x0 = (t % tiles_x) * TX, y0 = ((t / tiles_x) % tiles_y) * TY;
-    const int z0 = (t / (tiles_x * tiles_y)) * zc_len, z1 = min(S.nz, z0 + zc_len);
+    const int z0 = zlo + (t / (tiles_x * tiles_y)) * zc_len, z1 = min(zhi, z0 + zc_len);
     return x0 <= L || x0 + TX - 1 >= S.nx - 1 - L || y0 <= L || y0 + TY - 1 >= S.ny - 1 - L ||
            z0 <= L || z1 - 1 >= S.nz - 1 - L;
 }
@@ -626,22 +626,26 @@ template <typename R, int TX, int TY, int PAD, bool STORE_G = false, bool FR = f
           bool PML = false>
 __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
                              (PML ? 1 : MinWaves<R, StressTile<R, TX, TY, PAD>::NT>::v)) void
-k_stress_tiled(Shot<R> S, int tiles_x, int tiles_y, int zc_len)
+k_stress_tiled(Shot<R> S, int tiles_x, int tiles_y, int zc_len, int zlo, int zhi)
 {
-    const int nb = tiles_x * tiles_y * ((S.nz + zc_len - 1) / zc_len);
+    const int nb = tiles_x * tiles_y * ((zhi - zlo + zc_len - 1) / zc_len);
     const int t = xcd_remap(blockIdx.x, nb);
-    if (S.pml > 0 && stress_band_tile<R, TX, TY>(S, t, tiles_x, tiles_y, zc_len) != PML) return;
-    stress_tiled_body<R, TX, TY, PAD, STORE_G, FR, PML>(S, tiles_x, tiles_y, zc_len, t);
+    if (S.pml > 0 &&
+        stress_band_tile<R, TX, TY>(S, t, tiles_x, tiles_y, zc_len, zlo, zhi) != PML)
+        return;
+    stress_tiled_body<R, TX, TY, PAD, STORE_G, FR, PML>(S, tiles_x, tiles_y, zc_len, t, zlo, zhi);
 }
 
 template <typename R, int TX, int TY, bool RHO_LDS = false, bool PML = false>
 __global__ __launch_bounds__(TX *TY, (PML ? 1 : MinWaves<R, TX * TY>::v)) void k_velocity_tiled(
-    Shot<R> S, int tiles_x, int tiles_y, int zc_len)
+    Shot<R> S, int tiles_x, int tiles_y, int zc_len, int zlo, int zhi)
 {
-    const int nb = tiles_x * tiles_y * ((S.nz + zc_len - 1) / zc_len);
+    const int nb = tiles_x * tiles_y * ((zhi - zlo + zc_len - 1) / zc_len);
     const int t = xcd_remap(blockIdx.x, nb);
-    if (S.pml > 0 && velocity_band_tile<R, TX, TY>(S, t, tiles_x, tiles_y, zc_len) != PML) return;
-    velocity_tiled_body<R, TX, TY, RHO_LDS, PML>(S, tiles_x, tiles_y, zc_len, t);
+    if (S.pml > 0 &&
+        velocity_band_tile<R, TX, TY>(S, t, tiles_x, tiles_y, zc_len, zlo, zhi) != PML)
+        return;
+    velocity_tiled_body<R, TX, TY, RHO_LDS, PML>(S, tiles_x, tiles_y, zc_len, t, zlo, zhi);
 }
 
 // Multi-shot kernels: the workgroups of every shot of a batch in one launch.
@@ -659,7 +663,7 @@ __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
     if (step[s] >= e->nt) return;
     const Shot<R> S = e->S;
     stress_tiled_body<R, TX, TY, PAD, false, FR>(S, e->tiles_x[0], e->tiles_y[0], e->zc_len[0],
-                                                 g - e->blk_off[0]);
+                                                 g - e->blk_off[0], 0, S.nz);
 }
 
 template <typename R, int TX, int TY, bool RHO_LDS = false>
@@ -673,7 +677,18 @@ __global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_velocity_
     if (step[s] >= e->nt) return;
     const Shot<R> S = e->S;
     velocity_tiled_body<R, TX, TY, RHO_LDS>(S, e->tiles_x[1], e->tiles_y[1], e->zc_len[1],
-                                            g - e->blk_off[1]);
+                                            g - e->blk_off[1], 0, S.nz);
+}
+
+// CPML z ranges: the middle planes [m0, m1), whose tiles lie in a band only
+// through x or y, and the two slabs that contain the z bands.  A stress tile
+// of planes z0..z1-1 computes strains on z0-1 .. z1, so its middle range is
+// [L+2, nz-L-2); a velocity tile touches only its own planes: [L+1, nz-L-1).
+inline void pml_zranges(int nz, int L, int half, int &m0, int &m1)
+{
+    m0 = half == 0 ? L + 2 : L + 1;
+    m1 = half == 0 ? nz - L - 2 : nz - L - 1;
+    if (m1 <= m0) m0 = m1 = nz;                  // no band-free middle
 }
 
 template <typename R, int TX, int TY, int PAD, bool FR = false, bool PML = false>
@@ -681,32 +696,32 @@ hipError_t stress_launch(const Shot<R> &s, int zchunk, hipStream_t st, bool stor
 {
     using T = StressTile<R, TX, TY, PAD, FR>;
     const int tiles_x = (s.nx + TX - 1) / TX, tiles_y = (s.ny + TY - 1) / TY;
-    const int zl = chunk_len(s.nz, tiles_x * tiles_y, zchunk);
-    const int nb = tiles_x * tiles_y * ((s.nz + zl - 1) / zl);
+    auto run = [&](auto kern, size_t lds, int lo, int hi) -> hipError_t {
+        if (hi <= lo) return hipSuccess;
+        const int zl = chunk_len(hi - lo, tiles_x * tiles_y, zchunk);
+        const int nb = tiles_x * tiles_y * ((hi - lo + zl - 1) / zl);
+        hipError_t e = allow_lds(kern, lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(T::NT), lds, st, s, tiles_x, tiles_y, zl, lo, hi);
+        return hipGetLastError();
+    };
     if constexpr (PML) {
         if (store_g) return hipErrorInvalidValue;   // the strain history serves the adjoint only
         const size_t lds_pml = T::LDS + sizeof(R) * 18 * (2 * (size_t)s.pml + 2);
-        hipError_t e = allow_lds(k_stress_tiled<R, TX, TY, PAD, false, FR, false>, T::LDS);
-        if (e == hipSuccess) e = allow_lds(k_stress_tiled<R, TX, TY, PAD, false, FR, true>, lds_pml);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_stress_tiled<R, TX, TY, PAD, false, FR, false>), dim3(nb),
-                           dim3(T::NT), T::LDS, st, s, tiles_x, tiles_y, zl);
-        hipLaunchKernelGGL((k_stress_tiled<R, TX, TY, PAD, false, FR, true>), dim3(nb),
-                           dim3(T::NT), lds_pml, st, s, tiles_x, tiles_y, zl);
-        return hipGetLastError();
+        int m0, m1;
+        pml_zranges(s.nz, s.pml, 0, m0, m1);
+        // band-free middle tiles (plain kernel), the middle tiles in an x/y
+        // band, then the two slabs holding the z bands (CPML kernel)
+        hipError_t e = run(k_stress_tiled<R, TX, TY, PAD, false, FR, false>, T::LDS, m0, m1);
+        if (e == hipSuccess) e = run(k_stress_tiled<R, TX, TY, PAD, false, FR, true>, lds_pml, m0, m1);
+        if (e == hipSuccess)
+            e = run(k_stress_tiled<R, TX, TY, PAD, false, FR, true>, lds_pml, 0, m0 < m1 ? m0 : s.nz);
+        if (e == hipSuccess && m0 < m1)
+            e = run(k_stress_tiled<R, TX, TY, PAD, false, FR, true>, lds_pml, m1, s.nz);
+        return e;
     }
-    if (store_g) {
-        hipError_t e = allow_lds(k_stress_tiled<R, TX, TY, PAD, true, FR>, T::LDS);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_stress_tiled<R, TX, TY, PAD, true, FR>), dim3(nb), dim3(T::NT),
-                           T::LDS, st, s, tiles_x, tiles_y, zl);
-        return hipGetLastError();
-    }
-    hipError_t e = allow_lds(k_stress_tiled<R, TX, TY, PAD, false, FR>, T::LDS);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_stress_tiled<R, TX, TY, PAD, false, FR>), dim3(nb), dim3(T::NT), T::LDS,
-                       st, s, tiles_x, tiles_y, zl);
-    return hipGetLastError();
+    if (store_g) return run(k_stress_tiled<R, TX, TY, PAD, true, FR>, T::LDS, 0, s.nz);
+    return run(k_stress_tiled<R, TX, TY, PAD, false, FR>, T::LDS, 0, s.nz);
 }
 
 template <typename R, int TX, int TY, bool PML = false>
@@ -714,36 +729,41 @@ hipError_t velocity_launch(const Shot<R> &s, int zchunk, hipStream_t st)
 {
     using T = VelTile<R, TX, TY>;
     const int tiles_x = (s.nx + TX - 1) / TX, tiles_y = (s.ny + TY - 1) / TY;
-    const int zl = chunk_len(s.nz, tiles_x * tiles_y, zchunk);
-    const int nb = tiles_x * tiles_y * ((s.nz + zl - 1) / zl);
-    const size_t rho_bytes = sizeof(R) * (size_t)s.n_mat;
-    if (rho_bytes <= (size_t)kRhoLdsBytes) {
-        const size_t lds = T::LDS + rho_bytes;
-        hipError_t e = allow_lds(k_velocity_tiled<R, TX, TY, true>, lds);
+    auto run = [&](auto kern, size_t lds, int lo, int hi) -> hipError_t {
+        if (hi <= lo) return hipSuccess;
+        const int zl = chunk_len(hi - lo, tiles_x * tiles_y, zchunk);
+        const int nb = tiles_x * tiles_y * ((hi - lo + zl - 1) / zl);
+        hipError_t e = allow_lds(kern, lds);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_velocity_tiled<R, TX, TY, true>), dim3(nb), dim3(T::NT), lds, st, s,
-                           tiles_x, tiles_y, zl);
-        if constexpr (PML) {
-            const size_t lds_pml = lds + sizeof(R) * 18 * (2 * (size_t)s.pml + 2);
-            e = allow_lds(k_velocity_tiled<R, TX, TY, true, true>, lds_pml);
-            if (e != hipSuccess) return e;
-            hipLaunchKernelGGL((k_velocity_tiled<R, TX, TY, true, true>), dim3(nb), dim3(T::NT),
-                               lds_pml, st, s, tiles_x, tiles_y, zl);
-        }
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(T::NT), lds, st, s, tiles_x, tiles_y, zl, lo, hi);
         return hipGetLastError();
-    }
-    hipError_t e = allow_lds(k_velocity_tiled<R, TX, TY>, T::LDS);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_velocity_tiled<R, TX, TY>), dim3(nb), dim3(T::NT), T::LDS, st, s,
-                       tiles_x, tiles_y, zl);
+    };
+    const size_t rho_bytes = sizeof(R) * (size_t)s.n_mat;
+    const bool rho_lds = rho_bytes <= (size_t)kRhoLdsBytes;
+    const size_t lds = T::LDS + (rho_lds ? rho_bytes : 0);
     if constexpr (PML) {
-        const size_t lds_pml = T::LDS + sizeof(R) * 18 * (2 * (size_t)s.pml + 2);
-        e = allow_lds(k_velocity_tiled<R, TX, TY, false, true>, lds_pml);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_velocity_tiled<R, TX, TY, false, true>), dim3(nb), dim3(T::NT),
-                           lds_pml, st, s, tiles_x, tiles_y, zl);
+        const size_t lds_pml = lds + sizeof(R) * 18 * (2 * (size_t)s.pml + 2);
+        int m0, m1;
+        pml_zranges(s.nz, s.pml, 1, m0, m1);
+        const int lo_end = m0 < m1 ? m0 : s.nz;
+        hipError_t e;
+        if (rho_lds) {
+            e = run(k_velocity_tiled<R, TX, TY, true>, lds, m0, m1);
+            if (e == hipSuccess) e = run(k_velocity_tiled<R, TX, TY, true, true>, lds_pml, m0, m1);
+            if (e == hipSuccess) e = run(k_velocity_tiled<R, TX, TY, true, true>, lds_pml, 0, lo_end);
+            if (e == hipSuccess && m0 < m1)
+                e = run(k_velocity_tiled<R, TX, TY, true, true>, lds_pml, m1, s.nz);
+        } else {
+            e = run(k_velocity_tiled<R, TX, TY>, lds, m0, m1);
+            if (e == hipSuccess) e = run(k_velocity_tiled<R, TX, TY, false, true>, lds_pml, m0, m1);
+            if (e == hipSuccess) e = run(k_velocity_tiled<R, TX, TY, false, true>, lds_pml, 0, lo_end);
+            if (e == hipSuccess && m0 < m1)
+                e = run(k_velocity_tiled<R, TX, TY, false, true>, lds_pml, m1, s.nz);
+        }
+        return e;
     }
-    return hipGetLastError();
+    if (rho_lds) return run(k_velocity_tiled<R, TX, TY, true>, lds, 0, s.nz);
+    return run(k_velocity_tiled<R, TX, TY>, lds, 0, s.nz);
 }
 
 template <typename R, int TX, int TY, int PAD, bool FR = false>
