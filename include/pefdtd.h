@@ -141,9 +141,11 @@ int pe_run_store_strains(const pe_shot *shot, int32_t n0, int32_t n1, const doub
 /* Tile variants of the tiled kernels, for tuning.  half: 0 stress, 1
  * velocity.  Returns the number of variants at that precision (>= 1) and,
  * when info3 is not NULL, writes (tile x, tile y, flags) of `variant`:
- * flags bits 0-7 = LDS row padding, bit 8 = the stress kernel's frame ring
- * (each v plane's frame fetched together with the z rings' plane and kept
- * in LDS until used); PE_ERANGE when `variant` does not exist.  No upstream
+ * flags bits 0-7 = LDS row padding, bits 8-9 = stress-kernel mode: 1 the
+ * frame ring (each v plane's frame fetched together with the z rings' plane
+ * and kept in LDS until used), 2 the one-barrier pipeline (v tiles double
+ * buffered, strain ring 4 planes deep); PE_ERANGE when `variant` does not
+ * exist.  No upstream
  * counterpart (the upstream kernels have fixed launch shapes). */
 int pe_tile_info(int32_t precision, int32_t half, int32_t variant, int32_t *info3);
 

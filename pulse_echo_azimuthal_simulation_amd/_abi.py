@@ -148,8 +148,8 @@ def load():
 
 
 def tile_variants(precision: str, half: int):
-    """[(tx, ty, pad, frame_ring), ...] of the tiled kernels (half 0 = stress,
-    1 = velocity)."""
+    """[(tx, ty, pad, mode), ...] of the tiled kernels (half 0 = stress,
+    1 = velocity); stress mode 1 = frame ring, 2 = one-barrier pipeline."""
     lib = load()
     p = PE_F32 if precision == "f32" else PE_F64
     info = (C.c_int32 * 3)()
@@ -157,7 +157,7 @@ def tile_variants(precision: str, half: int):
     out = []
     for v in range(n):
         lib.pe_tile_info(p, half, v, info)
-        out.append((int(info[0]), int(info[1]), int(info[2]) & 0xFF, (int(info[2]) >> 8) & 1))
+        out.append((int(info[0]), int(info[1]), int(info[2]) & 0xFF, (int(info[2]) >> 8) & 3))
     return out
 
 

@@ -305,6 +305,203 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
     }
 }
 
+// Stress body with one barrier per step ("pipelined"): three stages of
+// different planes run in the same step, each reading only what an earlier
+// step wrote.  Step zc writes the v tile of plane zc (double-buffered),
+// computes the strains of plane zc-1 from the tile written one step earlier
+// into a 4-plane strain ring, and the stress of plane zc-3 from the strain
+// planes zc-4 .. zc-2.  Same arithmetic as stress_tiled_body, bit for bit;
+// plain tiles (no frame ring, no strain store, no CPML).  LDS:
+// [2][3][NV] v tiles + [6][4][NT] strains.
+template <typename R, int TX, int TY>
+struct PipeTile {
+    using T = StressTile<R, TX, TY, 0>;
+    static constexpr size_t LDS = sizeof(R) * (2 * 3 * T::NV + 6 * 4 * T::NT);
+};
+
+template <typename R, int TX, int TY>
+__device__ __forceinline__ void stress_pipe_body(const Shot<R> &S, int tiles_x, int tiles_y,
+                                                 int zc_len, int t, int zlo, int zhi)
+{
+    using T = StressTile<R, TX, TY, 0>;
+    constexpr int GX = T::GX, GY = T::GY, NG = T::NG, VX = T::VX, VY = T::VY, NV = T::NV;
+    constexpr int NT = T::NT, NF = T::NF, WX = T::WX;
+    extern __shared__ unsigned char lds_raw[];
+    R *vt0 = reinterpret_cast<R *>(lds_raw);      // [2][3][VY][VX]
+    R *gt = vt0 + 2 * 3 * NV;                     // [6][4][NT]: strains by thread
+
+    const int nz = S.nz, ny = S.ny, nx = S.nx;
+    const int sy = nx, sz = nx * ny;
+    const int tx = t % tiles_x, ty = (t / tiles_x) % tiles_y, chunk = t / (tiles_x * tiles_y);
+    const int x0 = tx * TX, y0 = ty * TY;
+    const int z0 = zlo + chunk * zc_len, z1 = min(zhi, z0 + zc_len);
+    const bool inner_xy = x0 >= 5 && x0 + TX <= nx - 5 && y0 >= 5 && y0 + TY <= ny - 5;
+    const bool full = S.act == 0xFFFFFFFFFull;
+    const bool has_damp = S.damp != nullptr;
+
+    const int tid = threadIdx.x;
+    const bool gact = tid < NG;
+    const int gx = gact ? tid % GX : 0, gy = gact ? tid / GX : 0;
+    const int ix = x0 - 1 + gx, iy = y0 - 1 + gy;
+    const bool col_in = gact && ix >= 0 && ix < nx && iy >= 0 && iy < ny;
+    const bool core = col_in && gx >= 1 && gx <= TX && gy >= 1 && gy <= TY;
+    const int col = clampi(iy, 0, ny - 1) * sy + clampi(ix, 0, nx - 1);
+    const int vc = gact ? (gy + 4) * VX + (gx + 4) : NV - 1;
+
+    const R *__restrict__ vx = S.v[0];
+    const R *__restrict__ vy = S.v[1];
+    const R *__restrict__ vz = S.v[2];
+
+    int f_lds[NF], f_xy[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+        const int e = tid + k * NT;
+        int r = 0, c = 0;
+        if (e < 4 * WX) {
+            r = e / WX;
+            c = e % WX;
+        } else if (e < 8 * WX) {
+            r = VY - 4 + (e - 4 * WX) / WX;
+            c = (e - 4 * WX) % WX;
+        } else if (e < 8 * WX + 4 * GY) {
+            r = 4 + (e - 8 * WX) / 4;
+            c = (e - 8 * WX) % 4;
+        } else {
+            r = 4 + (e - 8 * WX - 4 * GY) / 4;
+            c = GX + 4 + (e - 8 * WX - 4 * GY) % 4;
+        }
+        f_lds[k] = e < T::NFRAME ? r * VX + c : NV - 1;
+        f_xy[k] = clampi(y0 - 5 + r, 0, ny - 1) * sy + clampi(x0 - 5 + c, 0, nx - 1);
+    }
+
+    // z rings: after the shift at the top of step zc, r?[k] = plane zc-5+k
+    // (strains of plane zc-1 use r?[0..8]; the tile of plane zc takes
+    // r?[5]); p? = plane zc+3 before the shift.  fq = frame of plane zc.
+    // ns/nm/nd = output operands of plane zc-3 (loaded one step ahead).
+    R rx[9], ry[9], rz[9], px, py, pz, fq[NF][3];
+    const int zs = z0 - 1;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int i = clampi(zs - 6 + k, 0, nz - 1) * sz + col;
+        rx[k] = ldg(vx, i);
+        ry[k] = ldg(vy, i);
+        rz[k] = ldg(vz, i);
+    }
+    {
+        const int i = clampi(zs + 3, 0, nz - 1) * sz + col;
+        px = ldg(vx, i);
+        py = ldg(vy, i);
+        pz = ldg(vz, i);
+    }
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+        const int i = clampi(zs, 0, nz - 1) * sz + f_xy[k];
+        fq[k][0] = ldg(vx, i);
+        fq[k][1] = ldg(vy, i);
+        fq[k][2] = ldg(vz, i);
+    }
+    R ns[6] = {0, 0, 0, 0, 0, 0}, nd = 1;
+    int nm = 0;
+
+    for (int zc = zs; zc <= z1 + 2; ++zc) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            rx[k] = rx[k + 1];
+            ry[k] = ry[k + 1];
+            rz[k] = rz[k + 1];
+        }
+        rx[8] = px;
+        ry[8] = py;
+        rz[8] = pz;
+        // (1) v tile of plane zc (planes zs .. z1)
+        if (zc <= z1) {
+            R *vt = vt0 + (zc & 1) * 3 * NV;
+            vt[0 * NV + vc] = rx[5];
+            vt[1 * NV + vc] = ry[5];
+            vt[2 * NV + vc] = rz[5];
+#pragma unroll
+            for (int k = 0; k < NF; ++k) {
+                vt[0 * NV + f_lds[k]] = fq[k][0];
+                vt[1 * NV + f_lds[k]] = fq[k][1];
+                vt[2 * NV + f_lds[k]] = fq[k][2];
+            }
+        }
+        // this step's output operands (plane zc-3), then the loads for the
+        // next step: operands of plane zc-2, frame of plane zc+1, ring plane
+        // zc+4
+        const int zo = zc - 3;
+        const bool out = core && zo >= z0 && zo < z1;
+        R sv[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) sv[k] = ns[k];
+        const int m = nm;
+        const R dmp = nd;
+        if (core) {
+            const int io1 = clampi(zo + 1, 0, nz - 1) * sz + col;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) ns[k] = ldg(S.s[k], io1);
+            nm = ldg(S.mats, io1);
+            nd = has_damp ? ldg(S.damp, io1) : (R)1;
+        }
+        {
+            const int zf = clampi(zc + 1, 0, nz - 1) * sz;
+#pragma unroll
+            for (int k = 0; k < NF; ++k) {
+                fq[k][0] = ldg(vx, zf + f_xy[k]);
+                fq[k][1] = ldg(vy, zf + f_xy[k]);
+                fq[k][2] = ldg(vz, zf + f_xy[k]);
+            }
+        }
+        {
+            const int i = clampi(zc + 4, 0, nz - 1) * sz + col;
+            px = ldg(vx, i);
+            py = ldg(vy, i);
+            pz = ldg(vz, i);
+        }
+
+        // (2) strains of plane q = zc-1 (planes zs .. z1) from the tile of
+        //     the previous step
+        const int q = zc - 1;
+        if (q >= zs && q <= z1) {
+            const R *vt = vt0 + (q & 1) * 3 * NV;
+            const bool plane_in = q >= 0 && q < nz;
+            R a[6] = {0, 0, 0, 0, 0, 0};
+            if (inner_xy && q >= 4 && q <= nz - 5)
+                strains_col<false, R, VX>(S, vt + vc, vt + NV + vc, vt + 2 * NV + vc, rx, ry, rz,
+                                          ix, iy, q, a);
+            else if (col_in && plane_in)
+                strains_col<true, R, VX>(S, vt + vc, vt + NV + vc, vt + 2 * NV + vc, rx, ry, rz,
+                                         ix, iy, q, a);
+            R *gs = gt + (q & 3) * NT + tid;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) gs[k * 4 * NT] = a[k];
+        }
+
+        // (3) stress of plane zo = zc-3 from the strain planes zo-1 .. zo+1,
+        //     all written by earlier steps
+        if (out) {
+            const int sl[3] = {(zo - 1) & 3, zo & 3, (zo + 1) & 3};
+            const int io = clampi(zo, 0, nz - 1) * sz + col;
+            const bool fast = full && inner_xy && zo >= 1 && zo <= nz - 2;
+            const int mu = __builtin_amdgcn_readfirstlane(m);
+            if (fast && __all(m == mu)) {
+                ConstRow<R> cm = (ConstRow<R>)(S.Ct + (size_t)mu * 36);
+                stress_cell<false, true, R, GX, NT, ConstRow<R>, 4>(S, gt, sl, tid, zo, iy, ix, cm,
+                                                                     dmp, sv);
+            } else if (fast) {
+                stress_cell<false, true, R, GX, NT, const R *, 4>(S, gt, sl, tid, zo, iy, ix,
+                                                                   S.Ct + (size_t)m * 36, dmp, sv);
+            } else {
+                stress_cell<true, false, R, GX, NT, const R *, 4>(S, gt, sl, tid, zo, iy, ix,
+                                                                   S.Ct + (size_t)m * 36, dmp, sv);
+            }
+#pragma unroll
+            for (int k = 0; k < 6; ++k) stg(S.s[k], io, sv[k]);
+        }
+        lds_barrier();
+    }
+}
+
 // ---------------------------------------------------------------- velocity
 template <typename R, int TX, int TY>
 struct VelTile {
@@ -648,6 +845,15 @@ __global__ __launch_bounds__(TX *TY, (PML ? 1 : MinWaves<R, TX * TY>::v)) void k
     velocity_tiled_body<R, TX, TY, RHO_LDS, PML>(S, tiles_x, tiles_y, zc_len, t, zlo, zhi);
 }
 
+template <typename R, int TX, int TY>
+__global__ __launch_bounds__((StressTile<R, TX, TY, 0>::NT),
+                             (MinWaves<R, StressTile<R, TX, TY, 0>::NT>::v)) void
+k_stress_pipe(Shot<R> S, int tiles_x, int tiles_y, int zc_len, int zlo, int zhi)
+{
+    const int nb = tiles_x * tiles_y * ((zhi - zlo + zc_len - 1) / zc_len);
+    stress_pipe_body<R, TX, TY>(S, tiles_x, tiles_y, zc_len, xcd_remap(blockIdx.x, nb), zlo, zhi);
+}
+
 // Multi-shot kernels: the workgroups of every shot of a batch in one launch.
 // blk_shot maps a (remapped) workgroup id to its shot; a shot whose step
 // counter has reached its step count exits at once.
@@ -664,6 +870,21 @@ __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
     const Shot<R> S = e->S;
     stress_tiled_body<R, TX, TY, PAD, false, FR>(S, e->tiles_x[0], e->tiles_y[0], e->zc_len[0],
                                                  g - e->blk_off[0], 0, S.nz);
+}
+
+template <typename R, int TX, int TY>
+__global__ __launch_bounds__((StressTile<R, TX, TY, 0>::NT),
+                             (MinWaves<R, StressTile<R, TX, TY, 0>::NT>::v)) void
+k_stress_pipe_batched(const BatchEntry<R> *__restrict__ tab, const int32_t *__restrict__ blk_shot,
+                      int nb_total, const int32_t *__restrict__ step)
+{
+    const int g = xcd_remap(blockIdx.x, nb_total);
+    const int s = blk_shot[g];
+    const BatchEntry<R> *e = tab + s;
+    if (step[s] >= e->nt) return;
+    const Shot<R> S = e->S;
+    stress_pipe_body<R, TX, TY>(S, e->tiles_x[0], e->tiles_y[0], e->zc_len[0], g - e->blk_off[0],
+                                0, S.nz);
 }
 
 template <typename R, int TX, int TY, bool RHO_LDS = false>
@@ -724,6 +945,24 @@ hipError_t stress_launch(const Shot<R> &s, int zchunk, hipStream_t st, bool stor
     return run(k_stress_tiled<R, TX, TY, PAD, false, FR>, T::LDS, 0, s.nz);
 }
 
+// Pipelined stress tile (stress_pipe_body); the strain-history store runs
+// the plain kernel of the same tile.
+template <typename R, int TX, int TY>
+hipError_t stress_pipe_launch(const Shot<R> &s, int zchunk, hipStream_t st, bool store_g)
+{
+    if (store_g) return stress_launch<R, TX, TY, 0>(s, zchunk, st, true);
+    using T = StressTile<R, TX, TY, 0>;
+    const int tiles_x = (s.nx + TX - 1) / TX, tiles_y = (s.ny + TY - 1) / TY;
+    const int zl = chunk_len(s.nz, tiles_x * tiles_y, zchunk);
+    const int nb = tiles_x * tiles_y * ((s.nz + zl - 1) / zl);
+    const size_t lds = PipeTile<R, TX, TY>::LDS;
+    hipError_t e = allow_lds(k_stress_pipe<R, TX, TY>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_stress_pipe<R, TX, TY>), dim3(nb), dim3(T::NT), lds, st, s, tiles_x,
+                       tiles_y, zl, 0, s.nz);
+    return hipGetLastError();
+}
+
 template <typename R, int TX, int TY, bool PML = false>
 hipError_t velocity_launch(const Shot<R> &s, int zchunk, hipStream_t st)
 {
@@ -779,6 +1018,19 @@ hipError_t stress_batched_launch(const BatchEntry<R> *tab, const int32_t *blk, i
 }
 
 template <typename R, int TX, int TY>
+hipError_t stress_pipe_batched_launch(const BatchEntry<R> *tab, const int32_t *blk, int nb,
+                                      const int32_t *step, hipStream_t st)
+{
+    using T = StressTile<R, TX, TY, 0>;
+    const size_t lds = PipeTile<R, TX, TY>::LDS;
+    hipError_t e = allow_lds(k_stress_pipe_batched<R, TX, TY>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_stress_pipe_batched<R, TX, TY>), dim3(nb), dim3(T::NT), lds, st, tab,
+                       blk, nb, step);
+    return hipGetLastError();
+}
+
+template <typename R, int TX, int TY>
 hipError_t velocity_batched_launch(const BatchEntry<R> *tab, const int32_t *blk, int nb,
                                    const int32_t *step, hipStream_t st, int max_mat)
 {
@@ -800,15 +1052,18 @@ hipError_t velocity_batched_launch(const BatchEntry<R> *tab, const int32_t *blk,
 }
 
 // Tile variants (TX, TY, LDS row padding); variant 0 is the default.
-// fr: the stress kernel's frame ring (StressTile).
+// fr: 1 = the stress kernel's frame ring (StressTile), 2 = the one-barrier
+// pipelined stress body (stress_pipe_body).
 struct TileInfo {
     int tx, ty, pad, fr;
 };
 constexpr TileInfo kStressF32[] = {{32, 13, 24, 0}, {62, 6, 0, 0},  {30, 14, 24, 0},
                                    {32, 16, 24, 0}, {32, 8, 24, 0}, {16, 16, 0, 0},
                                    {32, 16, 0, 0},  {30, 14, 24, 1}, {32, 13, 24, 1},
-                                   {30, 14, 0, 1},  {62, 14, 0, 0}, {62, 14, 0, 1}};
-constexpr TileInfo kStressF64[] = {{32, 6, 0, 0}, {32, 4, 0, 0}, {16, 8, 0, 0}, {32, 6, 24, 0}};
+                                   {30, 14, 0, 1},  {62, 14, 0, 0}, {62, 14, 0, 1},
+                                   {32, 13, 0, 2},  {30, 14, 0, 2}};
+constexpr TileInfo kStressF64[] = {{32, 6, 0, 0}, {32, 4, 0, 0}, {16, 8, 0, 0}, {32, 6, 24, 0},
+                                   {32, 6, 0, 2}};
 constexpr TileInfo kVelF32[] = {{64, 8, 0, 0},  {32, 16, 0, 0}, {32, 8, 0, 0},
                                 {64, 4, 0, 0},  {128, 4, 0, 0}, {32, 32, 0, 0}};
 constexpr TileInfo kVelF64[] = {{32, 8, 0, 0}, {64, 4, 0, 0}, {32, 16, 0, 0}, {32, 4, 0, 0}};
@@ -860,6 +1115,8 @@ hipError_t launch_stress_batched<float>(int v, const BatchEntry<float> *tab, con
     case 9: return stress_batched_launch<float, 30, 14, 0, true>(tab, blk, nb, step, st);
     case 10: return stress_batched_launch<float, 62, 14, 0>(tab, blk, nb, step, st);
     case 11: return stress_batched_launch<float, 62, 14, 0, true>(tab, blk, nb, step, st);
+    case 12: return stress_pipe_batched_launch<float, 32, 13>(tab, blk, nb, step, st);
+    case 13: return stress_pipe_batched_launch<float, 30, 14>(tab, blk, nb, step, st);
     default: return stress_batched_launch<float, 32, 13, 24>(tab, blk, nb, step, st);
     }
 }
@@ -872,6 +1129,7 @@ hipError_t launch_stress_batched<double>(int v, const BatchEntry<double> *tab, c
     case 1: return stress_batched_launch<double, 32, 4, 0>(tab, blk, nb, step, st);
     case 2: return stress_batched_launch<double, 16, 8, 0>(tab, blk, nb, step, st);
     case 3: return stress_batched_launch<double, 32, 6, 24>(tab, blk, nb, step, st);
+    case 4: return stress_pipe_batched_launch<double, 32, 6>(tab, blk, nb, step, st);
     default: return stress_batched_launch<double, 32, 6, 0>(tab, blk, nb, step, st);
     }
 }
@@ -923,6 +1181,8 @@ hipError_t launch_stress_tiled<float>(const Shot<float> &s, int v, int zc, hipSt
     case 9: return stress_launch<float, 30, 14, 0, true>(s, zc, st, g);
     case 10: return stress_launch<float, 62, 14, 0>(s, zc, st, g);
     case 11: return stress_launch<float, 62, 14, 0, true>(s, zc, st, g);
+    case 12: return stress_pipe_launch<float, 32, 13>(s, zc, st, g);
+    case 13: return stress_pipe_launch<float, 30, 14>(s, zc, st, g);
     default: return stress_launch<float, 32, 13, 24>(s, zc, st, g);
     }
 }
@@ -936,6 +1196,7 @@ hipError_t launch_stress_tiled<double>(const Shot<double> &s, int v, int zc, hip
     case 1: return stress_launch<double, 32, 4, 0>(s, zc, st, g);
     case 2: return stress_launch<double, 16, 8, 0>(s, zc, st, g);
     case 3: return stress_launch<double, 32, 6, 24>(s, zc, st, g);
+    case 4: return stress_pipe_launch<double, 32, 6>(s, zc, st, g);
     default: return stress_launch<double, 32, 6, 0>(s, zc, st, g);
     }
 }

@@ -59,13 +59,14 @@ struct StressTile {
 };
 
 // Moved strain q_{P,J} read from the LDS strain ring (see moved_direct in
-// kernels_direct.hip for the rule).  sl[d+1] is the ring slot of plane z+d.
-// G = false drops the edge guards (the caller guarantees they all hold).
-template <bool G, int P, int J, typename R, int GX, int NG>
+// kernels_direct.hip for the rule), laid out [6][NS][NG].  sl[d+1] is the
+// ring slot of plane z+d.  G = false drops the edge guards (the caller
+// guarantees they all hold).
+template <bool G, int P, int J, typename R, int GX, int NG, int NS = 3>
 __device__ __forceinline__ R moved_lds(const R *__restrict__ gt, const int (&sl)[3], int c,
                                        int iz, int iy, int ix, int nz, int ny, int nx)
 {
-    const R *gJ = gt + J * 3 * NG;
+    const R *gJ = gt + J * NS * NG;
     constexpr int src = kVP[J];
     if constexpr (src == P) {
         return gJ[sl[1] * NG + c];
@@ -152,17 +153,17 @@ __device__ __forceinline__ void stress_out(const Shot<R> &S, Row cm, const R (&q
     if (any) sv = fma_r(S.dt, acc, sv);
 }
 
-template <bool G, int P, typename R, int GX, int NG>
+template <bool G, int P, typename R, int GX, int NG, int NS = 3>
 __device__ __forceinline__ void gather_lds(const R *__restrict__ gt, const int (&sl)[3], int c,
                                            int iz, int iy, int ix, int nz, int ny, int nx,
                                            R (&q)[6])
 {
-    q[0] = moved_lds<G, P, 0, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx);
-    q[1] = moved_lds<G, P, 1, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx);
-    q[2] = moved_lds<G, P, 2, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx);
-    q[3] = moved_lds<G, P, 3, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx);
-    q[4] = moved_lds<G, P, 4, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx);
-    q[5] = moved_lds<G, P, 5, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx);
+    q[0] = moved_lds<G, P, 0, R, GX, NG, NS>(gt, sl, c, iz, iy, ix, nz, ny, nx);
+    q[1] = moved_lds<G, P, 1, R, GX, NG, NS>(gt, sl, c, iz, iy, ix, nz, ny, nx);
+    q[2] = moved_lds<G, P, 2, R, GX, NG, NS>(gt, sl, c, iz, iy, ix, nz, ny, nx);
+    q[3] = moved_lds<G, P, 3, R, GX, NG, NS>(gt, sl, c, iz, iy, ix, nz, ny, nx);
+    q[4] = moved_lds<G, P, 4, R, GX, NG, NS>(gt, sl, c, iz, iy, ix, nz, ny, nx);
+    q[5] = moved_lds<G, P, 5, R, GX, NG, NS>(gt, sl, c, iz, iy, ix, nz, ny, nx);
 }
 
 // Strains g1..g6 of plane zc at one column (strains_zp statement order,
@@ -341,22 +342,22 @@ __device__ __forceinline__ void strains_col_pml(const Shot<R> &S, const R *bp, c
 template <typename R>
 using ConstRow = const __attribute__((address_space(4))) R *;
 
-template <bool G, bool FULL, typename R, int GX, int NG, typename Row>
+template <bool G, bool FULL, typename R, int GX, int NG, typename Row, int NS = 3>
 __device__ __forceinline__ void stress_cell(const Shot<R> &S, const R *__restrict__ gt,
                                             const int (&sl)[3], int c, int iz, int iy, int ix,
                                             Row cm, R dmp, R (&sv)[6])
 {
     const int nz = S.nz, ny = S.ny, nx = S.nx;
     R q[6];
-    gather_lds<G, 0, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx, q);
+    gather_lds<G, 0, R, GX, NG, NS>(gt, sl, c, iz, iy, ix, nz, ny, nx, q);
     stress_out<0, FULL>(S, cm, q, dmp, sv[0]);
     stress_out<1, FULL>(S, cm, q, dmp, sv[1]);
     stress_out<2, FULL>(S, cm, q, dmp, sv[2]);
-    gather_lds<G, 1, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx, q);
+    gather_lds<G, 1, R, GX, NG, NS>(gt, sl, c, iz, iy, ix, nz, ny, nx, q);
     stress_out<3, FULL>(S, cm, q, dmp, sv[3]);
-    gather_lds<G, 2, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx, q);
+    gather_lds<G, 2, R, GX, NG, NS>(gt, sl, c, iz, iy, ix, nz, ny, nx, q);
     stress_out<4, FULL>(S, cm, q, dmp, sv[4]);
-    gather_lds<G, 3, R, GX, NG>(gt, sl, c, iz, iy, ix, nz, ny, nx, q);
+    gather_lds<G, 3, R, GX, NG, NS>(gt, sl, c, iz, iy, ix, nz, ny, nx, q);
     stress_out<5, FULL>(S, cm, q, dmp, sv[5]);
 }
 
