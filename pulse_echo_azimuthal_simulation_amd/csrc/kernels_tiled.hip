@@ -19,6 +19,8 @@
 // consecutive workgroup ids to the 8 XCDs round robin; the id is remapped so
 // that each XCD receives a contiguous run of neighbouring tiles and their
 // shared halo planes stay in that XCD's L2.
+#include <mutex>
+
 #include "tiled_common.hpp"
 
 namespace pe {
@@ -901,6 +903,35 @@ __global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_velocity_
                                             g - e->blk_off[1], 0, S.nz);
 }
 
+// A second stream per device (default priority, created once) on which a
+// CPML half-step runs its band launches while the plain launch runs on the
+// caller's stream; the two sets of tiles write disjoint cells.  Calls that
+// overlap on one device share it.
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+
+inline hipError_t side_stream(SideStream **out)
+{
+    static std::mutex mu;
+    static SideStream per_dev[64];
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    SideStream &ss = per_dev[dev];
+    if (!ss.s) {
+        e = hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ss.join, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    }
+    *out = &ss;
+    return hipSuccess;
+}
+
 // CPML z ranges: the middle planes [m0, m1), whose tiles lie in a band only
 // through x or y, and the two slabs that contain the z bands.  A stress tile
 // of planes z0..z1-1 computes strains on z0-1 .. z1, so its middle range is
@@ -931,14 +962,24 @@ hipError_t stress_launch(const Shot<R> &s, int zchunk, hipStream_t st, bool stor
         const size_t lds_pml = T::LDS + sizeof(R) * 18 * (2 * (size_t)s.pml + 2);
         int m0, m1;
         pml_zranges(s.nz, s.pml, 0, m0, m1);
-        // band-free middle tiles (plain kernel), the middle tiles in an x/y
-        // band, then the two slabs holding the z bands (CPML kernel)
-        hipError_t e = run(k_stress_tiled<R, TX, TY, PAD, false, FR, false>, T::LDS, m0, m1);
+        // band-free middle tiles (plain kernel) on the caller's stream; the
+        // middle tiles in an x/y band and the two slabs holding the z bands
+        // (CPML kernel) concurrently on the side stream
+        SideStream *ss = nullptr;
+        hipError_t e = side_stream(&ss);
+        if (e == hipSuccess) e = hipEventRecord(ss->fork, st);
+        if (e == hipSuccess) e = hipStreamWaitEvent(ss->s, ss->fork, 0);
+        if (e == hipSuccess) e = run(k_stress_tiled<R, TX, TY, PAD, false, FR, false>, T::LDS, m0, m1);
+        const hipStream_t main = st;
+        st = ss ? ss->s : st;
         if (e == hipSuccess) e = run(k_stress_tiled<R, TX, TY, PAD, false, FR, true>, lds_pml, m0, m1);
         if (e == hipSuccess)
             e = run(k_stress_tiled<R, TX, TY, PAD, false, FR, true>, lds_pml, 0, m0 < m1 ? m0 : s.nz);
         if (e == hipSuccess && m0 < m1)
             e = run(k_stress_tiled<R, TX, TY, PAD, false, FR, true>, lds_pml, m1, s.nz);
+        st = main;
+        if (e == hipSuccess) e = hipEventRecord(ss->join, ss->s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, ss->join, 0);
         return e;
     }
     if (store_g) return run(k_stress_tiled<R, TX, TY, PAD, true, FR>, T::LDS, 0, s.nz);
@@ -985,20 +1026,29 @@ hipError_t velocity_launch(const Shot<R> &s, int zchunk, hipStream_t st)
         int m0, m1;
         pml_zranges(s.nz, s.pml, 1, m0, m1);
         const int lo_end = m0 < m1 ? m0 : s.nz;
-        hipError_t e;
+        SideStream *ss = nullptr;
+        hipError_t e = side_stream(&ss);
+        if (e == hipSuccess) e = hipEventRecord(ss->fork, st);
+        if (e == hipSuccess) e = hipStreamWaitEvent(ss->s, ss->fork, 0);
+        const hipStream_t main = st;
         if (rho_lds) {
-            e = run(k_velocity_tiled<R, TX, TY, true>, lds, m0, m1);
+            if (e == hipSuccess) e = run(k_velocity_tiled<R, TX, TY, true>, lds, m0, m1);
+            st = ss ? ss->s : st;
             if (e == hipSuccess) e = run(k_velocity_tiled<R, TX, TY, true, true>, lds_pml, m0, m1);
             if (e == hipSuccess) e = run(k_velocity_tiled<R, TX, TY, true, true>, lds_pml, 0, lo_end);
             if (e == hipSuccess && m0 < m1)
                 e = run(k_velocity_tiled<R, TX, TY, true, true>, lds_pml, m1, s.nz);
         } else {
-            e = run(k_velocity_tiled<R, TX, TY>, lds, m0, m1);
+            if (e == hipSuccess) e = run(k_velocity_tiled<R, TX, TY>, lds, m0, m1);
+            st = ss ? ss->s : st;
             if (e == hipSuccess) e = run(k_velocity_tiled<R, TX, TY, false, true>, lds_pml, m0, m1);
             if (e == hipSuccess) e = run(k_velocity_tiled<R, TX, TY, false, true>, lds_pml, 0, lo_end);
             if (e == hipSuccess && m0 < m1)
                 e = run(k_velocity_tiled<R, TX, TY, false, true>, lds_pml, m1, s.nz);
         }
+        st = main;
+        if (e == hipSuccess) e = hipEventRecord(ss->join, ss->s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, ss->join, 0);
         return e;
     }
     if (rho_lds) return run(k_velocity_tiled<R, TX, TY, true>, lds, 0, s.nz);
