@@ -1036,12 +1036,16 @@ __global__ __launch_bounds__((GradGTile<R, TX, TY>::NT)) void k_adj_grad_g_tiled
 
 inline unsigned blocks_of(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// Shortest z-chunk of the reverse kernels when the chunk length is chosen
+// for occupancy (small grids: more workgroups beat fewer ring refills).
+constexpr int kAdjMinChunk = 4;
+
 template <typename R, int TX, int TY>
 hipError_t adj_velocity_launch(const Shot<R> &s, const Adj<R> &a, int damp_ls, hipStream_t st)
 {
     using T = AdjVelTile<R, TX, TY>;
     const int tiles_x = (s.nx + TX - 1) / TX, tiles_y = (s.ny + TY - 1) / TY;
-    const int zl = chunk_len(s.nz, tiles_x * tiles_y, 0);
+    const int zl = chunk_len(s.nz, tiles_x * tiles_y, 0, kAdjMinChunk);
     const int nb = tiles_x * tiles_y * ((s.nz + zl - 1) / zl);
     hipError_t e = allow_lds(k_adj_velocity_tiled<R, TX, TY>, T::LDS);
     if (e != hipSuccess) return e;
@@ -1055,7 +1059,7 @@ hipError_t adj_strain_t_launch(const Shot<R> &s, const Adj<R> &a, hipStream_t st
 {
     using T = AdjStrTile<R, TX, TY>;
     const int tiles_x = (s.nx + TX - 1) / TX, tiles_y = (s.ny + TY - 1) / TY;
-    const int zl = chunk_len(s.nz, tiles_x * tiles_y, 0);
+    const int zl = chunk_len(s.nz, tiles_x * tiles_y, 0, kAdjMinChunk);
     const int nb = tiles_x * tiles_y * ((s.nz + zl - 1) / zl);
     hipError_t e = allow_lds(k_adj_strain_t_tiled<R, TX, TY>, T::LDS);
     if (e != hipSuccess) return e;
@@ -1069,7 +1073,7 @@ hipError_t adj_gbar_launch(const Shot<R> &s, const Adj<R> &a, hipStream_t st)
 {
     using T = GbarTile<R, TX, TY>;
     const int tiles_x = (s.nx + TX - 1) / TX, tiles_y = (s.ny + TY - 1) / TY;
-    const int zl = chunk_len(s.nz, tiles_x * tiles_y, 0);
+    const int zl = chunk_len(s.nz, tiles_x * tiles_y, 0, kAdjMinChunk);
     const int nb = tiles_x * tiles_y * ((s.nz + zl - 1) / zl);
     hipError_t e = allow_lds(k_adj_gbar_tiled<R, TX, TY>, T::LDS);
     if (e != hipSuccess) return e;

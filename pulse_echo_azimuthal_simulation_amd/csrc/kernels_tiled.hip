@@ -277,9 +277,12 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
             for (int k = 0; k < 6; ++k) gs[k * 3 * NT] = a[k];
             if constexpr (STORE_G) {
                 if (core && zc >= z0 && zc < z1) {
-                    const size_t n = (size_t)nz * sz, ig = (size_t)zc * sz + col;
+                    // component bases are wave-uniform; the offset within one
+                    // component fits 32 bits like every field
+                    const size_t n = (size_t)nz * sz;
+                    const int ig = zc * sz + col;
 #pragma unroll
-                    for (int k = 0; k < 6; ++k) S.g[k * n + ig] = a[k];
+                    for (int k = 0; k < 6; ++k) stg(S.g + k * n, ig, a[k]);
                 }
             }
         }

@@ -379,14 +379,14 @@ template <int NT> struct MinWaves<float, NT> {
 // z planes per workgroup when the caller does not fix it: about a dozen
 // workgroups per CU (measured best on 400x400x200), but no chunk shorter than
 // 16 planes (each chunk refills its 9-plane rings).
-inline int chunk_len(int nz, int tiles, int zchunk)
+inline int chunk_len(int nz, int tiles, int zchunk, int min_len = 16)
 {
     if (zchunk > 0) return zchunk > nz ? nz : zchunk;
     const int target = 256 * 12;
     int chunks = (target + tiles - 1) / tiles;
     chunks = chunks < 1 ? 1 : chunks;
     int len = (nz + chunks - 1) / chunks;
-    if (len < 16) len = 16;
+    if (len < min_len) len = min_len;
     return len > nz ? nz : len;
 }
 
