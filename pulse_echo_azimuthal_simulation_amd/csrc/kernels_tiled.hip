@@ -1219,8 +1219,8 @@ template <>
 hipError_t launch_stress_tiled<float>(const Shot<float> &s, int v, int zc, hipStream_t st,
                                       bool g)
 {
-    // CPML shots run the variant-0 tile shape whatever the variant (the
-    // band path is compiled once per precision)
+    // CPML shots run the variant-0 tile shape whatever the variant (smaller
+    // band tiles measured slower: the band path's redundant ring columns)
     if (s.pml > 0) return stress_launch<float, 32, 13, 24, false, true>(s, zc, st, g);
     switch (v) {
     case 1: return stress_launch<float, 62, 6, 0>(s, zc, st, g);
@@ -1257,7 +1257,13 @@ hipError_t launch_stress_tiled<double>(const Shot<double> &s, int v, int zc, hip
 template <>
 hipError_t launch_velocity_tiled<float>(const Shot<float> &s, int v, int zc, hipStream_t st)
 {
-    if (s.pml > 0) return velocity_launch<float, 64, 8, true>(s, zc, st);
+    // CPML shots: the band path is compiled for variants 0, 2 and 3 (the
+    // 4-wave tiles fit three band workgroups per CU); others run variant 0
+    if (s.pml > 0) {
+        if (v == 2) return velocity_launch<float, 32, 8, true>(s, zc, st);
+        if (v == 3) return velocity_launch<float, 64, 4, true>(s, zc, st);
+        return velocity_launch<float, 64, 8, true>(s, zc, st);
+    }
     switch (v) {
     case 1: return velocity_launch<float, 32, 16>(s, zc, st);
     case 2: return velocity_launch<float, 32, 8>(s, zc, st);

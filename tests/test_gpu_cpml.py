@@ -61,6 +61,37 @@ def test_cpml_matches_oracle(torch_gpu, kernel, precision):
     assert H.rel_l2(tr, otr) <= 1e-6
 
 
+@pytest.mark.parametrize("vel_variant", [2, 3])
+def test_cpml_velocity_tile_variants_match_oracle(torch_gpu, vel_variant):
+    """The float32 velocity tiles with their own band instantiation (32x8,
+    64x4) give the oracle's fields bit for bit."""
+    print("\n[gpu case] cpml velocity variant %d" % vel_variant, flush=True)
+    from oracle import oracle as O
+    from pulse_echo_azimuthal_simulation_amd.solver import DeviceShot
+    inp, cp = _medium(shape=(23, 27, 70), seed=9)
+    nt = inp["nt"]
+    sh = DeviceShot(inp["mats"], inp["Ct"], inp["rho_t"], inp["damp"], inp["coeffs"], inp["h"],
+                    inp["dt"], inp["src"][0], inp["src"][1], *inp["rec"], precision="f32",
+                    kernel="tiled", zchunk=(6, 5), cpml=cp)
+    sh.desc.tile[1] = vel_variant
+    wav = sh.wavelet(inp["wavelet"])
+    tr = sh.new_traces(nt)
+    sh.run(wav, 0, nt, tr)
+    v, s = sh.fields()
+    osh = O.OracleShot(inp["mats"], inp["Ct"], inp["rho_t"], inp["damp"], inp["coeffs"],
+                       inp["h"], inp["dt"], inp["src"], inp["rec"], precision="f32",
+                       cpml=(cp.width, cp.prof))
+    otr = osh.run(inp["wavelet"], 0, nt)
+    for k in range(3):
+        assert np.array_equal(v[k], osh.v[k]), "v%d" % k
+    for k in range(6):
+        assert np.array_equal(s[k], osh.s[k]), "s%d" % (k + 1)
+    psi = [p.cpu().numpy() for p in sh.psi]
+    for a in range(3):
+        assert np.array_equal(psi[a].reshape(osh.psi[a].shape), osh.psi[a]), "psi axis %d" % a
+    assert H.rel_l2(tr.cpu().numpy(), otr) <= 1e-6
+
+
 def test_cpml_drop_in_boundary_option(torch_gpu):
     """forward_fused_labels(boundary='cpml') runs the same kernels as a
     DeviceShot built with the same profiles."""
