@@ -273,10 +273,24 @@ def main():
         ms2 = sp.run_timed(wav, 0, args.sponge_steps, tr2)
         torch.cuda.synchronize()
         w2 = time.perf_counter() - t1
+        ts2, tv2 = ms2[0] / args.sponge_steps / 1e3, ms2[1] / args.sponge_steps / 1e3
+        dom2 = "stress" if ts2 >= tv2 else "velocity"
+        ach2 = (BYTES_STRESS[p] * ncell / ts2 if dom2 == "stress"
+                else BYTES_VELOCITY[p] * ncell / tv2) / 1e9
+        traffic2 = None
+        tfile2 = os.path.join(ROOT, "profiles", "traffic_%s_%s_%s_sponge.json"
+                              % (args.config, p, args.kernel))
+        if os.path.exists(tfile2):
+            with open(tfile2) as fh:
+                traffic2 = json.load(fh).get(dom2)
         out["sponge20"] = {"value": round(ncell * args.sponge_steps / w2 / 1e9, 4),
                            "steps": args.sponge_steps,
-                           "kernels_ms_per_step": {"stress": round(ms2[0] / args.sponge_steps, 5),
-                                                   "velocity": round(ms2[1] / args.sponge_steps, 5)}}
+                           "kernels_ms_per_step": {"stress": round(ts2 * 1e3, 5),
+                                                   "velocity": round(tv2 * 1e3, 5)},
+                           "roofline": {"kernel": dom2, "bound": "hbm", "achieved": round(ach2, 1),
+                                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                        "frac": round(ach2 / HBM_PEAK_GBS, 4),
+                                        "traffic": traffic2}}
         log("sponge20: %.2f Gcell-updates/s over %d steps" % (out["sponge20"]["value"],
                                                                args.sponge_steps))
         del sp, tr2

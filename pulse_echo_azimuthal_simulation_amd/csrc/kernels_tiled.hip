@@ -909,10 +909,13 @@ __global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_velocity_
 // A second stream per device (default priority, created once) on which a
 // CPML half-step runs its band launches while the plain launch runs on the
 // caller's stream; the two sets of tiles write disjoint cells.  Calls that
-// overlap on one device share it.
+// overlap on one device share it: a caller holds `use` from the fork record
+// to the join wait, so host threads enqueueing at the same time cannot
+// interleave their fork/join records (the lock covers enqueueing only).
 struct SideStream {
     hipStream_t s = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    std::mutex use;
 };
 
 inline hipError_t side_stream(SideStream **out)
@@ -970,11 +973,13 @@ hipError_t stress_launch(const Shot<R> &s, int zchunk, hipStream_t st, bool stor
         // (CPML kernel) concurrently on the side stream
         SideStream *ss = nullptr;
         hipError_t e = side_stream(&ss);
-        if (e == hipSuccess) e = hipEventRecord(ss->fork, st);
+        if (e != hipSuccess) return e;
+        std::lock_guard<std::mutex> hold(ss->use);
+        e = hipEventRecord(ss->fork, st);
         if (e == hipSuccess) e = hipStreamWaitEvent(ss->s, ss->fork, 0);
         if (e == hipSuccess) e = run(k_stress_tiled<R, TX, TY, PAD, false, FR, false>, T::LDS, m0, m1);
         const hipStream_t main = st;
-        st = ss ? ss->s : st;
+        st = ss->s;
         if (e == hipSuccess) e = run(k_stress_tiled<R, TX, TY, PAD, false, FR, true>, lds_pml, m0, m1);
         if (e == hipSuccess)
             e = run(k_stress_tiled<R, TX, TY, PAD, false, FR, true>, lds_pml, 0, m0 < m1 ? m0 : s.nz);
@@ -1031,19 +1036,21 @@ hipError_t velocity_launch(const Shot<R> &s, int zchunk, hipStream_t st)
         const int lo_end = m0 < m1 ? m0 : s.nz;
         SideStream *ss = nullptr;
         hipError_t e = side_stream(&ss);
-        if (e == hipSuccess) e = hipEventRecord(ss->fork, st);
+        if (e != hipSuccess) return e;
+        std::lock_guard<std::mutex> hold(ss->use);
+        e = hipEventRecord(ss->fork, st);
         if (e == hipSuccess) e = hipStreamWaitEvent(ss->s, ss->fork, 0);
         const hipStream_t main = st;
         if (rho_lds) {
             if (e == hipSuccess) e = run(k_velocity_tiled<R, TX, TY, true>, lds, m0, m1);
-            st = ss ? ss->s : st;
+            st = ss->s;
             if (e == hipSuccess) e = run(k_velocity_tiled<R, TX, TY, true, true>, lds_pml, m0, m1);
             if (e == hipSuccess) e = run(k_velocity_tiled<R, TX, TY, true, true>, lds_pml, 0, lo_end);
             if (e == hipSuccess && m0 < m1)
                 e = run(k_velocity_tiled<R, TX, TY, true, true>, lds_pml, m1, s.nz);
         } else {
             if (e == hipSuccess) e = run(k_velocity_tiled<R, TX, TY>, lds, m0, m1);
-            st = ss ? ss->s : st;
+            st = ss->s;
             if (e == hipSuccess) e = run(k_velocity_tiled<R, TX, TY, false, true>, lds_pml, m0, m1);
             if (e == hipSuccess) e = run(k_velocity_tiled<R, TX, TY, false, true>, lds_pml, 0, lo_end);
             if (e == hipSuccess && m0 < m1)

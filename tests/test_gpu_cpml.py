@@ -61,6 +61,36 @@ def test_cpml_matches_oracle(torch_gpu, kernel, precision):
     assert H.rel_l2(tr, otr) <= 1e-6
 
 
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+@pytest.mark.parametrize("nz", [10, 11])
+def test_cpml_thin_grid_without_band_free_middle(torch_gpu, nz, precision):
+    """nz = 2L+2 and 2L+3 with L = 4: no plane of the stress half-step is
+    clear of the z bands (one CPML launch over the whole grid), and the
+    velocity half-step has no middle plane (nz = 10) or one (nz = 11)."""
+    print("\n[gpu case] cpml thin grid nz %d %s" % (nz, precision), flush=True)
+    from oracle import oracle as O
+    from pulse_echo_azimuthal_simulation_amd.solver import DeviceShot
+    inp, cp = _medium(shape=(nz, 25, 29), seed=11)
+    nt = inp["nt"]
+    sh = DeviceShot(inp["mats"], inp["Ct"], inp["rho_t"], inp["damp"], inp["coeffs"], inp["h"],
+                    inp["dt"], inp["src"][0], inp["src"][1], *inp["rec"], precision=precision,
+                    kernel="tiled", zchunk=(4, 3), cpml=cp)
+    wav = sh.wavelet(inp["wavelet"])
+    tr = sh.new_traces(nt)
+    sh.run(wav, 0, nt, tr)
+    v, s = sh.fields()
+    osh = O.OracleShot(inp["mats"], inp["Ct"], inp["rho_t"], inp["damp"], inp["coeffs"],
+                       inp["h"], inp["dt"], inp["src"], inp["rec"], precision=precision,
+                       cpml=(cp.width, cp.prof))
+    otr = osh.run(inp["wavelet"], 0, nt)
+    assert np.abs(otr).max() > 0
+    for k in range(3):
+        assert np.array_equal(v[k], osh.v[k]), "v%d" % k
+    for k in range(6):
+        assert np.array_equal(s[k], osh.s[k]), "s%d" % (k + 1)
+    assert H.rel_l2(tr.cpu().numpy(), otr) <= 1e-6
+
+
 @pytest.mark.parametrize("vel_variant", [2, 3])
 def test_cpml_velocity_tile_variants_match_oracle(torch_gpu, vel_variant):
     """The float32 velocity tiles with their own band instantiation (32x8,
