@@ -32,14 +32,16 @@ using namespace tiled;
 // Body of the stress kernel for logical tile t (tile x fastest, then tile
 // y, then z-chunk) of shot S.  STORE_G: also write the strains g1..g6 of the
 // tile's own columns into S.g (6 x ncell, the history the adjoint's
-// stiffness gradient reads), each plane once.  FR: see StressTile.  PML: the
-// shot has a CPML boundary (band columns take strains_col_pml).
+// stiffness gradient reads), each plane once.  FR: see StressTile.  PM: the
+// CPML band kernel of the tile (band_kernel; 0 = no band path compiled), band
+// columns take strains_col_pml.
 template <typename R, int TX, int TY, int PAD, bool STORE_G = false, bool FR = false,
-          bool PML = false>
+          int PM = 0>
 __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x, int tiles_y,
                                                   int zc_len, int t, int zlo, int zhi)
 {
     using T = StressTile<R, TX, TY, PAD, FR>;
+    constexpr bool PML = PM != 0;
     constexpr int GX = T::GX, GY = T::GY, NG = T::NG, VX = T::VX, VY = T::VY, NV = T::NV;
     constexpr int NT = T::NT, NF = T::NF, WX = T::WX;
     extern __shared__ unsigned char lds_raw[];
@@ -64,7 +66,7 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
     const int gx = gact ? tid % GX : 0, gy = gact ? tid / GX : 0;
     const int ix = x0 - 1 + gx, iy = y0 - 1 + gy;                  // column of this thread
     const bool col_in = gact && ix >= 0 && ix < nx && iy >= 0 && iy < ny;
-    const bool pml_xy = PML && (pml_in(ix, nx, S.pml) || pml_in(iy, ny, S.pml));
+    const bool pml_xy = ((PM & 4) && pml_in(ix, nx, S.pml)) || ((PM & 2) && pml_in(iy, ny, S.pml));
     const bool core = col_in && gx >= 1 && gx <= TX && gy >= 1 && gy <= TY;
     const int col = clampi(iy, 0, ny - 1) * sy + clampi(ix, 0, nx - 1);
     // LDS slot of this column in each v plane tile; idle threads write the
@@ -183,8 +185,8 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
         R ps[9];
         bool band = false;
         if constexpr (PML) {
-            band = pml_xy || pml_in(zc, nz, S.pml);
-            if (band && col_in && plane_in) pml_psi_load(S, zc, iy, ix, 0, ps);
+            band = pml_xy || ((PM & 1) && pml_in(zc, nz, S.pml));
+            if (band && col_in && plane_in) pml_psi_load<PM>(S, zc, iy, ix, 0, ps);
         }
         // (1) plane zc of v into LDS: own column from the ring, frame from
         //     the delay line
@@ -263,7 +265,7 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
                 // computed by neighbouring workgroups)
                 if constexpr (PML)
                     if (col_in && plane_in)
-                        strains_col_pml<R, VX>(S, bp, ps, vt + vc, vt + NV + vc, vt + 2 * NV + vc,
+                        strains_col_pml<R, VX, PM>(S, bp, ps, vt + vc, vt + NV + vc, vt + 2 * NV + vc,
                                                rx, ry, rz, ix, iy, zc,
                                                core && zc >= z0 && zc < z1, a);
             } else if (inner_xy && zc >= 4 && zc <= nz - 5)
@@ -552,8 +554,8 @@ __device__ __forceinline__ void accel_col(const Shot<R> &S, const R *p1, const R
 // Accelerations of one column at a CPML band cell (see pml_d): the taps of
 // accel_col per derivative, scaled by 1/h, through pml_t (memory variables
 // 3..5 in ps); summed x, y, z per component.  Every velocity cell has one
-// owner.
-template <typename R, int VX>
+// owner.  M: see pml_psi_load.
+template <typename R, int VX, int M = 7>
 __device__ __forceinline__ void accel_col_pml(const Shot<R> &S, const R *bp, const R (&ps)[9],
                                               const R *p1, const R *p2, const R *p4,
                                               const R *p5, const R *p6, const R (&r3)[9],
@@ -584,26 +586,28 @@ __device__ __forceinline__ void accel_col_pml(const Shot<R> &S, const R *bp, con
     }
 #pragma unroll
     for (int t = 0; t < 9; ++t) e[t] = e[t] * S.inv_h;
-    ax = pml_t<2, 3, 1>(S, bp, z, iy, ix, e[0], ps, true) +
-         pml_t<1, 3, 0>(S, bp, z, iy, ix, e[1], ps, true) +
-         pml_t<0, 3, 0>(S, bp, z, iy, ix, e[2], ps, true);
-    ay = pml_t<2, 4, 0>(S, bp, z, iy, ix, e[3], ps, true) +
-         pml_t<1, 4, 1>(S, bp, z, iy, ix, e[4], ps, true) +
-         pml_t<0, 4, 0>(S, bp, z, iy, ix, e[5], ps, true);
-    az = pml_t<2, 5, 0>(S, bp, z, iy, ix, e[6], ps, true) +
-         pml_t<1, 5, 0>(S, bp, z, iy, ix, e[7], ps, true) +
-         pml_t<0, 5, 1>(S, bp, z, iy, ix, e[8], ps, true);
+    ax = pml_t<2, 3, 1, M>(S, bp, z, iy, ix, e[0], ps, true) +
+         pml_t<1, 3, 0, M>(S, bp, z, iy, ix, e[1], ps, true) +
+         pml_t<0, 3, 0, M>(S, bp, z, iy, ix, e[2], ps, true);
+    ay = pml_t<2, 4, 0, M>(S, bp, z, iy, ix, e[3], ps, true) +
+         pml_t<1, 4, 1, M>(S, bp, z, iy, ix, e[4], ps, true) +
+         pml_t<0, 4, 0, M>(S, bp, z, iy, ix, e[5], ps, true);
+    az = pml_t<2, 5, 0, M>(S, bp, z, iy, ix, e[6], ps, true) +
+         pml_t<1, 5, 0, M>(S, bp, z, iy, ix, e[7], ps, true) +
+         pml_t<0, 5, 1, M>(S, bp, z, iy, ix, e[8], ps, true);
 }
 
 // RHO_LDS: the density table sits in LDS after the five plane tiles, so the
 // face-density lookups of the output stage wait on LDS only.  A global
 // lookup there would make the wave wait for every prefetch issued before it
 // (vector memory counters retire in order).
-template <typename R, int TX, int TY, bool RHO_LDS = false, bool PML = false>
+// PM: the CPML band kernel of the tile, as in stress_tiled_body.
+template <typename R, int TX, int TY, bool RHO_LDS = false, int PM = 0>
 __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_x, int tiles_y,
                                                     int zc_len, int t, int zlo, int zhi)
 {
     using T = VelTile<R, TX, TY>;
+    constexpr bool PML = PM != 0;
     constexpr int VX = T::VX, NV = T::NV, NT = T::NT, NF = (T::NFRAME + NT - 1) / NT;
     extern __shared__ unsigned char lds_raw[];
     R *st = reinterpret_cast<R *>(lds_raw);   // [5][VY][VX]: s1, s2, s4, s5, s6
@@ -633,7 +637,7 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
     const int lx = tid % TX, ly = tid / TX;
     const int ix = x0 + lx, iy = y0 + ly;
     const bool col_in = ix < nx && iy < ny;
-    const bool pml_xy = PML && (pml_in(ix, nx, S.pml) || pml_in(iy, ny, S.pml));
+    const bool pml_xy = ((PM & 4) && pml_in(ix, nx, S.pml)) || ((PM & 2) && pml_in(iy, ny, S.pml));
     const int col = clampi(iy, 0, ny - 1) * sy + clampi(ix, 0, nx - 1);
     const int vc = (ly + 4) * VX + (lx + 4);
 
@@ -738,8 +742,8 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
         R ps[9];
         bool band = false;
         if constexpr (PML) {
-            band = pml_xy || pml_in(z, nz, S.pml);
-            if (band && col_in) pml_psi_load(S, z, iy, ix, 3, ps);
+            band = pml_xy || ((PM & 1) && pml_in(z, nz, S.pml));
+            if (band && col_in) pml_psi_load<PM>(S, z, iy, ix, 3, ps);
         }
         // (1) plane z of s1, s2, s4, s5, s6 into LDS
         t1[vc] = c1;
@@ -772,7 +776,7 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
             R tstep = S.dt_h;
             if (band) {
                 if constexpr (PML) {
-                    accel_col_pml<R, VX>(S, bp, ps, t1 + vc, t2 + vc, t4 + vc, t5 + vc, t6 + vc,
+                    accel_col_pml<R, VX, PM>(S, bp, ps, t1 + vc, t2 + vc, t4 + vc, t5 + vc, t6 + vc,
                                          r3, r4, r5, ix, iy, z, ax, ay, az);
                     tstep = S.dt;
                 }
@@ -796,58 +800,73 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
 }
 
 // ------------------------------------------------------------- launchers
-// Single-shot kernels.  With a CPML boundary the tiles are split over two
-// launches of the same geometry: the plain kernel (PML false) takes the
-// tiles clear of every band, the PML kernel the others.  The PML kernel may
-// use twice the registers (its band path needs them; the plain tiles keep
-// their occupancy).
+// Single-shot kernels.  With a CPML boundary each tile is given to one of
+// three kernels by the bands it meets: 0 (none: the plain kernel), 1 (a z
+// band only) and 7 (an x or y band, maybe also a z band); the template
+// argument PM is the mask of axes whose memory variables the kernel compiles
+// (bit 0 z, bit 1 y, bit 2 x).  Each kernel exits for the tiles of the
+// others, so launches of the same geometry over the same planes split the
+// tiles between them.  Kernel 7 may use twice the registers (its band path
+// needs them); kernel 1 keeps the plain kernel's register bound, and so its
+// occupancy.  (Kernels for the x and y bands apart, 3 and 5, were measured
+// slower: their launches run too few tiles to fill the device.)
+inline __device__ int band_kernel(bool x, bool y, bool z)
+{
+    return x || y ? 7 : (z ? 1 : 0);
+}
+
 template <typename R, int TX, int TY>
-__device__ __forceinline__ bool stress_band_tile(const Shot<R> &S, int t, int tiles_x,
-                                                 int tiles_y, int zc_len, int zlo, int zhi)
+__device__ __forceinline__ int stress_band_kernel(const Shot<R> &S, int t, int tiles_x,
+                                                  int tiles_y, int zc_len, int zlo, int zhi)
 {
     // strain columns x0-1 .. x0+TX, y0-1 .. y0+TY; planes z0-1 .. z1
     const int L = S.pml;
     const int x0 = (t % tiles_x) * TX, y0 = ((t / tiles_x) % tiles_y) * TY;
     const int z0 = zlo + (t / (tiles_x * tiles_y)) * zc_len, z1 = min(zhi, z0 + zc_len);
-    return x0 - 1 <= L || x0 + TX >= S.nx - 1 - L || y0 - 1 <= L || y0 + TY >= S.ny - 1 - L ||
-           z0 - 1 <= L || z1 >= S.nz - 1 - L;
+    return band_kernel(x0 - 1 <= L || x0 + TX >= S.nx - 1 - L,
+                       y0 - 1 <= L || y0 + TY >= S.ny - 1 - L, z0 - 1 <= L || z1 >= S.nz - 1 - L);
 }
 
 template <typename R, int TX, int TY>
-__device__ __forceinline__ bool velocity_band_tile(const Shot<R> &S, int t, int tiles_x,
-                                                   int tiles_y, int zc_len, int zlo, int zhi)
+__device__ __forceinline__ int velocity_band_kernel(const Shot<R> &S, int t, int tiles_x,
+                                                    int tiles_y, int zc_len, int zlo, int zhi)
 {
     const int L = S.pml;
     const int x0 = (t % tiles_x) * TX, y0 = ((t / tiles_x) % tiles_y) * TY;
     const int z0 = zlo + (t / (tiles_x * tiles_y)) * zc_len, z1 = min(zhi, z0 + zc_len);
-    return x0 <= L || x0 + TX - 1 >= S.nx - 1 - L || y0 <= L || y0 + TY - 1 >= S.ny - 1 - L ||
-           z0 <= L || z1 - 1 >= S.nz - 1 - L;
+    return band_kernel(x0 <= L || x0 + TX - 1 >= S.nx - 1 - L,
+                       y0 <= L || y0 + TY - 1 >= S.ny - 1 - L, z0 <= L || z1 - 1 >= S.nz - 1 - L);
 }
 
-template <typename R, int TX, int TY, int PAD, bool STORE_G = false, bool FR = false,
-          bool PML = false>
+// Register bound of band kernel PM: the plain kernel's (MinWaves) but for
+// kernel 7.
+template <typename R, int NT, int PM> struct ClassWaves {
+    static constexpr int v = PM == 7 ? 1 : MinWaves<R, NT>::v;
+};
+
+template <typename R, int TX, int TY, int PAD, bool STORE_G = false, bool FR = false, int PM = 0>
 __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
-                             (PML ? 1 : MinWaves<R, StressTile<R, TX, TY, PAD>::NT>::v)) void
+                             (ClassWaves<R, StressTile<R, TX, TY, PAD>::NT, PM>::v)) void
 k_stress_tiled(Shot<R> S, int tiles_x, int tiles_y, int zc_len, int zlo, int zhi)
 {
     const int nb = tiles_x * tiles_y * ((zhi - zlo + zc_len - 1) / zc_len);
     const int t = xcd_remap(blockIdx.x, nb);
     if (S.pml > 0 &&
-        stress_band_tile<R, TX, TY>(S, t, tiles_x, tiles_y, zc_len, zlo, zhi) != PML)
+        stress_band_kernel<R, TX, TY>(S, t, tiles_x, tiles_y, zc_len, zlo, zhi) != PM)
         return;
-    stress_tiled_body<R, TX, TY, PAD, STORE_G, FR, PML>(S, tiles_x, tiles_y, zc_len, t, zlo, zhi);
+    stress_tiled_body<R, TX, TY, PAD, STORE_G, FR, PM>(S, tiles_x, tiles_y, zc_len, t, zlo, zhi);
 }
 
-template <typename R, int TX, int TY, bool RHO_LDS = false, bool PML = false>
-__global__ __launch_bounds__(TX *TY, (PML ? 1 : MinWaves<R, TX * TY>::v)) void k_velocity_tiled(
+template <typename R, int TX, int TY, bool RHO_LDS = false, int PM = 0>
+__global__ __launch_bounds__(TX *TY, (ClassWaves<R, TX * TY, PM>::v)) void k_velocity_tiled(
     Shot<R> S, int tiles_x, int tiles_y, int zc_len, int zlo, int zhi)
 {
     const int nb = tiles_x * tiles_y * ((zhi - zlo + zc_len - 1) / zc_len);
     const int t = xcd_remap(blockIdx.x, nb);
     if (S.pml > 0 &&
-        velocity_band_tile<R, TX, TY>(S, t, tiles_x, tiles_y, zc_len, zlo, zhi) != PML)
+        velocity_band_kernel<R, TX, TY>(S, t, tiles_x, tiles_y, zc_len, zlo, zhi) != PM)
         return;
-    velocity_tiled_body<R, TX, TY, RHO_LDS, PML>(S, tiles_x, tiles_y, zc_len, t, zlo, zhi);
+    velocity_tiled_body<R, TX, TY, RHO_LDS, PM>(S, tiles_x, tiles_y, zc_len, t, zlo, zhi);
 }
 
 template <typename R, int TX, int TY>
@@ -968,23 +987,24 @@ hipError_t stress_launch(const Shot<R> &s, int zchunk, hipStream_t st, bool stor
         const size_t lds_pml = T::LDS + sizeof(R) * 18 * (2 * (size_t)s.pml + 2);
         int m0, m1;
         pml_zranges(s.nz, s.pml, 0, m0, m1);
-        // band-free middle tiles (plain kernel) on the caller's stream; the
-        // middle tiles in an x/y band and the two slabs holding the z bands
-        // (CPML kernel) concurrently on the side stream
+        // the caller's stream takes the band-free middle tiles (kernel 0)
+        // and then the z-only tiles of the two slabs that hold the z bands
+        // (kernel 1); concurrently the side stream takes the tiles in an x
+        // or y band (kernel 7) in one launch over every plane
+        const int lo_end = m0 < m1 ? m0 : s.nz;
         SideStream *ss = nullptr;
         hipError_t e = side_stream(&ss);
         if (e != hipSuccess) return e;
         std::lock_guard<std::mutex> hold(ss->use);
         e = hipEventRecord(ss->fork, st);
         if (e == hipSuccess) e = hipStreamWaitEvent(ss->s, ss->fork, 0);
-        if (e == hipSuccess) e = run(k_stress_tiled<R, TX, TY, PAD, false, FR, false>, T::LDS, m0, m1);
+        if (e == hipSuccess) e = run(k_stress_tiled<R, TX, TY, PAD, false, FR, 0>, T::LDS, m0, m1);
+        if (e == hipSuccess) e = run(k_stress_tiled<R, TX, TY, PAD, false, FR, 1>, lds_pml, 0, lo_end);
+        if (e == hipSuccess && m0 < m1)
+            e = run(k_stress_tiled<R, TX, TY, PAD, false, FR, 1>, lds_pml, m1, s.nz);
         const hipStream_t main = st;
         st = ss->s;
-        if (e == hipSuccess) e = run(k_stress_tiled<R, TX, TY, PAD, false, FR, true>, lds_pml, m0, m1);
-        if (e == hipSuccess)
-            e = run(k_stress_tiled<R, TX, TY, PAD, false, FR, true>, lds_pml, 0, m0 < m1 ? m0 : s.nz);
-        if (e == hipSuccess && m0 < m1)
-            e = run(k_stress_tiled<R, TX, TY, PAD, false, FR, true>, lds_pml, m1, s.nz);
+        if (e == hipSuccess) e = run(k_stress_tiled<R, TX, TY, PAD, false, FR, 7>, lds_pml, 0, s.nz);
         st = main;
         if (e == hipSuccess) e = hipEventRecord(ss->join, ss->s);
         if (e == hipSuccess) e = hipStreamWaitEvent(st, ss->join, 0);
@@ -1034,32 +1054,31 @@ hipError_t velocity_launch(const Shot<R> &s, int zchunk, hipStream_t st)
         int m0, m1;
         pml_zranges(s.nz, s.pml, 1, m0, m1);
         const int lo_end = m0 < m1 ? m0 : s.nz;
-        SideStream *ss = nullptr;
-        hipError_t e = side_stream(&ss);
-        if (e != hipSuccess) return e;
-        std::lock_guard<std::mutex> hold(ss->use);
-        e = hipEventRecord(ss->fork, st);
-        if (e == hipSuccess) e = hipStreamWaitEvent(ss->s, ss->fork, 0);
-        const hipStream_t main = st;
-        if (rho_lds) {
-            if (e == hipSuccess) e = run(k_velocity_tiled<R, TX, TY, true>, lds, m0, m1);
+        // the launch sequence of stress_launch: kernels 0 and 1 on the
+        // caller's stream, kernel 7 on the side stream
+        auto seq = [&](auto k0, auto k1, auto k7) -> hipError_t {
+            SideStream *ss = nullptr;
+            hipError_t e = side_stream(&ss);
+            if (e != hipSuccess) return e;
+            std::lock_guard<std::mutex> hold(ss->use);
+            e = hipEventRecord(ss->fork, st);
+            if (e == hipSuccess) e = hipStreamWaitEvent(ss->s, ss->fork, 0);
+            if (e == hipSuccess) e = run(k0, lds, m0, m1);
+            if (e == hipSuccess) e = run(k1, lds_pml, 0, lo_end);
+            if (e == hipSuccess && m0 < m1) e = run(k1, lds_pml, m1, s.nz);
+            const hipStream_t main = st;
             st = ss->s;
-            if (e == hipSuccess) e = run(k_velocity_tiled<R, TX, TY, true, true>, lds_pml, m0, m1);
-            if (e == hipSuccess) e = run(k_velocity_tiled<R, TX, TY, true, true>, lds_pml, 0, lo_end);
-            if (e == hipSuccess && m0 < m1)
-                e = run(k_velocity_tiled<R, TX, TY, true, true>, lds_pml, m1, s.nz);
-        } else {
-            if (e == hipSuccess) e = run(k_velocity_tiled<R, TX, TY>, lds, m0, m1);
-            st = ss->s;
-            if (e == hipSuccess) e = run(k_velocity_tiled<R, TX, TY, false, true>, lds_pml, m0, m1);
-            if (e == hipSuccess) e = run(k_velocity_tiled<R, TX, TY, false, true>, lds_pml, 0, lo_end);
-            if (e == hipSuccess && m0 < m1)
-                e = run(k_velocity_tiled<R, TX, TY, false, true>, lds_pml, m1, s.nz);
-        }
-        st = main;
-        if (e == hipSuccess) e = hipEventRecord(ss->join, ss->s);
-        if (e == hipSuccess) e = hipStreamWaitEvent(st, ss->join, 0);
-        return e;
+            if (e == hipSuccess) e = run(k7, lds_pml, 0, s.nz);
+            st = main;
+            if (e == hipSuccess) e = hipEventRecord(ss->join, ss->s);
+            if (e == hipSuccess) e = hipStreamWaitEvent(st, ss->join, 0);
+            return e;
+        };
+        if (rho_lds)
+            return seq(k_velocity_tiled<R, TX, TY, true, 0>, k_velocity_tiled<R, TX, TY, true, 1>,
+                       k_velocity_tiled<R, TX, TY, true, 7>);
+        return seq(k_velocity_tiled<R, TX, TY, false, 0>, k_velocity_tiled<R, TX, TY, false, 1>,
+                   k_velocity_tiled<R, TX, TY, false, 7>);
     }
     if (rho_lds) return run(k_velocity_tiled<R, TX, TY, true>, lds, 0, s.nz);
     return run(k_velocity_tiled<R, TX, TY>, lds, 0, s.nz);

@@ -267,21 +267,27 @@ __device__ __forceinline__ void pml_psi_load_axis(const Shot<R> &S, int iz, int 
     ps[3 * A + 2] = q[2 * vol];
 }
 
-template <typename R>
+// M: the axes whose band the caller's tile can meet (bit 0 z, bit 1 y,
+// bit 2 x); the memory variables of the other axes are compiled out.
+template <int M = 7, typename R>
 __device__ __forceinline__ void pml_psi_load(const Shot<R> &S, int iz, int iy, int ix, int K0,
                                              R (&ps)[9])
 {
-    pml_psi_load_axis<0>(S, iz, iy, ix, K0, ps);
-    pml_psi_load_axis<1>(S, iz, iy, ix, K0, ps);
-    pml_psi_load_axis<2>(S, iz, iy, ix, K0, ps);
+    if constexpr (M & 1) pml_psi_load_axis<0>(S, iz, iy, ix, K0, ps);
+    else ps[0] = ps[1] = ps[2] = 0;
+    if constexpr (M & 2) pml_psi_load_axis<1>(S, iz, iy, ix, K0, ps);
+    else ps[3] = ps[4] = ps[5] = 0;
+    if constexpr (M & 4) pml_psi_load_axis<2>(S, iz, iy, ix, K0, ps);
+    else ps[6] = ps[7] = ps[8] = 0;
 }
 
 // pml_d with the profile from the LDS band table and psi given (ps[3A +
 // K % 3]); the new psi goes to the write buffer when `write`.
-template <int A, int K, int ST, typename R>
+template <int A, int K, int ST, int M = 7, typename R>
 __device__ __forceinline__ R pml_t(const Shot<R> &S, const R *bp, int iz, int iy, int ix, R dd,
                                    const R (&ps)[9], bool write)
 {
+    if constexpr (!((M >> A) & 1)) return dd;   // not in this axis's band (see pml_psi_load)
     const int bi = pml_bi<A>(S, iz, iy, ix);
     if (bi < 0) return dd;
     const int W = 2 * S.pml + 2;
@@ -298,8 +304,8 @@ __device__ __forceinline__ R pml_t(const Shot<R> &S, const R *bp, int iz, int iy
 // Strains of one column at a CPML band cell (see pml_d): the edge-guarded
 // taps of strains_col accumulated per derivative, scaled by 1/h, through
 // pml_t; ps: the cell's psi (components 0..2); write: this thread owns the
-// cell's memory variables.
-template <typename R, int VX>
+// cell's memory variables.  M: see pml_psi_load.
+template <typename R, int VX, int M = 7>
 __device__ __forceinline__ void strains_col_pml(const Shot<R> &S, const R *bp, const R (&ps)[9],
                                                 const R *tx_, const R *ty_, const R *tz_,
                                                 const R (&rx)[9], const R (&ry)[9],
@@ -324,15 +330,15 @@ __device__ __forceinline__ void strains_col_pml(const Shot<R> &S, const R *bp, c
     }
 #pragma unroll
     for (int t = 0; t < 9; ++t) d[t] = d[t] * S.inv_h;
-    a[0] = pml_t<2, 0, 0>(S, bp, zc, iy, ix, d[0], ps, write);
-    a[1] = pml_t<1, 0, 0>(S, bp, zc, iy, ix, d[1], ps, write);
-    a[2] = pml_t<0, 0, 0>(S, bp, zc, iy, ix, d[2], ps, write);
-    a[3] = pml_t<1, 1, 1>(S, bp, zc, iy, ix, d[3], ps, write) +
-           pml_t<0, 1, 1>(S, bp, zc, iy, ix, d[4], ps, write);
-    a[4] = pml_t<2, 1, 1>(S, bp, zc, iy, ix, d[5], ps, write) +
-           pml_t<0, 2, 1>(S, bp, zc, iy, ix, d[6], ps, write);
-    a[5] = pml_t<2, 2, 1>(S, bp, zc, iy, ix, d[7], ps, write) +
-           pml_t<1, 2, 1>(S, bp, zc, iy, ix, d[8], ps, write);
+    a[0] = pml_t<2, 0, 0, M>(S, bp, zc, iy, ix, d[0], ps, write);
+    a[1] = pml_t<1, 0, 0, M>(S, bp, zc, iy, ix, d[1], ps, write);
+    a[2] = pml_t<0, 0, 0, M>(S, bp, zc, iy, ix, d[2], ps, write);
+    a[3] = pml_t<1, 1, 1, M>(S, bp, zc, iy, ix, d[3], ps, write) +
+           pml_t<0, 1, 1, M>(S, bp, zc, iy, ix, d[4], ps, write);
+    a[4] = pml_t<2, 1, 1, M>(S, bp, zc, iy, ix, d[5], ps, write) +
+           pml_t<0, 2, 1, M>(S, bp, zc, iy, ix, d[6], ps, write);
+    a[5] = pml_t<2, 2, 1, M>(S, bp, zc, iy, ix, d[7], ps, write) +
+           pml_t<1, 2, 1, M>(S, bp, zc, iy, ix, d[8], ps, write);
 }
 
 // Stiffness rows are read in place.  A wave whose active lanes share one
