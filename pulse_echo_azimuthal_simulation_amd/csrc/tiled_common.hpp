@@ -220,21 +220,23 @@ __device__ __forceinline__ void pml_profiles_to_lds(const Shot<R> &S, R *bp, int
 }
 
 // Flat offset of cell (iz, iy, ix) in one psi component of axis A (band
-// index bi along A) and that component's volume.
+// index bi along A) and that component's volume.  A component is smaller
+// than a field, so the offset fits the 32-bit addressing of ldg / stg; the
+// component bases are wave-uniform.
 template <int A, typename R>
-__device__ __forceinline__ size_t pml_flat(const Shot<R> &S, int iz, int iy, int ix, int bi,
-                                           size_t &vol)
+__device__ __forceinline__ int pml_flat(const Shot<R> &S, int iz, int iy, int ix, int bi,
+                                        size_t &vol)
 {
     const int W = 2 * S.pml + 2;
     if constexpr (A == 0) {
         vol = (size_t)W * S.ny * S.nx;
-        return ((size_t)bi * S.ny + iy) * S.nx + ix;
+        return (bi * S.ny + iy) * S.nx + ix;
     } else if constexpr (A == 1) {
         vol = (size_t)S.nz * W * S.nx;
-        return ((size_t)iz * W + bi) * S.nx + ix;
+        return (iz * W + bi) * S.nx + ix;
     } else {
         vol = (size_t)S.nz * S.ny * W;
-        return ((size_t)iz * S.ny + iy) * W + bi;
+        return (iz * S.ny + iy) * W + bi;
     }
 }
 
@@ -260,11 +262,11 @@ __device__ __forceinline__ void pml_psi_load_axis(const Shot<R> &S, int iz, int 
         return;
     }
     size_t vol;
-    const size_t f = pml_flat<A>(S, iz, iy, ix, bi, vol);
-    const R *q = S.psi[A] + (size_t)S.pml_par * 6 * vol + (size_t)K0 * vol + f;
-    ps[3 * A] = q[0];
-    ps[3 * A + 1] = q[vol];
-    ps[3 * A + 2] = q[2 * vol];
+    const int f = pml_flat<A>(S, iz, iy, ix, bi, vol);
+    const R *q = S.psi[A] + ((size_t)S.pml_par * 6 + K0) * vol;
+    ps[3 * A] = ldg(q, f);
+    ps[3 * A + 1] = ldg(q + vol, f);
+    ps[3 * A + 2] = ldg(q + 2 * vol, f);
 }
 
 // M: the axes whose band the caller's tile can meet (bit 0 z, bit 1 y,
@@ -295,8 +297,8 @@ __device__ __forceinline__ R pml_t(const Shot<R> &S, const R *bp, int iz, int iy
     const R p = fma_r(pr[0], ps[3 * A + K % 3], pr[W] * dd);
     if (write) {
         size_t vol;
-        const size_t f = pml_flat<A>(S, iz, iy, ix, bi, vol);
-        S.psi[A][(size_t)(1 - S.pml_par) * 6 * vol + (size_t)K * vol + f] = p;
+        const int f = pml_flat<A>(S, iz, iy, ix, bi, vol);
+        stg(S.psi[A] + ((size_t)(1 - S.pml_par) * 6 + K) * vol, f, p);
     }
     return fma_r(dd, pr[2 * W], p);
 }
