@@ -52,7 +52,7 @@ def cpu_model():
     return "unknown CPU"
 
 
-def cpu_baseline(cfg, nz_sample=20, warm=2, timed=3):
+def cpu_baseline(cfg, nz_sample=20, warm=2, timed=20):
     """Upstream array-path loop (NumPy, float64, one thread) on a z-slab of
     the same specimen: cell-updates per second."""
     import numpy as np
