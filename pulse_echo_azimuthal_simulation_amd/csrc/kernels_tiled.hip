@@ -926,8 +926,9 @@ __global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_velocity_
 }
 
 // A second stream per device (default priority, created once) on which a
-// CPML half-step runs its band launches while the plain launch runs on the
-// caller's stream; the two sets of tiles write disjoint cells.  Calls that
+// CPML half-step runs its full band kernel (tiles in an x or y band) while
+// the plain and z-band launches run on the caller's stream; the sets of
+// tiles write disjoint cells.  Calls that
 // overlap on one device share it: a caller holds `use` from the fork record
 // to the join wait, so host threads enqueueing at the same time cannot
 // interleave their fork/join records (the lock covers enqueueing only).
@@ -1245,8 +1246,8 @@ template <>
 hipError_t launch_stress_tiled<float>(const Shot<float> &s, int v, int zc, hipStream_t st,
                                       bool g)
 {
-    // CPML shots run the variant-0 tile shape whatever the variant (smaller
-    // band tiles measured slower: the band path's redundant ring columns)
+    // CPML shots run the variant-0 tile shape whatever the variant (6-wave
+    // band tiles at 168 VGPRs measured 1.5-1.8x slower)
     if (s.pml > 0) return stress_launch<float, 32, 13, 24, false, true>(s, zc, st, g);
     switch (v) {
     case 1: return stress_launch<float, 62, 6, 0>(s, zc, st, g);
