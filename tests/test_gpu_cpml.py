@@ -61,6 +61,55 @@ def test_cpml_matches_oracle(torch_gpu, kernel, precision):
     assert H.rel_l2(tr, otr) <= 1e-6
 
 
+def _xy_band_free_tile(shape, L, TX, TY, half):
+    """Whether some TX x TY tile (stress: with its 1-column ring) lies clear
+    of the x and y bands: the tiles the plain and z band kernels take."""
+    nz, ny, nx = shape
+    r = 1 if half == 0 else 0
+    xs = [x0 for x0 in range(0, nx, TX) if x0 - r > L and x0 + TX - 1 + r < nx - 1 - L]
+    ys = [y0 for y0 in range(0, ny, TY) if y0 - r > L and y0 + TY - 1 + r < ny - 1 - L]
+    return bool(xs and ys)
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_cpml_every_band_kernel_matches_oracle(torch_gpu, precision):
+    """A grid on which each half-step gives tiles to all three tiled
+    kernels of a CPML shot: the plain kernel (tiles clear of every band),
+    the z band kernel (tiles in a z slab, clear of the x and y bands) and
+    the full band kernel; fields, traces and memory variables bit for bit."""
+    print("\n[gpu case] cpml every band kernel %s" % precision, flush=True)
+    from oracle import oracle as O
+    from pulse_echo_azimuthal_simulation_amd import _abi
+    from pulse_echo_azimuthal_simulation_amd.solver import DeviceShot
+    shape, L = (24, 40, 136), 4
+    for h in (0, 1):
+        tx, ty = _abi.tile_variants(precision, h)[0][:2]
+        assert _xy_band_free_tile(shape, L, tx, ty, h), (h, tx, ty)
+    assert shape[0] - (L + 2) > L + 2          # stress planes clear of the z bands exist
+    inp, cp = _medium(shape=shape, seed=13, L=L)
+    nt = inp["nt"]
+    sh = DeviceShot(inp["mats"], inp["Ct"], inp["rho_t"], inp["damp"], inp["coeffs"], inp["h"],
+                    inp["dt"], inp["src"][0], inp["src"][1], *inp["rec"], precision=precision,
+                    kernel="tiled", zchunk=(5, 5), cpml=cp)
+    wav = sh.wavelet(inp["wavelet"])
+    tr = sh.new_traces(nt)
+    sh.run(wav, 0, nt, tr)
+    v, s = sh.fields()
+    psi = [p.cpu().numpy() for p in sh.psi]
+    osh = O.OracleShot(inp["mats"], inp["Ct"], inp["rho_t"], inp["damp"], inp["coeffs"],
+                       inp["h"], inp["dt"], inp["src"], inp["rec"], precision=precision,
+                       cpml=(cp.width, cp.prof))
+    otr = osh.run(inp["wavelet"], 0, nt)
+    assert np.abs(otr).max() > 0
+    for k in range(3):
+        assert np.array_equal(v[k], osh.v[k]), "v%d" % k
+    for k in range(6):
+        assert np.array_equal(s[k], osh.s[k]), "s%d" % (k + 1)
+    for a in range(3):
+        assert np.array_equal(psi[a].reshape(osh.psi[a].shape), osh.psi[a]), "psi axis %d" % a
+    assert H.rel_l2(tr.cpu().numpy(), otr) <= 1e-6
+
+
 @pytest.mark.parametrize("precision", ["f32", "f64"])
 @pytest.mark.parametrize("nz", [10, 11])
 def test_cpml_thin_grid_without_band_free_middle(torch_gpu, nz, precision):
