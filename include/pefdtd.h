@@ -229,8 +229,12 @@ int pe_adjoint_gradient(const pe_shot *shot, const pe_adjoint *adj, double *grad
  * launches take no per-step host arguments and a captured graph of k steps
  * can be replayed until every shot is done (finished shots exit at once).
  * All shots share one precision and the tile variants of shots[0]; every
- * shot must be PE_KERNEL_TILED.  Results are bit-identical to pe_run of
- * each shot alone.
+ * shot must be PE_KERNEL_TILED.  Shots may have a CPML (cpml_width > 0, with
+ * zeroed memory variables; then shots[0] must use tile variant 0 for both
+ * half-steps): such a batch runs each half-step as three launches (the
+ * plain, z-band and full band kernels) over all workgroups, and takes the
+ * memory-variable buffer from each shot's step counter.  Results are
+ * bit-identical to pe_run of each shot alone.
  * ------------------------------------------------------------------------ */
 typedef struct pe_batch {
     int32_t n_shot;

@@ -170,8 +170,9 @@ int batch_layout(const pe_shot *shots, int32_t n, BatchLayout *L)
         if (p->precision != prec) return fail(PE_EINVAL, "shot %d: precision differs from shot 0", i);
         if (p->kernel != PE_KERNEL_TILED)
             return fail(PE_EINVAL, "shot %d: batches run the tiled kernels only", i);
-        if (p->cpml_width > 0)
-            return fail(PE_EINVAL, "shot %d: batches run the sponge boundary only (no CPML)", i);
+        if (p->cpml_width > 0 && (shots[0].tile[0] != 0 || shots[0].tile[1] != 0))
+            return fail(PE_EINVAL, "shot %d: a batch with CPML shots runs tile variant 0 "
+                                   "(shot 0 has %d, %d)", i, shots[0].tile[0], shots[0].tile[1]);
         for (int h = 0; h < 2; ++h) {
             int tx, ty, zl;
             const int nb = pe::tiled_geometry(prec, h, shots[0].tile[h], p->zchunk[h], p->nz, p->ny,
@@ -240,12 +241,21 @@ int batch_step_impl(const pe_batch *b, const BatchLayout &L, int32_t steps, hipS
     const auto *blk_s = reinterpret_cast<const int32_t *>((const char *)b->table + L.blk_s_off);
     const auto *blk_v = reinterpret_cast<const int32_t *>((const char *)b->table + L.blk_v_off);
     const int vs = b->shots[0].tile[0], vv = b->shots[0].tile[1];
-    int max_mat = 1;
-    for (int32_t i = 0; i < b->n_shot; ++i) max_mat = std::max(max_mat, (int)b->shots[i].n_mat);
+    int max_mat = 1, max_pml = 0;
+    for (int32_t i = 0; i < b->n_shot; ++i) {
+        max_mat = std::max(max_mat, (int)b->shots[i].n_mat);
+        max_pml = std::max(max_pml, (int)b->shots[i].cpml_width);
+    }
     for (int32_t k = 0; k < steps; ++k) {
-        hipError_t e = pe::launch_stress_batched<R>(vs, tab, blk_s, (int)L.nb[0], b->step, st);
+        hipError_t e = max_pml > 0
+            ? pe::launch_stress_batched_pml<R>(tab, blk_s, (int)L.nb[0], b->step, st, max_pml)
+            : pe::launch_stress_batched<R>(vs, tab, blk_s, (int)L.nb[0], b->step, st);
         if (e == hipSuccess) e = pe::launch_source_record_batched<R>(tab, b->n_shot, b->step, 0, st);
-        if (e == hipSuccess) e = pe::launch_velocity_batched<R>(vv, tab, blk_v, (int)L.nb[1], b->step, st, max_mat);
+        if (e == hipSuccess)
+            e = max_pml > 0 ? pe::launch_velocity_batched_pml<R>(tab, blk_v, (int)L.nb[1], b->step,
+                                                                 st, max_mat, max_pml)
+                            : pe::launch_velocity_batched<R>(vv, tab, blk_v, (int)L.nb[1], b->step,
+                                                             st, max_mat);
         if (e == hipSuccess) e = pe::launch_source_record_batched<R>(tab, b->n_shot, b->step, 1, st);
         if (e != hipSuccess) return hip_fail(e, "batched step launch");
     }

@@ -223,6 +223,15 @@ hipError_t launch_stress_batched(int variant, const BatchEntry<R> *tab, const in
 template <typename R>
 hipError_t launch_velocity_batched(int variant, const BatchEntry<R> *tab, const int32_t *blk,
                                    int nb, const int32_t *step, hipStream_t st, int max_mat);
+// The same for a batch with CPML shots (tile variant 0; max_pml: the widest
+// band of the batch).
+template <typename R>
+hipError_t launch_stress_batched_pml(const BatchEntry<R> *tab, const int32_t *blk, int nb,
+                                     const int32_t *step, hipStream_t st, int max_pml);
+template <typename R>
+hipError_t launch_velocity_batched_pml(const BatchEntry<R> *tab, const int32_t *blk, int nb,
+                                       const int32_t *step, hipStream_t st, int max_mat,
+                                       int max_pml);
 // Source / record of every active shot (phase 0, after the stress
 // half-step) and velocity-mode record plus step advance (phase 1, after the
 // velocity half-step); one workgroup per shot.

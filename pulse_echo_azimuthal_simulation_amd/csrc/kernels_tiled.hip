@@ -880,20 +880,34 @@ k_stress_pipe(Shot<R> S, int tiles_x, int tiles_y, int zc_len, int zlo, int zhi)
 
 // Multi-shot kernels: the workgroups of every shot of a batch in one launch.
 // blk_shot maps a (remapped) workgroup id to its shot; a shot whose step
-// counter has reached its step count exits at once.
-template <typename R, int TX, int TY, int PAD, bool FR = false>
+// counter has reached its step count exits at once.  A batch with CPML shots
+// runs the band kernels PM = 0, 1, 7 one after another over every
+// workgroup; a tile of a CPML shot runs in the kernel of its band class
+// (stress_band_kernel) and a sponge shot's tiles in kernel 0.  The memory
+// variables' buffer parity is the shot's own step counter.
+template <typename R, int TX, int TY, int PAD, bool FR = false, int PM = 0>
 __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
-                             (MinWaves<R, StressTile<R, TX, TY, PAD>::NT>::v)) void k_stress_batched(
-    const BatchEntry<R> *__restrict__ tab, const int32_t *__restrict__ blk_shot, int nb_total,
-    const int32_t *__restrict__ step)
+                             (ClassWaves<R, StressTile<R, TX, TY, PAD>::NT, PM>::v)) void
+k_stress_batched(const BatchEntry<R> *__restrict__ tab, const int32_t *__restrict__ blk_shot,
+                 int nb_total, const int32_t *__restrict__ step)
 {
     const int g = xcd_remap(blockIdx.x, nb_total);
     const int s = blk_shot[g];
     const BatchEntry<R> *e = tab + s;
-    if (step[s] >= e->nt) return;
-    const Shot<R> S = e->S;
-    stress_tiled_body<R, TX, TY, PAD, false, FR>(S, e->tiles_x[0], e->tiles_y[0], e->zc_len[0],
-                                                 g - e->blk_off[0], 0, S.nz);
+    const int n = step[s];
+    if (n >= e->nt) return;
+    Shot<R> S = e->S;
+    const int t = g - e->blk_off[0];
+    if (S.pml > 0) {
+        if (stress_band_kernel<R, TX, TY>(S, t, e->tiles_x[0], e->tiles_y[0], e->zc_len[0], 0,
+                                          S.nz) != PM)
+            return;
+        S.pml_par = n & 1;
+    } else if (PM != 0) {
+        return;
+    }
+    stress_tiled_body<R, TX, TY, PAD, false, FR, PM>(S, e->tiles_x[0], e->tiles_y[0],
+                                                     e->zc_len[0], t, 0, S.nz);
 }
 
 template <typename R, int TX, int TY>
@@ -911,18 +925,28 @@ k_stress_pipe_batched(const BatchEntry<R> *__restrict__ tab, const int32_t *__re
                                 0, S.nz);
 }
 
-template <typename R, int TX, int TY, bool RHO_LDS = false>
-__global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_velocity_batched(
+template <typename R, int TX, int TY, bool RHO_LDS = false, int PM = 0>
+__global__ __launch_bounds__(TX *TY, (ClassWaves<R, TX * TY, PM>::v)) void k_velocity_batched(
     const BatchEntry<R> *__restrict__ tab, const int32_t *__restrict__ blk_shot, int nb_total,
     const int32_t *__restrict__ step)
 {
     const int g = xcd_remap(blockIdx.x, nb_total);
     const int s = blk_shot[g];
     const BatchEntry<R> *e = tab + s;
-    if (step[s] >= e->nt) return;
-    const Shot<R> S = e->S;
-    velocity_tiled_body<R, TX, TY, RHO_LDS>(S, e->tiles_x[1], e->tiles_y[1], e->zc_len[1],
-                                            g - e->blk_off[1], 0, S.nz);
+    const int n = step[s];
+    if (n >= e->nt) return;
+    Shot<R> S = e->S;
+    const int t = g - e->blk_off[1];
+    if (S.pml > 0) {
+        if (velocity_band_kernel<R, TX, TY>(S, t, e->tiles_x[1], e->tiles_y[1], e->zc_len[1], 0,
+                                            S.nz) != PM)
+            return;
+        S.pml_par = n & 1;
+    } else if (PM != 0) {
+        return;
+    }
+    velocity_tiled_body<R, TX, TY, RHO_LDS, PM>(S, e->tiles_x[1], e->tiles_y[1], e->zc_len[1], t,
+                                                0, S.nz);
 }
 
 // A second stream per device (default priority, created once) on which a
@@ -1131,6 +1155,55 @@ hipError_t velocity_batched_launch(const BatchEntry<R> *tab, const int32_t *blk,
     return hipGetLastError();
 }
 
+// CPML batches: the three band kernels over every workgroup, variant-0
+// tiles; max_pml sizes the LDS band-profile table for the widest band.
+template <typename R, int TX, int TY, int PAD>
+hipError_t stress_batched_pml_launch(const BatchEntry<R> *tab, const int32_t *blk, int nb,
+                                     const int32_t *step, hipStream_t st, int max_pml)
+{
+    using T = StressTile<R, TX, TY, PAD>;
+    const size_t lds_pml = T::LDS + sizeof(R) * 18 * (2 * (size_t)max_pml + 2);
+    auto go = [&](auto kern, size_t lds) -> hipError_t {
+        hipError_t e = allow_lds(kern, lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(T::NT), lds, st, tab, blk, nb, step);
+        return hipGetLastError();
+    };
+    hipError_t e = go(k_stress_batched<R, TX, TY, PAD, false, 0>, T::LDS);
+    if (e == hipSuccess) e = go(k_stress_batched<R, TX, TY, PAD, false, 1>, lds_pml);
+    if (e == hipSuccess) e = go(k_stress_batched<R, TX, TY, PAD, false, 7>, lds_pml);
+    return e;
+}
+
+template <typename R, int TX, int TY>
+hipError_t velocity_batched_pml_launch(const BatchEntry<R> *tab, const int32_t *blk, int nb,
+                                       const int32_t *step, hipStream_t st, int max_mat,
+                                       int max_pml)
+{
+    using T = VelTile<R, TX, TY>;
+    const size_t rho_bytes = sizeof(R) * (size_t)max_mat;
+    const bool rho_lds = rho_bytes <= (size_t)kRhoLdsBytes;
+    const size_t lds = T::LDS + (rho_lds ? rho_bytes : 0);
+    const size_t lds_pml = lds + sizeof(R) * 18 * (2 * (size_t)max_pml + 2);
+    auto go = [&](auto kern, size_t l) -> hipError_t {
+        hipError_t e = allow_lds(kern, l);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(T::NT), l, st, tab, blk, nb, step);
+        return hipGetLastError();
+    };
+    hipError_t e;
+    if (rho_lds) {
+        e = go(k_velocity_batched<R, TX, TY, true, 0>, lds);
+        if (e == hipSuccess) e = go(k_velocity_batched<R, TX, TY, true, 1>, lds_pml);
+        if (e == hipSuccess) e = go(k_velocity_batched<R, TX, TY, true, 7>, lds_pml);
+    } else {
+        e = go(k_velocity_batched<R, TX, TY, false, 0>, lds);
+        if (e == hipSuccess) e = go(k_velocity_batched<R, TX, TY, false, 1>, lds_pml);
+        if (e == hipSuccess) e = go(k_velocity_batched<R, TX, TY, false, 7>, lds_pml);
+    }
+    return e;
+}
+
 // Tile variants (TX, TY, LDS row padding); variant 0 is the default.
 // fr: 1 = the stress kernel's frame ring (StressTile), 2 = the one-barrier
 // pipelined stress body (stress_pipe_body).
@@ -1240,6 +1313,38 @@ hipError_t launch_velocity_batched<double>(int v, const BatchEntry<double> *tab,
     case 3: return velocity_batched_launch<double, 32, 4>(tab, blk, nb, step, st, max_mat);
     default: return velocity_batched_launch<double, 32, 8>(tab, blk, nb, step, st, max_mat);
     }
+}
+
+template <>
+hipError_t launch_stress_batched_pml<float>(const BatchEntry<float> *tab, const int32_t *blk,
+                                            int nb, const int32_t *step, hipStream_t st,
+                                            int max_pml)
+{
+    return stress_batched_pml_launch<float, 32, 13, 24>(tab, blk, nb, step, st, max_pml);
+}
+
+template <>
+hipError_t launch_stress_batched_pml<double>(const BatchEntry<double> *tab, const int32_t *blk,
+                                             int nb, const int32_t *step, hipStream_t st,
+                                             int max_pml)
+{
+    return stress_batched_pml_launch<double, 32, 6, 0>(tab, blk, nb, step, st, max_pml);
+}
+
+template <>
+hipError_t launch_velocity_batched_pml<float>(const BatchEntry<float> *tab, const int32_t *blk,
+                                              int nb, const int32_t *step, hipStream_t st,
+                                              int max_mat, int max_pml)
+{
+    return velocity_batched_pml_launch<float, 64, 8>(tab, blk, nb, step, st, max_mat, max_pml);
+}
+
+template <>
+hipError_t launch_velocity_batched_pml<double>(const BatchEntry<double> *tab, const int32_t *blk,
+                                               int nb, const int32_t *step, hipStream_t st,
+                                               int max_mat, int max_pml)
+{
+    return velocity_batched_pml_launch<double, 32, 8>(tab, blk, nb, step, st, max_mat, max_pml);
 }
 
 template <>

@@ -151,3 +151,18 @@ def test_cpml_is_refused_by_adjoint_and_history():
     assert "CPML" in lib.pe_last_error().decode()
     assert lib.pe_run_store_strains(C.byref(d), 0, 1, fake, None, fake, None) == _abi.PE_EINVAL
     assert "CPML" in lib.pe_last_error().decode()
+
+
+def test_batch_with_cpml_needs_tile_variant_zero():
+    """A batch accepts CPML shots with tile variant 0 and refuses others."""
+    lib = _abi.load()
+    fake = 0x1000
+    d = _desc(cpml_width=1)
+    for a in range(3):
+        d.cpml_prof[a] = fake
+        d.cpml_psi[a] = fake
+    arr = (_abi.PeShot * 2)(d, _desc())
+    assert lib.pe_batch_table_bytes(arr, 2) > 0
+    arr[0].tile[1] = 2
+    assert lib.pe_batch_table_bytes(arr, 2) == 0
+    assert "tile variant 0" in lib.pe_last_error().decode()
