@@ -115,6 +115,9 @@ def main():
     ap.add_argument("--sponge-steps", type=int, default=2000,
                     help="forward cpml runs: also time this many steps with the 20-cell "
                          "sponge (0 = skip)")
+    ap.add_argument("--sweep-boundary", default="sponge", choices=["sponge", "cpml"],
+                    help="--mode sweep: the upstream sweep's 20-cell sponge (default) or a "
+                         "20-cell CPML")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--launch", default=None,
                     help="fixed launch shapes 'stress_variant,stress_zchunk,vel_variant,vel_zchunk' "
@@ -318,7 +321,8 @@ def bench_sweep(args):
     single-shot benchmark (h = 0.285 mm) rotated under a +x rim transducer at
     N azimuths (360/N deg apart), each a (169, 397, 397) grid, plus 8 smaller
     seeded disks whose grids run from 128x128x64 to 256x256x128; every shot
-    runs --steps steps (2.25 MHz Ricker, 20-cell sponge, couplant mask).
+    runs --steps steps (2.25 MHz Ricker, couplant mask, and the 20-cell sponge
+    or with --sweep-boundary cpml a 20-cell CPML).
     Batched: every shot prepared on the GPU (re-grid, distance transform,
     dt) and all shots advanced together in one multi-shot launch per
     half-step, replayed as a hipGraph.  One at a time: the upstream sweep
@@ -334,6 +338,7 @@ def bench_sweep(args):
     nt = max(args.steps, 1)
     f0 = 2.25e6
     h = specimen.C_REF / f0 / 6
+    cw = 20 if args.sweep_boundary == "cpml" else 0
     co = host.optimised_coeffs(8, kh_max=2.0, multistart=True)
     t0 = time.perf_counter()
     big = sweep.disk_build(100e-3, 35e-3, h, 3000, seed=11, kappa=-3.0,
@@ -351,7 +356,7 @@ def bench_sweep(args):
     # ---------------- batched
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    prep_big = sweep.GpuAzimuthPrep(big, h, co, f0, sponge=20)
+    prep_big = sweep.GpuAzimuthPrep(big, h, co, f0, sponge=20, cpml_width=cw)
     tables = sweep.rotated_tables(big["axes"], azs)
     shots, wavs, nts = [], [], []
     for az, tab in zip(azs, tables):
@@ -360,7 +365,8 @@ def bench_sweep(args):
         wavs.append(wav)
         nts.append(n_)
     for b in smalls:
-        sh, wav, n_, _ = sweep.GpuAzimuthPrep(b, h, co, f0, sponge=20).shot(0.0, nt=nt)
+        sh, wav, n_, _ = sweep.GpuAzimuthPrep(b, h, co, f0, sponge=20,
+                                              cpml_width=cw).shot(0.0, nt=nt)
         shots.append(sh)
         wavs.append(wav)
         nts.append(n_)
@@ -368,6 +374,10 @@ def bench_sweep(args):
     t_prep = time.perf_counter() - t0
     t1 = time.perf_counter()
     best, _ = autotune.tune(shots[0], reps=3)
+    if cw:
+        # a batch with CPML shots runs tile variant 0 (its z-chunk as tuned
+        # if the autotuner chose that variant)
+        best = {hh: best[hh] if best[hh][0] == 0 else (0, 0, best[hh][2]) for hh in (0, 1)}
     for sh in shots:
         for hh in (0, 1):
             sh.desc.tile[hh] = best[hh][0]
@@ -397,6 +407,8 @@ def bench_sweep(args):
     torch.cuda.empty_cache()
 
     # ---------------- one at a time (sample, extrapolated)
+    plain_boundary = (dict(sponge_width=0, boundary="cpml", cpml_width=cw, cpml_f0=f0) if cw
+                      else dict(sponge_width=20))
     plain_az, plain_small = [], []
     match = True
     for k in range(min(args.plain_sample, len(azs))):
@@ -406,8 +418,8 @@ def bench_sweep(args):
         a = sweep.azimuth_shot(big, h, az, co, f0, sponge=20, nt=nt)
         tr = solver.forward_fused_labels(a["lab"], a["axes"], h, a["dt"], nt, a["src_pts"],
                                          a["wavelet"], a["rec_groups"], coeffs=co,
-                                         sponge_width=20, damp_mask=a["damp_mask"],
-                                         mat_tables=a["mat_tables"])
+                                         damp_mask=a["damp_mask"], mat_tables=a["mat_tables"],
+                                         **plain_boundary)
         plain_az.append(time.perf_counter() - ts)
         ref = tr_batch[azs.index(az)]
         match &= bool(np.allclose(tr, ref, rtol=0, atol=1e-6 * np.abs(ref).max()))
@@ -416,8 +428,9 @@ def bench_sweep(args):
         ts = time.perf_counter()
         a = sweep.azimuth_shot(smalls[k], h, 0.0, co, f0, sponge=20, nt=nt)
         solver.forward_fused_labels(a["lab"], a["axes"], h, a["dt"], nt, a["src_pts"],
-                                    a["wavelet"], a["rec_groups"], coeffs=co, sponge_width=20,
-                                    damp_mask=a["damp_mask"], mat_tables=a["mat_tables"])
+                                    a["wavelet"], a["rec_groups"], coeffs=co,
+                                    damp_mask=a["damp_mask"], mat_tables=a["mat_tables"],
+                                    **plain_boundary)
         plain_small.append(time.perf_counter() - ts)
     t_plain = float(np.mean(plain_az)) * len(azs) + float(np.mean(plain_small)) * len(smalls)
     log("one at a time: %.2f s per azimuth, %.2f s per small disk (sampled) -> %.1f s; "
@@ -431,7 +444,8 @@ def bench_sweep(args):
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": "batched azimuth sweep: 100x35 mm 3000-grain disk at %d azimuths "
-                               "(169x397x397 each) + 8 disks 128x128x64..256x256x128" % len(azs),
+                               "(169x397x397 each) + 8 disks 128x128x64..256x256x128, 20-cell %s"
+                               % (len(azs), "CPML" if cw else "sponge"),
                    "shots": len(azs) + len(smalls), "mode": "sweep",
                    "launch": {"stress": best[0][:2], "velocity": best[1][:2]}},
         "batched_s": {"prep": round(t_prep, 3), "tune": round(t_tune, 3),

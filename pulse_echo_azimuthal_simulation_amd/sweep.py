@@ -171,17 +171,21 @@ class GpuAzimuthPrep:
     (one plane of labels) are formed on the host.  Gives the same labels,
     dt, footprint and damping volume as ``azimuth_shot`` (nearest-neighbour
     re-grid, or with ``single_raster`` the power diagram evaluated at the
-    rotated solver points).
+    rotated solver points).  ``cpml_width`` > 0 absorbs with a CPML of that
+    width instead of the sponge (the disk is padded by cpml_width + 3 couplant
+    cells, as by sponge + 3 otherwise; profiles from the azimuth's dt and
+    tables, R = 1e-4, kappa = 1, f0 the wavelet's).
     """
 
     def __init__(self, build, h, coeffs, f0, sponge=10, fluid_damp=0.02, element_d_mm=6.35,
-                 record_factor=2.7, single_raster=False, device=None):
+                 record_factor=2.7, single_raster=False, device=None, cpml_width=0):
         import torch
         self.torch = torch
         self.dev = torch.device("cuda", torch.cuda.current_device() if device is None else device)
         self.build = build
         self.h, self.coeffs, self.f0 = float(h), np.asarray(coeffs, float), float(f0)
         self.sponge, self.fluid_damp = int(sponge), float(fluid_damp)
+        self.cpml_width = int(cpml_width)
         self.element_d_mm, self.record_factor = float(element_d_mm), float(record_factor)
         lab = np.ascontiguousarray(build["labels"], np.int32)
         self.src_shape = lab.shape
@@ -203,7 +207,8 @@ class GpuAzimuthPrep:
         nd = int(round(D / self.h))
         ndz = int(round(nz_s * h_spec / self.h))
         a = np.radians(az)
-        return nd, ndz, self.sponge + 3, float(np.cos(a)), float(np.sin(a))
+        pad = (self.cpml_width if self.cpml_width > 0 else self.sponge) + 3
+        return nd, ndz, pad, float(np.cos(a)), float(np.sin(a))
 
     def labels(self, az):
         """(lab device int32 (nz, n, n), nd, m, axes_rot)."""
@@ -235,8 +240,9 @@ class GpuAzimuthPrep:
         if self._scratch is None or self._scratch[0].numel() < ncell:
             self._scratch = [torch.empty(ncell, dtype=torch.int32, device=self.dev)
                              for _ in range(2)]
+        width = 0 if self.cpml_width > 0 else self.sponge
         pz, py, px = [torch.as_tensor(p, device=self.dev)
-                      for p in host.sponge_axis_profiles((nz, ny, nx), self.sponge)]
+                      for p in host.sponge_axis_profiles((nz, ny, nx), width)]
         damp = torch.empty((nz, ny, nx), dtype=torch.float32, device=self.dev)
         _abi.check(self.lib.pe_prep_edt_damp(lab.data_ptr(), nz, ny, nx, self.fluid_damp,
                                              pz.data_ptr(), py.data_ptr(), px.data_ptr(),
@@ -284,9 +290,14 @@ class GpuAzimuthPrep:
         shape = tuple(lab.shape)
         src_idx = np.array([np.ravel_multi_index(p, shape) for p in pts], np.int64)
         rec_ptr, rec_idx, rec_w = _csr([(pts, np.full(len(pts), w))], shape)
+        cp = None
+        if self.cpml_width > 0:
+            from . import cpml
+            cp = cpml.Cpml(shape, self.cpml_width, self.h, dt, cpml.vmax_table(Ct, rho_t),
+                           self.f0)
         shot = DeviceShot(mats, Ct, rho_t, damp, self.coeffs, self.h, dt, src_idx,
                           np.full(len(pts), w), rec_ptr, rec_idx, rec_w, precision=precision,
-                          kernel=kernel)
+                          kernel=kernel, cpml=cp)
         wav = shot.wavelet(host.ricker(self.f0, dt, int(nt)))
         return shot, wav, int(nt), dict(dt=dt, nd=nd, m=m, pts=pts, axes=axes_rot, lab=lab)
 
