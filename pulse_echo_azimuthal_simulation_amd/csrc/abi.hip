@@ -203,6 +203,9 @@ int batch_setup_impl(pe_batch *b, const BatchLayout &L)
         const pe_shot *p = &b->shots[i];
         pe::BatchEntry<R> &e = ent[i];
         e.S = pe::make_shot<R>(p);
+        e.S.pml_par = 0;
+        e.S_odd = e.S;
+        e.S_odd.pml_par = 1;
         e.nt = b->nt[i];
         for (int h = 0; h < 2; ++h) {
             const int nb = pe::tiled_geometry(p->precision, h, b->shots[0].tile[h], p->zchunk[h],

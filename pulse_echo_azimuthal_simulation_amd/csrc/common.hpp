@@ -180,7 +180,8 @@ __device__ __forceinline__ R moved_direct(const R *__restrict__ gJ, int i, int i
 // first workgroup in the batched launch, and its source / receiver arrays.
 template <typename R>
 struct BatchEntry {
-    Shot<R> S;
+    Shot<R> S;          // the shot (CPML memory-variable buffer parity 0: even steps)
+    Shot<R> S_odd;      // the same with parity 1 (odd steps)
     int nt;
     int tiles_x[2], tiles_y[2], zc_len[2], blk_off[2];
     int n_src, n_rec, mode;

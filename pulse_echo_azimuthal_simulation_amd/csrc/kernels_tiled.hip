@@ -884,7 +884,9 @@ k_stress_pipe(Shot<R> S, int tiles_x, int tiles_y, int zc_len, int zlo, int zhi)
 // runs the band kernels PM = 0, 1, 7 one after another over every
 // workgroup; a tile of a CPML shot runs in the kernel of its band class
 // (stress_band_kernel) and a sponge shot's tiles in kernel 0.  The memory
-// variables' buffer parity is the shot's own step counter.
+// variables' buffer parity is that of the shot's own step counter (the entry
+// holds the shot once per parity, so the kernel reads it in place: a
+// modified local copy would be kept in scratch memory).
 template <typename R, int TX, int TY, int PAD, bool FR = false, int PM = 0>
 __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
                              (ClassWaves<R, StressTile<R, TX, TY, PAD>::NT, PM>::v)) void
@@ -896,13 +898,12 @@ k_stress_batched(const BatchEntry<R> *__restrict__ tab, const int32_t *__restric
     const BatchEntry<R> *e = tab + s;
     const int n = step[s];
     if (n >= e->nt) return;
-    Shot<R> S = e->S;
+    const Shot<R> &S = (PM != 0 && (n & 1)) ? e->S_odd : e->S;
     const int t = g - e->blk_off[0];
     if (S.pml > 0) {
         if (stress_band_kernel<R, TX, TY>(S, t, e->tiles_x[0], e->tiles_y[0], e->zc_len[0], 0,
                                           S.nz) != PM)
             return;
-        S.pml_par = n & 1;
     } else if (PM != 0) {
         return;
     }
@@ -935,13 +936,12 @@ __global__ __launch_bounds__(TX *TY, (ClassWaves<R, TX * TY, PM>::v)) void k_vel
     const BatchEntry<R> *e = tab + s;
     const int n = step[s];
     if (n >= e->nt) return;
-    Shot<R> S = e->S;
+    const Shot<R> &S = (PM != 0 && (n & 1)) ? e->S_odd : e->S;
     const int t = g - e->blk_off[1];
     if (S.pml > 0) {
         if (velocity_band_kernel<R, TX, TY>(S, t, e->tiles_x[1], e->tiles_y[1], e->zc_len[1], 0,
                                             S.nz) != PM)
             return;
-        S.pml_par = n & 1;
     } else if (PM != 0) {
         return;
     }
@@ -1156,23 +1156,45 @@ hipError_t velocity_batched_launch(const BatchEntry<R> *tab, const int32_t *blk,
 }
 
 // CPML batches: the three band kernels over every workgroup, variant-0
-// tiles; max_pml sizes the LDS band-profile table for the widest band.
+// tiles; max_pml sizes the LDS band-profile table for the widest band.  As
+// for a single shot, kernels 0 and 1 run on the caller's stream and kernel 7
+// concurrently on the side stream (fork / join by events, which a graph
+// capture of the caller's stream follows).
+template <typename R, int NT, typename K0, typename K1, typename K7>
+hipError_t batched_pml_fork(K0 k0, K1 k1, K7 k7, size_t lds, size_t lds_pml,
+                            const BatchEntry<R> *tab, const int32_t *blk, int nb,
+                            const int32_t *step, hipStream_t st)
+{
+    auto go = [&](auto kern, size_t l, hipStream_t q) -> hipError_t {
+        hipError_t e = allow_lds(kern, l);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(NT), l, q, tab, blk, nb, step);
+        return hipGetLastError();
+    };
+    SideStream *ss = nullptr;
+    hipError_t e = side_stream(&ss);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> hold(ss->use);
+    e = hipEventRecord(ss->fork, st);
+    if (e == hipSuccess) e = hipStreamWaitEvent(ss->s, ss->fork, 0);
+    if (e == hipSuccess) e = go(k0, lds, st);
+    if (e == hipSuccess) e = go(k1, lds_pml, st);
+    if (e == hipSuccess) e = go(k7, lds_pml, ss->s);
+    if (e == hipSuccess) e = hipEventRecord(ss->join, ss->s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(st, ss->join, 0);
+    return e;
+}
+
 template <typename R, int TX, int TY, int PAD>
 hipError_t stress_batched_pml_launch(const BatchEntry<R> *tab, const int32_t *blk, int nb,
                                      const int32_t *step, hipStream_t st, int max_pml)
 {
     using T = StressTile<R, TX, TY, PAD>;
     const size_t lds_pml = T::LDS + sizeof(R) * 18 * (2 * (size_t)max_pml + 2);
-    auto go = [&](auto kern, size_t lds) -> hipError_t {
-        hipError_t e = allow_lds(kern, lds);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(kern, dim3(nb), dim3(T::NT), lds, st, tab, blk, nb, step);
-        return hipGetLastError();
-    };
-    hipError_t e = go(k_stress_batched<R, TX, TY, PAD, false, 0>, T::LDS);
-    if (e == hipSuccess) e = go(k_stress_batched<R, TX, TY, PAD, false, 1>, lds_pml);
-    if (e == hipSuccess) e = go(k_stress_batched<R, TX, TY, PAD, false, 7>, lds_pml);
-    return e;
+    return batched_pml_fork<R, T::NT>(k_stress_batched<R, TX, TY, PAD, false, 0>,
+                                      k_stress_batched<R, TX, TY, PAD, false, 1>,
+                                      k_stress_batched<R, TX, TY, PAD, false, 7>, T::LDS, lds_pml,
+                                      tab, blk, nb, step, st);
 }
 
 template <typename R, int TX, int TY>
@@ -1185,23 +1207,15 @@ hipError_t velocity_batched_pml_launch(const BatchEntry<R> *tab, const int32_t *
     const bool rho_lds = rho_bytes <= (size_t)kRhoLdsBytes;
     const size_t lds = T::LDS + (rho_lds ? rho_bytes : 0);
     const size_t lds_pml = lds + sizeof(R) * 18 * (2 * (size_t)max_pml + 2);
-    auto go = [&](auto kern, size_t l) -> hipError_t {
-        hipError_t e = allow_lds(kern, l);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(kern, dim3(nb), dim3(T::NT), l, st, tab, blk, nb, step);
-        return hipGetLastError();
-    };
-    hipError_t e;
-    if (rho_lds) {
-        e = go(k_velocity_batched<R, TX, TY, true, 0>, lds);
-        if (e == hipSuccess) e = go(k_velocity_batched<R, TX, TY, true, 1>, lds_pml);
-        if (e == hipSuccess) e = go(k_velocity_batched<R, TX, TY, true, 7>, lds_pml);
-    } else {
-        e = go(k_velocity_batched<R, TX, TY, false, 0>, lds);
-        if (e == hipSuccess) e = go(k_velocity_batched<R, TX, TY, false, 1>, lds_pml);
-        if (e == hipSuccess) e = go(k_velocity_batched<R, TX, TY, false, 7>, lds_pml);
-    }
-    return e;
+    if (rho_lds)
+        return batched_pml_fork<R, T::NT>(k_velocity_batched<R, TX, TY, true, 0>,
+                                          k_velocity_batched<R, TX, TY, true, 1>,
+                                          k_velocity_batched<R, TX, TY, true, 7>, lds, lds_pml,
+                                          tab, blk, nb, step, st);
+    return batched_pml_fork<R, T::NT>(k_velocity_batched<R, TX, TY, false, 0>,
+                                      k_velocity_batched<R, TX, TY, false, 1>,
+                                      k_velocity_batched<R, TX, TY, false, 7>, lds, lds_pml, tab,
+                                      blk, nb, step, st);
 }
 
 // Tile variants (TX, TY, LDS row padding); variant 0 is the default.
