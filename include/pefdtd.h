@@ -85,7 +85,9 @@ typedef struct pe_shot {
      *                 components 0..2 belong to the stress half-step, 3..5
      *                 to the velocity half-step
      * Step n reads buffer n & 1 and writes the other; pe_run takes n from
-     * its loop, pe_stress / pe_velocity from cpml_step.  Forward only. */
+     * its loop, pe_stress / pe_velocity from cpml_step, a batch from each
+     * shot's step counter.  Forward runs and batches only (the adjoint and
+     * the strain history refuse it). */
     int32_t cpml_width;
     int32_t cpml_step;
     const void *cpml_prof[3];
