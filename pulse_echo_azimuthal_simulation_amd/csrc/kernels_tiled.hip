@@ -726,7 +726,7 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
     load_frame(z0);
     load_col(z0);
 
-    for (int z = z0; z < z1; ++z) {
+    auto step = [&](const int z) __attribute__((always_inline)) {
         const int i = z * sz + col;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
@@ -796,7 +796,17 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
             stg(S.v[2], i, (vzo + tstep * az / rvz) * dmp);
         }
         lds_barrier();
+    };
+    // two planes per iteration: the loads one plane issues feed the next
+    // without the register moves of a loop latch, each of which waits for
+    // its load (measured 0.481 -> 0.457 ms at 400x400x200, 64x8 tiles; the
+    // kernel stays within the registers of its occupancy)
+    int z = z0;
+    for (; z + 1 < z1; z += 2) {
+        step(z);
+        step(z + 1);
     }
+    if (z < z1) step(z);
 }
 
 // ------------------------------------------------------------- launchers
