@@ -84,6 +84,77 @@ def cpu_baseline(cfg, nz_sample=20, warm=2, timed=20):
                        % (nz_sample, ny, nx, cells / 1e6, timed, warm, cpu_model(), dt / timed))
 
 
+def forward_shot(cfg, boundary, precision, kernel="tiled"):
+    """DeviceShot of a forward workload (specimen.config_c1 / config_c0) with
+    the 20-cell CPML ("cpml", the configurations' boundary) or sponge."""
+    from pulse_echo_azimuthal_simulation_amd import solver
+    cpml_w = cfg["sponge_width"] if boundary == "cpml" else 0
+    prep = solver.prepare_shot(cfg["lab"], cfg["axes"], cfg["h"], cfg["dt"], cfg["src_pts"],
+                               cfg["rec_groups"], coeffs=cfg["coeffs"],
+                               sponge_width=0 if boundary == "cpml" else cfg["sponge_width"],
+                               damp_mask=cfg["damp_mask"], mat_tables=cfg["mat_tables"])
+    cp = None
+    if cpml_w:
+        from pulse_echo_azimuthal_simulation_amd import cpml
+        cp = cpml.Cpml(prep["mats"].shape, cpml_w, cfg["h"], cfg["dt"],
+                       cpml.vmax_table(prep["Ct"], prep["rho_t"]),
+                       cpml.dominant_frequency(cfg["wavelet"], cfg["dt"]))
+    return solver.DeviceShot(prep["mats"], prep["Ct"], prep["rho_t"], prep["damp"],
+                             prep["coeffs"], cfg["h"], cfg["dt"], prep["src_idx"],
+                             prep["src_w"], prep["rec_ptr"], prep["rec_idx"], prep["rec_w"],
+                             precision=precision, kernel=kernel, cpml=cp)
+
+
+def bench_precision(args):
+    """Error of the float32 traces against a float64 run of the same kernels:
+    the forward workload (400x400x200 by default) run for --steps steps in
+    both precisions with each boundary; rel-L2 over the whole trace and over
+    the window after the first 10% of the peak (the echoes)."""
+    import numpy as np
+    import torch
+
+    from pulse_echo_azimuthal_simulation_amd import specimen
+    nt = max(args.steps, 1)
+    cfg = specimen.config_c1(nt=nt) if args.config in (None, "c1") else specimen.config_c0(nt=nt)
+    res = {}
+    for boundary in ("cpml", "sponge"):
+        tr = {}
+        for prec in ("f32", "f64"):
+            sh = forward_shot(cfg, boundary, prec, args.kernel)
+            wav = sh.wavelet(cfg["wavelet"])
+            t = sh.new_traces(nt)
+            t0 = time.perf_counter()
+            sh.run(wav, 0, nt, t)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            tr[prec] = t.cpu().numpy().astype(np.float64)[:, 0]
+            log("%s %s: %d steps in %.1f s (%.2f Gcell-updates/s)"
+                % (boundary, prec, nt, wall, int(np.prod(sh.shape)) * nt / wall / 1e9))
+            del sh, t
+            torch.cuda.empty_cache()
+        a, b = tr["f32"], tr["f64"]
+        peak = np.abs(b).max()
+        late = np.argmax(np.abs(b) > 0.1 * peak)
+        res[boundary] = {
+            "rel_l2": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
+            "max_abs_err_over_peak": float(np.abs(a - b).max() / peak),
+            "rel_l2_after_onset": float(np.linalg.norm(a[late:] - b[late:])
+                                        / np.linalg.norm(b[late:])),
+            "peak_abs": float(peak),
+            "traces_finite": bool(np.isfinite(a).all() and np.isfinite(b).all()),
+        }
+        log("%s: f32 vs f64 rel-L2 %.3e" % (boundary, res[boundary]["rel_l2"]))
+    nz, ny, nx = cfg["lab"].shape
+    print(json.dumps({
+        "metric": "f32 trace error against an f64 run of the same kernels (rel-L2)",
+        "value": res["cpml"]["rel_l2"], "unit": "relative L2", "higher_is_better": False,
+        "steps": nt, "dtype": "f32 vs f64", "data": "synthetic",
+        "config": {"workload": "single-shot forward, %dx%dx%d grid, %d grains, pulse-echo "
+                               "trace of the top piston" % (nx, ny, nz, len(cfg["axes"])),
+                   "kernel": args.kernel},
+        "boundaries": res}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,10 +165,11 @@ def main():
     ap.add_argument("--config", default=None, choices=["c1", "c0", "c3"],
                     help="workload (default c1 forward; c3 for --mode adjoint)")
     ap.add_argument("--mode", default="forward",
-                    choices=["forward", "adjoint", "sweep", "inversion"],
+                    choices=["forward", "adjoint", "sweep", "inversion", "precision"],
                     help="forward: single-shot forward run; adjoint: misfit gradient w.r.t. "
                          "per-grain c-axis angles (forward + checkpointed reverse pass); sweep: "
-                         "batched azimuth sweep of the 400x400x200 disk plus 8 smaller disks")
+                         "batched azimuth sweep of the 400x400x200 disk plus 8 smaller disks; "
+                         "precision: f32 traces against an f64 run of the same kernels")
     ap.add_argument("--checkpoint-every", type=int, default=200)
     ap.add_argument("--hist", default="auto", choices=["auto", "strain", "velocity"],
                     help="adjoint mode: per-step history of the reverse pass")
@@ -142,6 +214,8 @@ def main():
         return bench_sweep(args)
     if args.mode == "inversion":
         return bench_inversion(args)
+    if args.mode == "precision":
+        return bench_precision(args)
     if args.config is None:
         args.config = "c1"
     t_setup = time.perf_counter()
@@ -159,21 +233,7 @@ def main():
                     % ("CPML" if args.boundary == "cpml" else "sponge", nt))
 
     def make_shot(boundary):
-        cpml_w = cfg["sponge_width"] if boundary == "cpml" else 0
-        prep = solver.prepare_shot(cfg["lab"], cfg["axes"], cfg["h"], cfg["dt"], cfg["src_pts"],
-                                   cfg["rec_groups"], coeffs=cfg["coeffs"],
-                                   sponge_width=0 if boundary == "cpml" else cfg["sponge_width"],
-                                   damp_mask=cfg["damp_mask"], mat_tables=cfg["mat_tables"])
-        cp = None
-        if cpml_w:
-            from pulse_echo_azimuthal_simulation_amd import cpml
-            cp = cpml.Cpml(prep["mats"].shape, cpml_w, cfg["h"], cfg["dt"],
-                           cpml.vmax_table(prep["Ct"], prep["rho_t"]),
-                           cpml.dominant_frequency(cfg["wavelet"], cfg["dt"]))
-        return solver.DeviceShot(prep["mats"], prep["Ct"], prep["rho_t"], prep["damp"],
-                                 prep["coeffs"], cfg["h"], cfg["dt"], prep["src_idx"],
-                                 prep["src_w"], prep["rec_ptr"], prep["rec_idx"], prep["rec_w"],
-                                 precision=args.precision, kernel=args.kernel, cpml=cp)
+        return forward_shot(cfg, boundary, args.precision, args.kernel)
 
     shot = make_shot(args.boundary)
     tuned = None
