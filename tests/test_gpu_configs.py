@@ -1,0 +1,126 @@
+"""GPU parity at the benchmark configurations' own sizes, through the C ABI,
+against the CPU oracle.
+
+* 400 x 400 x 200 (the single-shot production grid: 3000-grain Voronoi disk
+  in couplant, fluid damping mask, 6 mm top piston): from one random state
+  of every field and CPML memory variable, a few steps with the 20-cell
+  CPML and with the 20-cell sponge.  The random state puts every cell,
+  tile, z chunk and band path of both half-steps to work from the first
+  step, which a shot from rest reaches only after thousands of steps.
+* 96 x 96 x 48 disk (+ 23-cell couplant pads; 200 grains, 1 MHz 3-cycle
+  Hann burst from the 6 mm piston, 20-cell CPML): the first 300 of its
+  2000 steps from rest, by which the wave has crossed the couplant into
+  the top CPML band.
+
+Bars: fields and memory variables bit for bit (tiled f32 kernels with the
+benchmark's 25-plane z chunks); traces within 1e-6 rel-L2 (the receiver
+sum's tree order differs from the oracle's sequential sum).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_gpu():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch
+
+
+def _case(name):
+    print("\n[gpu case] %s" % name, flush=True)
+
+
+@pytest.fixture(scope="module")
+def cfg_c1():
+    from pulse_echo_azimuthal_simulation_amd import specimen
+    return specimen.config_c1(nt=8)
+
+
+def _shots(cfg, boundary):
+    """DeviceShot (tiled f32, 25-plane chunks) and OracleShot of one config."""
+    from oracle import oracle as O
+    from pulse_echo_azimuthal_simulation_amd import cpml, solver
+    sponge = 0 if boundary == "cpml" else cfg["sponge_width"]
+    prep = solver.prepare_shot(cfg["lab"], cfg["axes"], cfg["h"], cfg["dt"], cfg["src_pts"],
+                               cfg["rec_groups"], coeffs=cfg["coeffs"], sponge_width=sponge,
+                               damp_mask=cfg["damp_mask"], mat_tables=cfg["mat_tables"])
+    cp = None
+    if boundary == "cpml":
+        cp = cpml.Cpml(prep["mats"].shape, cfg["sponge_width"], cfg["h"], cfg["dt"],
+                       cpml.vmax_table(prep["Ct"], prep["rho_t"]),
+                       cpml.dominant_frequency(cfg["wavelet"], cfg["dt"]))
+    sh = solver.DeviceShot(prep["mats"], prep["Ct"], prep["rho_t"], prep["damp"], prep["coeffs"],
+                           cfg["h"], cfg["dt"], prep["src_idx"], prep["src_w"], prep["rec_ptr"],
+                           prep["rec_idx"], prep["rec_w"], precision="f32", kernel="tiled",
+                           zchunk=(25, 25), cpml=cp)
+    osh = O.OracleShot(prep["mats"], prep["Ct"], prep["rho_t"], prep["damp"], prep["coeffs"],
+                       cfg["h"], cfg["dt"], (prep["src_idx"], prep["src_w"]),
+                       (prep["rec_ptr"], prep["rec_idx"], prep["rec_w"]), precision="f32",
+                       cpml=None if cp is None else (cp.width, cp.prof))
+    return sh, osh
+
+
+def _random_state(torch, sh, osh, seed):
+    """The same random v, s (and both buffers of every memory variable) on
+    the device and in the oracle."""
+    rng = np.random.default_rng(seed)
+    for dev, host in zip(sh.v + sh.s, osh.v + osh.s):
+        a = rng.standard_normal(host.shape, dtype=np.float32)
+        host[...] = a
+        dev.view(-1).copy_(torch.from_numpy(a.ravel()))
+    for dev, host in zip(sh.psi, osh.psi):
+        a = (0.1 * rng.standard_normal(host.shape)).astype(np.float32)
+        host[...] = a
+        dev.view(-1).copy_(torch.from_numpy(a.ravel()))
+    torch.cuda.synchronize()
+
+
+def _compare(torch, sh, osh, tr, otr):
+    torch.cuda.synchronize()
+    v, s = sh.fields()
+    for k in range(3):
+        assert np.array_equal(v[k], osh.v[k]), "v%d" % k
+    for k in range(6):
+        assert np.array_equal(s[k], osh.s[k]), "s%d" % (k + 1)
+    for a, p in enumerate(sh.psi):
+        assert np.array_equal(p.cpu().numpy().reshape(osh.psi[a].shape), osh.psi[a]), \
+            "psi axis %d" % a
+    assert np.isfinite(otr).all() and np.abs(otr).max() > 0
+    assert H.rel_l2(tr.cpu().numpy(), otr) <= 1e-6
+
+
+@pytest.mark.parametrize("boundary,nt", [("cpml", 3), ("sponge", 2)])
+def test_c1_grid_from_random_state_matches_oracle(torch_gpu, cfg_c1, boundary, nt):
+    _case("400x400x200 %s, %d steps from a random state" % (boundary, nt))
+    torch = torch_gpu
+    sh, osh = _shots(cfg_c1, boundary)
+    assert sh.shape == (200, 400, 400)
+    _random_state(torch, sh, osh, seed=3)
+    tr = sh.new_traces(nt)
+    sh.run(sh.wavelet(cfg_c1["wavelet"]), 0, nt, tr)
+    otr = osh.run(cfg_c1["wavelet"], 0, nt)
+    _compare(torch, sh, osh, tr, otr)
+
+
+def test_c0_config_first_steps_match_oracle(torch_gpu):
+    _case("96x96x48 config, cpml, 300 steps from rest")
+    torch = torch_gpu
+    from pulse_echo_azimuthal_simulation_amd import specimen
+    nt = 300
+    cfg = specimen.config_c0(nt=nt)
+    sh, osh = _shots(cfg, "cpml")
+    assert sh.shape == (94, 142, 142)
+    tr = sh.new_traces(nt)
+    sh.run(sh.wavelet(cfg["wavelet"]), 0, nt, tr)
+    otr = osh.run(cfg["wavelet"], 0, nt)
+    # the wave has reached the top CPML band: its memory variables moved
+    assert np.abs(osh.psi[0]).max() > 0
+    _compare(torch, sh, osh, tr, otr)
