@@ -18,7 +18,11 @@ import ctypes as C
 from . import _abi
 
 _CACHE: dict = {}
-ZCHUNKS = (0, 25, 50, 100)
+# z planes per workgroup (0: the library's default, at least 16).  Short
+# chunks give small grids enough workgroups: on the 96x96x48 disk (+ 23-cell
+# pads) with a 20-cell CPML, 6-plane chunks run 6.3 G cell-updates/s against
+# 4.9 with the default.
+ZCHUNKS = (0, 6, 8, 12, 25, 50, 100)
 
 
 def _time_half(shot, half, reps):
