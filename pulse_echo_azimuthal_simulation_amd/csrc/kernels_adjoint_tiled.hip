@@ -1139,10 +1139,10 @@ hipError_t adjoint_step_tiled(const Shot<R> &S, const Shot<R> &Sv, const Adj<R> 
                               int damp_ls, hipStream_t st, const R *gh)
 {
     constexpr bool f32 = sizeof(R) == 4;
-    if (stage == 0) {
-        return f32 ? adj_velocity_launch<R, 64, 8>(S, A, damp_ls, st)
-                   : adj_velocity_launch<R, 32, 8>(S, A, damp_ls, st);
-    }
+    // D^T and G^T: 32 x 8 tiles in both precisions (float32 at 80x160x160,
+    // 4000 steps: gradient 1.427 s with 64 x 8, 1.402 / 1.407 s with 32 x 8
+    // for D^T / G^T alone; 64 x 4 and 32 x 16 no better)
+    if (stage == 0) return adj_velocity_launch<R, 32, 8>(S, A, damp_ls, st);
     if (stage == 1) {
         switch (A.n_proj) {
         case 1: return grad_tiled<R, 1>(Sv, A, st, gh);
@@ -1153,7 +1153,7 @@ hipError_t adjoint_step_tiled(const Shot<R> &S, const Shot<R> &Sv, const Adj<R> 
     }
     hipError_t e = f32 ? adj_gbar_launch<R, 32, 8>(S, A, st) : adj_gbar_launch<R, 32, 4>(S, A, st);
     if (e != hipSuccess) return e;
-    return f32 ? adj_strain_t_launch<R, 64, 8>(S, A, st) : adj_strain_t_launch<R, 32, 8>(S, A, st);
+    return adj_strain_t_launch<R, 32, 8>(S, A, st);
 }
 
 template <typename R>
