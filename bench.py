@@ -54,7 +54,9 @@ def cpu_model():
 
 def cpu_baseline(cfg, nz_sample=20, warm=2, timed=20):
     """Upstream array-path loop (NumPy, float64, one thread) on a z-slab of
-    the same specimen: cell-updates per second."""
+    the same specimen: cell-updates per second.  Builds its own wavelet of
+    warm + timed samples (the workload's Ricker or Hann burst at its dt), so
+    it does not depend on how many steps the GPU legs ran."""
     import numpy as np
 
     from oracle import numpy_forward as NF
@@ -69,7 +71,10 @@ def cpu_baseline(cfg, nz_sample=20, warm=2, timed=20):
     shot = NF.ArrayPathShot(sub + 1, Ct, rho_t, cfg["coeffs"], cfg["h"], cfg["dt"],
                             (idx, np.array([1.0])),
                             (np.array([0, 1]), idx, np.array([1.0])))
+    nw = warm + timed
     wav = cfg["wavelet"]
+    if len(wav) < nw:
+        wav = cfg["wavelet_fn"](cfg["dt"], nw)
     for n in range(warm):
         shot.step(float(wav[n]))
     t0 = time.perf_counter()
@@ -219,7 +224,9 @@ def main():
     if args.config is None:
         args.config = "c1"
     t_setup = time.perf_counter()
-    nt = max(args.steps, args.warmup, 1)
+    # the wavelet (and the configuration's record length) must cover every
+    # leg that runs from n = 0: the warm-up, the timed steps and the sponge run
+    nt = max(args.steps, args.warmup, args.sponge_steps if args.boundary == "cpml" else 0, 1)
     if args.config == "c1":
         cfg = specimen.config_c1(nt=nt)
         workload = ("single-shot forward, 400x400x200 FP32 grid (32M cells), 3000-grain Voronoi "
@@ -251,7 +258,7 @@ def main():
                  for h in (0, 1)}
         log("autotune: stress %s, velocity %s" % (tuned[0], tuned[1]))
     wav = shot.wavelet(cfg["wavelet"])
-    traces = shot.new_traces(nt)
+    traces = shot.new_traces(max(args.steps, args.warmup, 1))
     nz, ny, nx = shot.shape
     ncell = nz * ny * nx
     log("setup %.1f s: grid %s, %d grains, dt %.4g s, %d footprint cells"

@@ -148,6 +148,23 @@ class DeviceAdjoint:
 
     def run(self, n_hi, n_lo, vhist=None, residual=None, grad_wavelet=None):
         """Reverse steps n_hi-1 .. n_lo (asynchronous)."""
+        shot = self.shot
+        if not (0 <= int(n_lo) <= int(n_hi)):
+            raise ValueError(f"bad reverse step range [{n_lo}, {n_hi})")
+        if residual is not None and (residual.dim() != 2 or residual.shape[0] < int(n_hi)
+                                     or residual.shape[1] != shot.n_rec
+                                     or residual.dtype != shot.real
+                                     or not residual.is_contiguous()):
+            raise ValueError("residual must be a contiguous (nt >= n_hi, n_rec) tensor of the "
+                             "shot's real type")
+        if grad_wavelet is not None and (grad_wavelet.numel() < int(n_hi)
+                                         or grad_wavelet.dtype != self.torch.float64):
+            raise ValueError("grad_wavelet must be a float64 tensor of at least n_hi samples")
+        if vhist is not None and self.run_acc is not None and (
+                vhist.shape[0] < int(n_hi) - int(n_lo) or vhist.dtype != shot.real
+                or vhist.shape[1] != self.hist_fields or not vhist.is_contiguous()):
+            raise ValueError("history must hold %d slots of %d fields"
+                             % (int(n_hi) - int(n_lo), self.hist_fields))
         _abi.check(self.lib.pe_adjoint_run(
             C.byref(self.shot.desc), C.byref(self.desc), int(n_hi), int(n_lo),
             None if vhist is None else vhist.data_ptr(),
@@ -178,8 +195,7 @@ def forward_checkpointed(shot, wav, nt, traces, checkpoint_every=200, history=No
     during this pass (segments listed in ``history``): v (3 fields) or the
     strains of v (6 fields)."""
     torch = shot.torch
-    for f in shot.v + shot.s:
-        f.zero_()
+    shot.reset()
     ckpt, stored = {}, {}
     history = history or ()
     for a, b in _segments(nt, checkpoint_every):
@@ -195,6 +211,10 @@ def forward_checkpointed(shot, wav, nt, traces, checkpoint_every=200, history=No
 
 
 def _run_storing(shot, wav, a, b, traces, hist):
+    shot._check_run(wav, a, b, traces)
+    if hist.shape[0] < b - a or hist.shape[2:] != shot.shape or hist.dtype != shot.real \
+            or not hist.is_contiguous():
+        raise ValueError("history buffer too small for steps [%d, %d)" % (a, b))
     fn = shot.lib.pe_run_store_strains if hist.shape[1] == 6 else shot.lib.pe_run_store
     _abi.check(fn(C.byref(shot.desc), int(a), int(b), wav.data_ptr(),
                   None if traces is None else traces.data_ptr(), hist.data_ptr(),
@@ -290,7 +310,12 @@ def _autograd_function():
 
         @staticmethod
         def forward(ctx, Ct, shot, wav, nt, checkpoint_every, history_bytes):
-            shot.Ct.copy_(Ct.detach().reshape(shot.Ct.shape).to(shot.real))
+            if shot.cpml is not None:
+                raise ValueError("the adjoint runs the sponge boundary only: this shot has a "
+                                 "CPML")
+            table = Ct.detach().reshape(shot.Ct.shape).to(device=shot.device,
+                                                          dtype=shot.real).clone()
+            shot.Ct.copy_(table)
             traces = shot.new_traces(nt)
             real_bytes = 4 if shot.real == torch.float32 else 8
             hist = choose_history(shot, checkpoint_every)
@@ -300,18 +325,26 @@ def _autograd_function():
             ckpt, stored = forward_checkpointed(shot, wav, nt, traces, checkpoint_every, keep,
                                                 fields)
             ctx.hist = hist
+            # the shot's table may be overwritten by another forward before this
+            # backward runs (a line search): the reverse pass and the segment
+            # recomputes must see the table of this forward
+            ctx.table = table
             ctx.state = (shot, wav, nt, checkpoint_every, ckpt, stored)
             ctx.ct_shape, ctx.ct_dtype, ctx.ct_device = Ct.shape, Ct.dtype, Ct.device
             return traces
 
         @staticmethod
         def backward(ctx, grad_traces):
+            if ctx.state is None:
+                raise RuntimeError("backward through the same traces twice is not supported")
             shot, wav, nt, every, ckpt, stored = ctx.state
+            shot.Ct.copy_(ctx.table)
             adj = DeviceAdjoint(shot, hist=ctx.hist)
             res = grad_traces.to(shot.real).contiguous()
             reverse_pass(shot, adj, wav, nt, res, ckpt, stored, every)
             g = adj.gradient()
             ctx.state = None
+            ctx.table = None
             return (g.to(ctx.ct_dtype).to(ctx.ct_device).reshape(ctx.ct_shape),
                     None, None, None, None, None)
 

@@ -154,15 +154,42 @@ class DeviceShot:
         _abi.check(self.lib.pe_velocity(C.byref(self.desc), self._stream()), "pe_velocity")
 
     def source(self, wavelet_dev, n):
+        self._check_run(wavelet_dev, n, int(n) + 1, None)
         _abi.check(self.lib.pe_source(C.byref(self.desc), wavelet_dev.data_ptr(), int(n),
                                       self._stream()), "pe_source")
 
     def record(self, traces_dev, n):
+        if not (0 <= int(n) < traces_dev.shape[0]) or traces_dev.dtype != self.real \
+                or traces_dev.dim() != 2 or traces_dev.shape[1] != self.n_rec \
+                or not traces_dev.is_contiguous():
+            raise ValueError(f"record: row {n} outside the (nt, n_rec) traces buffer")
         _abi.check(self.lib.pe_record(C.byref(self.desc), traces_dev.data_ptr(), int(n),
                                       self._stream()), "pe_record")
 
+    def _check_run(self, wavelet_dev, n0, n1, traces_dev):
+        """The kernels read wavelet[n] and write traces row n for every step
+        n0 <= n < n1 through raw pointers: refuse buffers that are too short."""
+        torch = self.torch
+        if not (0 <= int(n0) <= int(n1)):
+            raise ValueError(f"bad step range [{n0}, {n1})")
+        if not torch.is_tensor(wavelet_dev) or wavelet_dev.device != self.device \
+                or wavelet_dev.dtype != torch.float64 or not wavelet_dev.is_contiguous():
+            raise ValueError("wavelet must be a contiguous float64 tensor on the shot's device "
+                             "(DeviceShot.wavelet)")
+        if wavelet_dev.numel() < int(n1):
+            raise ValueError(f"wavelet has {wavelet_dev.numel()} samples, steps run to n = {n1}")
+        if traces_dev is not None:
+            if traces_dev.device != self.device or traces_dev.dtype != self.real \
+                    or not traces_dev.is_contiguous() or traces_dev.dim() != 2 \
+                    or traces_dev.shape[1] != self.n_rec:
+                raise ValueError("traces must be a contiguous (nt, n_rec) tensor of the shot's "
+                                 "real type on its device (DeviceShot.new_traces)")
+            if traces_dev.shape[0] < int(n1):
+                raise ValueError(f"traces hold {traces_dev.shape[0]} rows, steps run to n = {n1}")
+
     def run(self, wavelet_dev, n0, n1, traces_dev=None):
         """Advance steps n0..n1-1 (asynchronous on the current stream)."""
+        self._check_run(wavelet_dev, n0, n1, traces_dev)
         _abi.check(self.lib.pe_run(C.byref(self.desc), int(n0), int(n1),
                                    wavelet_dev.data_ptr(),
                                    None if traces_dev is None else traces_dev.data_ptr(),
@@ -171,6 +198,7 @@ class DeviceShot:
     def run_timed(self, wavelet_dev, n0, n1, traces_dev=None):
         """Advance steps n0..n1-1 with per-kernel hipEvent timing; blocks.
         Returns milliseconds summed over the run: (stress, velocity, source/record)."""
+        self._check_run(wavelet_dev, n0, n1, traces_dev)
         ms = (C.c_double * 3)()
         _abi.check(self.lib.pe_run_timed(C.byref(self.desc), int(n0), int(n1),
                                          wavelet_dev.data_ptr(),

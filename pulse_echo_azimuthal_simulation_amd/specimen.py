@@ -121,8 +121,8 @@ def shot_config(lab, axes, h, nt, wavelet_fn, piston_d=6e-3, sponge=20, fluid_da
     dm = host.fluid_damp_mask(lab, fluid_damp)
     return dict(lab=lab, axes=axes, h=h, dt=dt, nt=int(nt), coeffs=co,
                 src_pts=[(p, w) for p in pts], rec_groups=[(pts, np.full(len(pts), w))],
-                wavelet=wavelet_fn(dt, int(nt)), sponge_width=sponge, damp_mask=dm,
-                mat_tables=(Ct, rho_t))
+                wavelet=wavelet_fn(dt, int(nt)), wavelet_fn=wavelet_fn, sponge_width=sponge,
+                damp_mask=dm, mat_tables=(Ct, rho_t))
 
 
 def config_c0(seed=7, nt=2000):

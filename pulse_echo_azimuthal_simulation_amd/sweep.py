@@ -318,6 +318,16 @@ class ShotBatch:
         self.shots = list(shots)
         self.nts = [int(n) for n in nts]
         self.wavelets = list(wavelets)
+        if not (len(self.shots) == len(self.nts) == len(self.wavelets)) or not self.shots:
+            raise ValueError("one wavelet and one step count per shot are needed")
+        for i, (s, w, n) in enumerate(zip(self.shots, self.wavelets, self.nts)):
+            # the batched source kernel reads wavelet[n] for every step n < nt
+            if not torch.is_tensor(w) or w.device != s.device or w.dtype != torch.float64 \
+                    or not w.is_contiguous():
+                raise ValueError(f"shot {i}: the wavelet must be a contiguous float64 tensor "
+                                 f"on the shot's device")
+            if w.numel() < n or n < 0:
+                raise ValueError(f"shot {i}: wavelet has {w.numel()} samples, nt = {n}")
         self.traces = [s.new_traces(n) for s, n in zip(self.shots, self.nts)] if record else None
         self.lib = _abi.load()
         n = len(self.shots)

@@ -257,6 +257,10 @@ def run(root, cfg, build=None, mem_budget_gb=None, graph_steps=50, log=print, sh
         stopped = False
         done_steps = 0
         stream = torch.cuda.Stream(device=shots[0].device)
+        # the shot preparation and the zero fills of the new buffers ran on the
+        # current stream: the graph's stream must start after them
+        cur = torch.cuda.current_stream(shots[0].device)
+        stream.wait_stream(cur)
         g = C.c_void_p()
         _abi.check(batch.lib.pe_batch_graph(C.byref(batch.desc), k, stream.cuda_stream,
                                             C.byref(g)), "pe_batch_graph")
@@ -271,6 +275,7 @@ def run(root, cfg, build=None, mem_budget_gb=None, graph_steps=50, log=print, sh
                     stream.synchronize()
         finally:
             stream.synchronize()
+            cur.wait_stream(stream)
             batch.lib.pe_graph_destroy(g)
         if stopped:
             log("[sweep] STOP: batch %s discarded; saved azimuths are final" % group)

@@ -88,3 +88,53 @@ def rel_l2(a, b):
     b = np.asarray(b, float)
     nb = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def random_state(torch, sh, osh, seed):
+    """The same random v, s (and both buffers of every CPML memory variable)
+    on a DeviceShot and its OracleShot: every cell, tile, z chunk and band
+    path is then at work from the first step."""
+    rng = np.random.default_rng(seed)
+    for dev, hst in zip(sh.v + sh.s, osh.v + osh.s):
+        a = rng.standard_normal(hst.shape, dtype=np.float32)
+        hst[...] = a
+        dev.view(-1).copy_(torch.from_numpy(a.ravel()))
+    for dev, hst in zip(sh.psi, osh.psi):
+        a = (0.1 * rng.standard_normal(hst.shape)).astype(np.float32)
+        hst[...] = a
+        dev.view(-1).copy_(torch.from_numpy(a.ravel()))
+    torch.cuda.synchronize()
+
+
+def compare_fields(torch, sh, osh, tr, otr, trace_tol=1e-6):
+    """Fields and memory variables bit for bit, traces within trace_tol rel-L2.
+
+    From a random state a pressure trace sums hundreds of random-signed
+    terms, so its relative rounding error is amplified by the cancellation;
+    the bar is then trace_tol times the conditioning of the last row's sum,
+    sum |w p| / |sum w p| (computed in float64 from the final stress, which
+    the velocity half-step leaves as it was when the last row was recorded).
+    """
+    torch.cuda.synchronize()
+    v, s = sh.fields()
+    for k in range(3):
+        assert np.array_equal(v[k], osh.v[k]), "v%d" % k
+    for k in range(6):
+        assert np.array_equal(s[k], osh.s[k]), "s%d" % (k + 1)
+    for a, p in enumerate(sh.psi):
+        assert np.array_equal(p.cpu().numpy().reshape(osh.psi[a].shape), osh.psi[a]), \
+            "psi axis %d" % a
+    assert np.isfinite(otr).all() and np.abs(otr).max() > 0
+    cond = 1.0
+    if sh.desc.record_mode == 0:
+        ptr = sh.rec_ptr.cpu().numpy()
+        idx = sh.rec_idx.cpu().numpy()
+        w = sh.rec_w.cpu().numpy().astype(np.float64)
+        p = -(osh.s[0].ravel()[idx].astype(np.float64) + osh.s[1].ravel()[idx]
+              + osh.s[2].ravel()[idx]) / 3.0
+        for g in range(len(ptr) - 1):
+            terms = w[ptr[g]:ptr[g + 1]] * p[ptr[g]:ptr[g + 1]]
+            tot = abs(terms.sum())
+            if tot > 0:
+                cond = max(cond, float(np.abs(terms).sum() / tot))
+    assert rel_l2(tr.cpu().numpy(), otr) <= trace_tol * cond

@@ -190,6 +190,44 @@ def test_autograd_function_matches_misfit_gradient(torch_gpu):
     assert H.rel_l2(pp.grad.numpy(), gp[1:, 1]) <= 1e-9
 
 
+def test_autograd_two_forwards_then_backward_of_first(torch_gpu):
+    """Two forwards through one shot with different tables (a line search),
+    then backward of the first: the gradient is the first table's, because
+    the backward restores the table its forward saw."""
+    _case("adjoint autograd two forwards")
+    torch = torch_gpu
+    from pulse_echo_azimuthal_simulation_amd import adjoint
+    inp = _medium(seed=12)
+    _, _, _, d = _oracle_grad(inp, precision="f64")
+    _, g_ref, _ = _gpu_grad(torch, inp, d, "f64", every=9)
+    sh = _device_shot(inp, "f64", "tiled")
+    dd = torch.as_tensor(d, device=sh.device)
+    t1 = torch.tensor(np.asarray(inp["Ct"], float).reshape(-1, 36), requires_grad=True)
+    t2 = torch.tensor(np.asarray(inp["Ct_obs"], float).reshape(-1, 36), requires_grad=True)
+    tr1 = adjoint.traces_from_table(t1, sh, inp["wavelet"], inp["nt"], checkpoint_every=9)
+    tr2 = adjoint.traces_from_table(t2, sh, inp["wavelet"], inp["nt"], checkpoint_every=9)
+    assert H.rel_l2(tr2.detach().cpu().numpy(), d) <= 1e-9
+    J1 = 0.5 * ((tr1.double() - dd) ** 2).sum()
+    J1.backward()
+    assert H.rel_l2(t1.grad.numpy(), g_ref) <= 1e-9
+
+
+def test_autograd_refuses_cpml_shot(torch_gpu):
+    _case("adjoint autograd cpml refusal")
+    torch = torch_gpu
+    from pulse_echo_azimuthal_simulation_amd import adjoint, cpml
+    from pulse_echo_azimuthal_simulation_amd.solver import DeviceShot
+    inp = H.random_medium((14, 16, 18), n_grain=4, seed=3, sponge=0)
+    cp = cpml.Cpml(inp["shape"], 3, inp["h"], inp["dt"],
+                   cpml.vmax_table(inp["Ct"], inp["rho_t"]), 2e6)
+    sh = DeviceShot(inp["mats"], inp["Ct"], inp["rho_t"], inp["damp"], inp["coeffs"], inp["h"],
+                    inp["dt"], inp["src"][0], inp["src"][1], *inp["rec"], precision="f64",
+                    kernel="tiled", cpml=cp)
+    t = torch.tensor(np.asarray(inp["Ct"], float).reshape(-1, 36), requires_grad=True)
+    with pytest.raises(ValueError, match="CPML"):
+        adjoint.traces_from_table(t, sh, inp["wavelet"], inp["nt"])
+
+
 @pytest.mark.parametrize("precision,shape", [("f32", (40, 44, 150)), ("f64", (36, 40, 90)),
                                              ("f32", (23, 29, 71))])
 def test_tiled_adjoint_matches_direct(torch_gpu, precision, shape):

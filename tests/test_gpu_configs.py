@@ -68,46 +68,17 @@ def _shots(cfg, boundary):
     return sh, osh
 
 
-def _random_state(torch, sh, osh, seed):
-    """The same random v, s (and both buffers of every memory variable) on
-    the device and in the oracle."""
-    rng = np.random.default_rng(seed)
-    for dev, host in zip(sh.v + sh.s, osh.v + osh.s):
-        a = rng.standard_normal(host.shape, dtype=np.float32)
-        host[...] = a
-        dev.view(-1).copy_(torch.from_numpy(a.ravel()))
-    for dev, host in zip(sh.psi, osh.psi):
-        a = (0.1 * rng.standard_normal(host.shape)).astype(np.float32)
-        host[...] = a
-        dev.view(-1).copy_(torch.from_numpy(a.ravel()))
-    torch.cuda.synchronize()
-
-
-def _compare(torch, sh, osh, tr, otr):
-    torch.cuda.synchronize()
-    v, s = sh.fields()
-    for k in range(3):
-        assert np.array_equal(v[k], osh.v[k]), "v%d" % k
-    for k in range(6):
-        assert np.array_equal(s[k], osh.s[k]), "s%d" % (k + 1)
-    for a, p in enumerate(sh.psi):
-        assert np.array_equal(p.cpu().numpy().reshape(osh.psi[a].shape), osh.psi[a]), \
-            "psi axis %d" % a
-    assert np.isfinite(otr).all() and np.abs(otr).max() > 0
-    assert H.rel_l2(tr.cpu().numpy(), otr) <= 1e-6
-
-
 @pytest.mark.parametrize("boundary,nt", [("cpml", 3), ("sponge", 2)])
 def test_c1_grid_from_random_state_matches_oracle(torch_gpu, cfg_c1, boundary, nt):
     _case("400x400x200 %s, %d steps from a random state" % (boundary, nt))
     torch = torch_gpu
     sh, osh = _shots(cfg_c1, boundary)
     assert sh.shape == (200, 400, 400)
-    _random_state(torch, sh, osh, seed=3)
+    H.random_state(torch, sh, osh, seed=3)
     tr = sh.new_traces(nt)
     sh.run(sh.wavelet(cfg_c1["wavelet"]), 0, nt, tr)
     otr = osh.run(cfg_c1["wavelet"], 0, nt)
-    _compare(torch, sh, osh, tr, otr)
+    H.compare_fields(torch, sh, osh, tr, otr)
 
 
 def test_c0_config_first_steps_match_oracle(torch_gpu):
@@ -123,4 +94,4 @@ def test_c0_config_first_steps_match_oracle(torch_gpu):
     otr = osh.run(cfg["wavelet"], 0, nt)
     # the wave has reached the top CPML band: its memory variables moved
     assert np.abs(osh.psi[0]).max() > 0
-    _compare(torch, sh, osh, tr, otr)
+    H.compare_fields(torch, sh, osh, tr, otr)

@@ -1,6 +1,6 @@
 """Central finite differences of the misfit in per-grain c-axis angles against
 the adjoint gradient, on the 80x160x160 / 500-grain / 4000-step adjoint
-workload (GPU; run directly, not collected by pytest).
+workload (GPU; collected with -m gpu, about 100 s; also runs as a script).
 
 J(theta, phi) = 0.5 * ||y(theta, phi) - d||^2 with d the traces of the axes
 tilted by N(0, 5 deg).  The adjoint gradient (checkpoints every 200 steps,
@@ -9,7 +9,12 @@ recompute in between, projections on dC/dtheta and dC/dphi) is compared with
 random grains, in float64; the float32 adjoint gradient is compared with
 the float64 one.  Prints one JSON line.
 
-    python tests/fd_check_c3.py [--steps 4000] [--grains 20] [--out file.json]
+    python tests/test_gpu_fd_c3.py [--steps 4000] [--grains 20] [--out file.json]
+
+Reference gradient: vendor/ringfwi/elastic3d.py:348-463 (exact adjoint of
+the array-path solver) chained by vendor/ringfwi/axisfield.py:37-161; the
+check itself follows the reference inversion's central-difference pattern
+on the per-grain angles.
 """
 from __future__ import annotations
 
@@ -19,26 +24,21 @@ import os
 import sys
 import time
 
+import pytest
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--steps", type=int, default=4000)
-    ap.add_argument("--grains", type=int, default=20)
-    ap.add_argument("--h", type=float, default=1e-3)
-    ap.add_argument("--checkpoint-every", type=int, default=200)
-    ap.add_argument("--out", default=None)
-    args = ap.parse_args()
-
+def fd_check(steps=4000, grains=20, h=1e-3, checkpoint_every=200):
+    """Adjoint angle gradient against central differences; returns a dict."""
     import numpy as np
     import torch
 
     from pulse_echo_azimuthal_simulation_amd import adjoint, fabric, solver, specimen
 
     t0 = time.perf_counter()
-    nt = args.steps
+    nt = steps
     cfg = specimen.config_c3(nt=nt)
     theta, phi = fabric.angles_from_axes(cfg["axes"])
     tt, pt = fabric.angles_from_axes(cfg["axes_true"])
@@ -74,7 +74,7 @@ def main():
             return 0.5 * float((r.double() ** 2).sum()), r
 
         J, g, _ = adjoint.misfit_gradient(sh, cfg["wavelet"], nt, resid,
-                                          checkpoint_every=args.checkpoint_every, n_proj=2,
+                                          checkpoint_every=checkpoint_every, n_proj=2,
                                           proj=P)
         grads[prec] = (J, g.cpu().numpy()[1:])
         del sh
@@ -90,8 +90,7 @@ def main():
         return 0.5 * float((r ** 2).sum())
 
     rng = np.random.default_rng(2024)
-    picks = rng.choice(len(theta), size=args.grains, replace=False)
-    h = args.h
+    picks = rng.choice(len(theta), size=grains, replace=False)
     rows = []
     for gi in picks:
         fd = []
@@ -108,19 +107,46 @@ def main():
     rel = np.abs(fd - ad) / np.maximum(np.abs(ad), 1e-3 * scale)
     out = {
         "workload": "80x160x160 grid, 500 grains, %d steps, checkpoints every %d" % (
-            nt, args.checkpoint_every),
+            nt, checkpoint_every),
         "misfit": J0, "h_rad": h, "grains": [r[0] for r in rows],
         "fd": fd.tolist(), "adjoint": ad.tolist(),
         "max_rel_err": float(rel.max()), "median_rel_err": float(np.median(rel)),
         "f32_vs_f64_rel_l2": float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)),
         "max_abs_grad": float(scale), "seconds": round(time.perf_counter() - t0, 1),
     }
+    return out
+
+
+@pytest.mark.gpu
+def test_c3_adjoint_gradient_matches_central_differences():
+    """Bar: every one of the 40 angle derivatives within 1e-3 relative of
+    its central difference (relative to max(|adjoint|, 1e-3 max |grad|));
+    the float32 adjoint gradient within 1e-3 rel-L2 of the float64 one."""
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    print("\n[gpu case] C3 FD check 80x160x160, 4000 steps, 20 grains", flush=True)
+    out = fd_check()
+    print(json.dumps(out), flush=True)
+    assert out["max_rel_err"] <= 1e-3
+    assert out["f32_vs_f64_rel_l2"] <= 1e-3
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--grains", type=int, default=20)
+    ap.add_argument("--h", type=float, default=1e-3)
+    ap.add_argument("--checkpoint-every", type=int, default=200)
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args()
+    out = fd_check(args.steps, args.grains, args.h, args.checkpoint_every)
     line = json.dumps(out)
     print(line, flush=True)
     if args.out:
         with open(args.out, "w") as fh:
             fh.write(line + "\n")
-    return 0 if rel.max() <= 1e-3 else 1
+    return 0 if out["max_rel_err"] <= 1e-3 else 1
 
 
 if __name__ == "__main__":
