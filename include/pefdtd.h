@@ -256,6 +256,24 @@ int64_t pe_batch_table_bytes(const pe_shot *shots, int32_t n_shot);
  * counters.  Must precede pe_batch_step / pe_batch_graph. */
 int pe_batch_setup(pe_batch *batch, void *stream);
 
+/* pe_batch_setup for callers that cannot allocate device memory (code under
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  No
+ * upstream counterpart.  Ignores batch->table and batch->step on entry and
+ * puts both in device memory the library owns: a static array of
+ * PE_BATCH_SCRATCH_BYTES per device (held until the process ends), the step
+ * counters in the first 256-byte-rounded n_shot*4 bytes, then the table; on
+ * return batch->table / batch->step hold those device addresses, and
+ * pe_batch_step / pe_batch_graph use them as usual.  Allocates nothing (the
+ * table is uploaded with a blocking copy and the counters zeroed, as by
+ * pe_batch_setup).  Returns PE_ERANGE, before touching the device, when the
+ * table and counters exceed PE_BATCH_SCRATCH_BYTES (pe_batch_table_bytes
+ * gives the table size).  The array is shared by every batch set up this
+ * way on one device and keeps the last setup's contents: a later setup
+ * replaces the table an earlier batch (or a graph captured from it) reads,
+ * so such batches must not overlap on one device. */
+#define PE_BATCH_SCRATCH_BYTES (8 * 1024 * 1024)
+int pe_batch_setup_noalloc(pe_batch *batch, void *stream);
+
 /* Enqueue `steps` batched steps (asynchronous, graph-capturable). */
 int pe_batch_step(const pe_batch *batch, int32_t steps, void *stream);
 

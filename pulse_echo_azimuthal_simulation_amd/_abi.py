@@ -23,7 +23,11 @@ EXPORTS = ("pe_abi_version", "pe_last_error", "pe_check_shot", "pe_stress", "pe_
            "pe_check_adjoint", "pe_adjoint_reset", "pe_adjoint_run", "pe_adjoint_gradient",
            "pe_batch_table_bytes", "pe_batch_setup", "pe_batch_step", "pe_batch_graph",
            "pe_graph_launch", "pe_graph_destroy", "pe_prep_rotate", "pe_prep_edt_damp",
-           "pe_prep_vmax", "pe_prep_label_power", "pe_run_store_strains")
+           "pe_prep_vmax", "pe_prep_label_power", "pe_run_store_strains",
+           "pe_batch_setup_noalloc")
+
+#: bytes of library-owned device memory behind pe_batch_setup_noalloc
+BATCH_SCRATCH_BYTES = 8 * 1024 * 1024
 
 
 class PeShot(C.Structure):
@@ -118,6 +122,7 @@ def load():
     lib.pe_batch_table_bytes.argtypes = [C.POINTER(PeShot), C.c_int32]
     lib.pe_batch_table_bytes.restype = C.c_int64
     lib.pe_batch_setup.argtypes = [B, C.c_void_p]
+    lib.pe_batch_setup_noalloc.argtypes = [B, C.c_void_p]
     lib.pe_batch_step.argtypes = [B, C.c_int32, C.c_void_p]
     lib.pe_batch_graph.argtypes = [B, C.c_int32, C.c_void_p, C.POINTER(C.c_void_p)]
     lib.pe_graph_launch.argtypes = [C.c_void_p, C.c_void_p]
@@ -138,7 +143,7 @@ def load():
                  "pe_adjoint_run", "pe_adjoint_gradient", "pe_batch_setup", "pe_batch_step",
                  "pe_batch_graph", "pe_graph_launch", "pe_graph_destroy", "pe_prep_rotate",
                  "pe_prep_edt_damp", "pe_prep_vmax", "pe_prep_label_power",
-                 "pe_run_store_strains"):
+                 "pe_run_store_strains", "pe_batch_setup_noalloc"):
         getattr(lib, name).restype = C.c_int
     ver = lib.pe_abi_version()
     if ver != ABI_VERSION:

@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -276,6 +277,35 @@ int batch_check(const pe_batch *b, BatchLayout *L)
         if (b->nt[i] < 0) return fail(PE_EINVAL, "shot %d: negative step count", i);
         if (!b->wavelet[i] && b->shots[i].n_src > 0) return fail(PE_EINVAL, "shot %d: wavelet is NULL", i);
     }
+    return PE_OK;
+}
+
+// Library-owned device memory of pe_batch_setup_noalloc: the batch table
+// and the step counters of the last batch set up that way on each device.
+// Host code never uses the array's name as an address: the per-device
+// address comes from hipGetSymbolAddress, taken once per device with that
+// device current, and only that address is passed to kernels and copies.
+__device__ __align__(256) unsigned char scratch[PE_BATCH_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(void **out)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return fail(PE_ERANGE, "device %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        if (!ptr) return fail(PE_EHIP, "hipGetSymbolAddress(scratch) returned NULL");
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
     return PE_OK;
 }
 
@@ -592,6 +622,32 @@ int pe_batch_setup(pe_batch *b, void *stream)
     (void)stream;
     BatchLayout L;
     int rc = batch_check(b, &L);
+    if (rc) return rc;
+    return b->shots[0].precision == PE_F32 ? batch_setup_impl<float>(b, L)
+                                           : batch_setup_impl<double>(b, L);
+}
+
+int pe_batch_setup_noalloc(pe_batch *b, void *stream)
+{
+    (void)stream;
+    if (!b) return fail(PE_EINVAL, "batch is NULL");
+    BatchLayout L;
+    int rc = batch_layout(b->shots, b->n_shot, &L);
+    if (rc) return rc;
+    // step counters first, then the table at the next 256-byte boundary
+    const size_t step_bytes = ((size_t)b->n_shot * sizeof(int32_t) + 255) / 256 * 256;
+    if (L.total > (size_t)PE_BATCH_SCRATCH_BYTES - step_bytes)
+        return fail(PE_ERANGE,
+                    "batch table of %zu bytes (+%zu bytes of step counters) exceeds the "
+                    "%d-byte library scratch of pe_batch_setup_noalloc: use pe_batch_setup "
+                    "with a caller-allocated table",
+                    L.total, step_bytes, (int)PE_BATCH_SCRATCH_BYTES);
+    void *base = nullptr;
+    rc = scratch_address(&base);
+    if (rc) return rc;
+    b->step = reinterpret_cast<int32_t *>(base);
+    b->table = reinterpret_cast<unsigned char *>(base) + step_bytes;
+    rc = batch_check(b, &L);
     if (rc) return rc;
     return b->shots[0].precision == PE_F32 ? batch_setup_impl<float>(b, L)
                                            : batch_setup_impl<double>(b, L);

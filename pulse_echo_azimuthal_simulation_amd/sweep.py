@@ -310,9 +310,14 @@ class ShotBatch:
     float64 tensors; ``nts``: per-shot step counts.  Traces are allocated
     per shot.  ``run(graph_steps)`` captures ``graph_steps`` batched steps in
     a hipGraph and replays it until every shot has run its steps.
+
+    ``noalloc=True`` keeps the batch table and step counters in the
+    library's own device memory (pe_batch_setup_noalloc) instead of tensors
+    allocated here: batches set up that way on one device share that memory,
+    so only the most recently set-up one may run.
     """
 
-    def __init__(self, shots, wavelets, nts, record=True):
+    def __init__(self, shots, wavelets, nts, record=True, noalloc=False):
         torch = shots[0].torch
         self.torch = torch
         self.shots = list(shots)
@@ -339,16 +344,20 @@ class ShotBatch:
         if nbytes <= 0:
             raise ValueError("invalid batch: %s" % self.lib.pe_last_error().decode())
         dev = shots[0].device
-        self.table = torch.empty(int(nbytes), dtype=torch.uint8, device=dev)
-        self.step = torch.zeros(n, dtype=torch.int32, device=dev)
+        self.noalloc = bool(noalloc)
+        if self.noalloc:
+            self.table = self.step = None
+        else:
+            self.table = torch.empty(int(nbytes), dtype=torch.uint8, device=dev)
+            self.step = torch.zeros(n, dtype=torch.int32, device=dev)
         b = _abi.PeBatch()
         b.n_shot = n
         b.shots = C.cast(self._descs, C.c_void_p)
         b.wavelet = C.cast(self._wav, C.c_void_p)
         b.traces = None if self._tr is None else C.cast(self._tr, C.c_void_p)
         b.nt = C.cast(self._nt, C.c_void_p)
-        b.table = self.table.data_ptr()
-        b.step = self.step.data_ptr()
+        b.table = None if self.noalloc else self.table.data_ptr()
+        b.step = None if self.noalloc else self.step.data_ptr()
         self.desc = b
         self._graph = None
         self._graph_steps = 0
@@ -358,7 +367,12 @@ class ShotBatch:
 
     def setup(self):
         """Upload the table and zero the step counters (blocking)."""
-        _abi.check(self.lib.pe_batch_setup(C.byref(self.desc), self._stream()), "pe_batch_setup")
+        if self.noalloc:
+            _abi.check(self.lib.pe_batch_setup_noalloc(C.byref(self.desc), self._stream()),
+                       "pe_batch_setup_noalloc")
+        else:
+            _abi.check(self.lib.pe_batch_setup(C.byref(self.desc), self._stream()),
+                       "pe_batch_setup")
 
     def step(self, k=1):
         _abi.check(self.lib.pe_batch_step(C.byref(self.desc), int(k), self._stream()),

@@ -166,3 +166,26 @@ def test_batch_with_cpml_needs_tile_variant_zero():
     arr[0].tile[1] = 2
     assert lib.pe_batch_table_bytes(arr, 2) == 0
     assert "tile variant 0" in lib.pe_last_error().decode()
+
+
+def test_noalloc_batch_refuses_oversized_batch_before_device_work():
+    """pe_batch_setup_noalloc checks the batch against its fixed library
+    memory on the host, before it touches a device (runs without a GPU)."""
+    lib = _abi.load()
+    fake = 0x1000
+    n = 20000
+    arr = (_abi.PeShot * n)(*([_desc(n_src=0)] * n))
+    nbytes = lib.pe_batch_table_bytes(arr, n)
+    assert nbytes > _abi.BATCH_SCRATCH_BYTES
+    nts = (C.c_int32 * n)(*([5] * n))
+    wavs = (C.c_void_p * n)(*([fake] * n))
+    b = _abi.PeBatch()
+    b.n_shot = n
+    b.shots = C.cast(arr, C.c_void_p)
+    b.wavelet = C.cast(wavs, C.c_void_p)
+    b.nt = C.cast(nts, C.c_void_p)
+    assert lib.pe_batch_setup_noalloc(C.byref(b), None) == _abi.PE_ERANGE
+    msg = lib.pe_last_error().decode()
+    assert "exceeds" in msg and str(_abi.BATCH_SCRATCH_BYTES) in msg
+    assert not b.table and not b.step
+    assert lib.pe_batch_setup_noalloc(None, None) == _abi.PE_EINVAL
