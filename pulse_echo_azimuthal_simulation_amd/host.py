@@ -178,6 +178,20 @@ def fluid_damp_mask(lab, fluid_damp: float = 0.02) -> np.ndarray:
     return np.exp(-fluid_damp * dist).astype(np.float32)
 
 
+def absorption_mask(lab, alpha_np_m: float, c_ref: float, dt: float) -> np.ndarray:
+    """Intrinsic absorption of the ice as a per-step amplitude factor,
+    float32: exp(-alpha * c_ref * dt) in every specimen cell (lab >= 0), 1 in
+    the couplant (sim/core/fdtd.py:1560-1577).  The wave advances c_ref * dt
+    per step, so the factor accumulated along a path of length L is
+    exp(-alpha * L) for a narrowband pulse with alpha (Np/m) taken at its
+    centre frequency.  Callers multiply it into the damping mask."""
+    lab = np.asarray(lab)
+    f = np.float32(np.exp(-float(alpha_np_m) * float(c_ref) * float(dt)))
+    m = np.ones(lab.shape, np.float32)
+    m[lab >= 0] = f
+    return m
+
+
 # --------------------------------------------------------------------------
 # wavelets
 # --------------------------------------------------------------------------

@@ -186,3 +186,51 @@ def test_lattice_receivers_match_archive():
     case = R.lattice_case(6)
     assert np.array_equal(np.array([(q[0] - case["c"], q[1] - case["c"], q[2] - case["c"])
                                     for q in case["idx"]]), case["ref"]["off"])
+
+
+# ------------------------------------------------------------- absorption
+def test_absorption_mask_closed_form():
+    lab = np.array([[[-1, 0, 3], [2, -1, -1]]], np.int32)
+    alpha, c, dt = 12.5, 3850.0, 1.2e-8
+    m = host.absorption_mask(lab, alpha, c, dt)
+    assert m.dtype == np.float32 and m.shape == lab.shape
+    f = np.float32(np.exp(-alpha * c * dt))
+    assert np.array_equal(m[lab >= 0], np.full(int((lab >= 0).sum()), f))
+    assert np.array_equal(m[lab < 0], np.ones(int((lab < 0).sum()), np.float32))
+    assert np.array_equal(host.absorption_mask(lab, 0.0, c, dt), np.ones(lab.shape, np.float32))
+
+
+def test_absorption_mask_accumulates_exp_minus_alpha_path(oracle_lib):
+    """The physical statement of sim/core/fdtd.py:1563-1568: a pulse that
+    travels a path R through ice whose damping includes the mask arrives
+    with its amplitude scaled by exp(-alpha R).  Homogeneous isotropic ice
+    (vp = 3850 m/s, the c_ref of the mask), point source, receiver 25 cells
+    away on the x axis; peak ratio with / without absorption against
+    exp(-alpha R) within 2% (the upstream's own validation bar)."""
+    O = oracle_lib
+    N, h, f0 = 80, 2.5e-4, 2.0e6
+    vp = 3850.0
+    Ct = np.tile(np.asarray(R.iso_C(vp), np.float32).reshape(1, 36), (2, 1))
+    rho_t = np.full(2, np.float32(R.RHO_ICE))
+    lab = np.zeros((N, N, N), np.int32)
+    co = host.optimised_coeffs(8)
+    dt = host.safe_dt_labels(lab + 1, Ct, rho_t, h, co, safety=0.5)
+    c = N // 2
+    dist = 25
+    nt = int((dist * h / vp + 3.0 / f0) / dt)
+    wav = host.ricker(f0, dt, nt)
+    shape = (N, N, N)
+    src = (np.array([np.ravel_multi_index((c, c, c), shape)]), np.array([1.0]))
+    rec = (np.array([0, 1], np.int32), np.array([np.ravel_multi_index((c, c, c + dist), shape)]),
+           np.array([1.0]))
+    sponge = host.sponge_profile(shape, 10, 0.09)
+    alpha = 20.0
+    peaks = []
+    for a in (0.0, alpha):
+        damp = sponge * host.absorption_mask(lab, a, vp, dt)
+        sh = O.OracleShot(lab + 1, Ct, rho_t, damp, co, h, dt, src, rec, precision="f32")
+        tr = sh.run(wav, 0, nt)[:, 0].astype(float)
+        peaks.append(np.abs(tr).max())
+    ratio = peaks[1] / peaks[0]
+    expect = np.exp(-alpha * dist * h)
+    assert abs(ratio / expect - 1.0) < 0.02, (ratio, expect)
