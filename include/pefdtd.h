@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PE_ABI_VERSION 8
+#define PE_ABI_VERSION 9
 
 enum { PE_OK = 0, PE_EINVAL = -1, PE_EHIP = -2, PE_ERANGE = -3 };
 enum { PE_F32 = 0, PE_F64 = 1 };
@@ -75,23 +75,28 @@ typedef struct pe_shot {
     /* CPML absorbing boundary (no upstream counterpart: the upstream absorber
      * is the sponge folded into damp).  cpml_width L = 0: none.  Otherwise
      * axis a (0 z, 1 y, 2 x, length n >= 2L+2) has a band of L+1 cells at
-     * each end (i <= L or i >= n-1-L, band index i or i-(n-2L-2)); cells in
-     * any band replace each derivative dd along a band axis by
-     * fma(dd, 1/kappa, psi'), psi' = fma(b, psi, a*dd).
+     * each end (i <= L or i >= n-1-L, band index i or i-(n-2L-2)).  A cell
+     * in any band is the plain cell plus corrections: for each band axis (z,
+     * then y, then x) each of its three derivatives dd, taken on its own, is
+     * passed through psi' = fma(b, psi, a*dd) and contributes
+     * fma(dd, 1/kappa - 1, psi') to its strain (or acceleration), so that
+     * the stretched derivative is dd/kappa + psi'.
      *   cpml_prof[a]: real[2][3][n]: [stagger: 0 integer points, 1 half
      *                 points][b, a, 1/kappa][index]
-     *   cpml_psi[a]:  real[2][6][band volume], zero-initialised; band volume
-     *                 (2L+2)*ny*nx (z), nz*(2L+2)*nx (y), nz*ny*(2L+2) (x);
-     *                 components 0..2 belong to the stress half-step, 3..5
-     *                 to the velocity half-step
-     * Step n reads buffer n & 1 and writes the other; pe_run takes n from
-     * its loop, pe_stress / pe_velocity from cpml_step, a batch from each
-     * shot's step counter.  Forward runs and batches only (the adjoint and
-     * the strain history refuse it). */
+     *   cpml_psi[a]:  real[6][band volume], zero-initialised, advanced in
+     *                 place; band volume (2L+2)*ny*nx (z), nz*(2L+2)*nx (y),
+     *                 nz*ny*(2L+2) (x); components 0..2 belong to the stress
+     *                 half-step, 3..5 to the velocity half-step
+     *   cpml_corr:    real[6][ncell] (PE_KERNEL_TILED): the corrections a
+     *                 band pre-pass writes for the tiled kernels to add (no
+     *                 initial value needed; only band cells are used)
+     * Forward runs and batches only (the adjoint and the strain history
+     * refuse it). */
     int32_t cpml_width;
-    int32_t cpml_step;
+    int32_t cpml_reserved;       /* set to 0 */
     const void *cpml_prof[3];
     void *cpml_psi[3];
+    void *cpml_corr;
 } pe_shot;
 
 /* ABI version (PE_ABI_VERSION) and the last error message of this thread. */
@@ -232,11 +237,9 @@ int pe_adjoint_gradient(const pe_shot *shot, const pe_adjoint *adj, double *grad
  * can be replayed until every shot is done (finished shots exit at once).
  * All shots share one precision and the tile variants of shots[0]; every
  * shot must be PE_KERNEL_TILED.  Shots may have a CPML (cpml_width > 0, with
- * zeroed memory variables; then shots[0] must use tile variant 0 for both
- * half-steps): such a batch runs each half-step as three launches (the
- * plain, z-band and full band kernels) over all workgroups, and takes the
- * memory-variable buffer from each shot's step counter.  Results are
- * bit-identical to pe_run of each shot alone.
+ * zeroed memory variables): each half-step of such a batch is preceded by
+ * one band pre-pass launch over the band cells of every CPML shot.  Results
+ * are bit-identical to pe_run of each shot alone.
  * ------------------------------------------------------------------------ */
 typedef struct pe_batch {
     int32_t n_shot;
@@ -246,7 +249,8 @@ typedef struct pe_batch {
     const int32_t *nt;               /* host array: step count of each shot */
     void *table;                     /* device scratch of pe_batch_table_bytes() bytes */
     int32_t *step;                   /* device, n_shot counters: next step of each shot */
-    int64_t nb[2];                   /* set by pe_batch_setup: workgroups per half-step */
+    int64_t nb[3];                   /* set by pe_batch_setup: workgroups per half-step, and
+                                        of the CPML band pre-passes */
 } pe_batch;
 
 /* Device scratch needed by a batch of these shots (0 on invalid input). */

@@ -61,10 +61,10 @@ typedef struct {
     /* CPML (no upstream counterpart; cpml_width 0 = none).  Axis a (0 z,
      * 1 y, 2 x) of length n: band cells i <= L or i >= n-1-L, band index
      * i or i - (n - 2L - 2); prof[a] = real[2 staggers][b, a, 1/kappa][n];
-     * psi[a] = real[2 buffers][6][band volume], read from buffer
-     * cpml_parity, written to the other. */
+     * psi[a] = real[6][band volume], advanced in place (components 0..2 by
+     * the stress half-step, 3..5 by the velocity half-step). */
     int32_t cpml_width;
-    int32_t cpml_parity;
+    int32_t cpml_unused;
     const void *cpml_prof[3];
     void *cpml_psi[3];
 } orc_shot;

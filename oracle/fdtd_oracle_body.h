@@ -11,33 +11,155 @@
 #define CAT_(a, b) a##b
 #define CAT(a, b) CAT_(a, b)
 
-/* CPML derivative of psi component k along axis a (no upstream
- * counterpart): dd is the plain derivative (sum_j c_j * difference / h) at
- * stagger st (0 integer point, 1 half point) of axis index ia.  Outside the
- * band of axis a it is dd itself; inside,
- *   psi' = fma(b, psi, a_coef * dd),  D = fma(dd, 1/kappa, psi'),
- * psi read from buffer cpml_parity and psi' written to the other. */
-static inline REAL CAT(pml_d, SUF)(const orc_shot *S, int a, int k, int st, int iz, int iy,
-                                   int ix, REAL dd)
+/* CPML correction of one derivative (no upstream counterpart): dd is the
+ * derivative along axis a taken on its own (sum_j c_j * difference, with the
+ * per-term edge guards of the plain sums; scaled by 1/h for strains, unscaled
+ * for accelerations) at stagger st (0 integer point, 1 half point), for a
+ * cell inside the band of axis a.  The memory variable k of the cell is
+ * advanced in place,
+ *   psi' = fma(b, psi, a_coef * dd),
+ * and the correction returned is corr = fma(dd, 1/kappa - 1, psi'), so that
+ * the stretched derivative dd/kappa + psi' equals dd + corr. */
+static inline REAL CAT(pml_corr, SUF)(const orc_shot *S, int a, int k, int st, int iz, int iy,
+                                      int ix, REAL dd)
 {
     const int dims[3] = {S->nz, S->ny, S->nx}, idx[3] = {iz, iy, ix};
     const int n = dims[a], ia = idx[a];
-    if (!pml_in(ia, n, S->cpml_width)) return dd;
     const REAL *pr = (const REAL *)S->cpml_prof[a] + (int64_t)st * 3 * n;
-    const int64_t vol = pml_vol(S, a), f = (int64_t)k * vol + pml_flat(S, a, iz, iy, ix);
+    const int64_t f = (int64_t)k * pml_vol(S, a) + pml_flat(S, a, iz, iy, ix);
     REAL *psi = (REAL *)S->cpml_psi[a];
-    const REAL p = FMA(pr[ia], psi[(int64_t)S->cpml_parity * 6 * vol + f], pr[n + ia] * dd);
-    psi[(int64_t)(1 - S->cpml_parity) * 6 * vol + f] = p;
-    return FMA(dd, pr[2 * n + ia], p);
+    const REAL p = FMA(pr[ia], psi[f], pr[n + ia] * dd);
+    psi[f] = p;
+    return FMA(dd, pr[2 * n + ia] - (REAL)1, p);
+}
+
+/* Strain corrections ga[6] of a CPML band cell: for each band axis in the
+ * order z, y, x, its three derivatives (z: Db_z vz -> g3, Df_z vy -> g4,
+ * Df_z vx -> g5; y: Db_y vy -> g2, Df_y vz -> g4, Df_y vx -> g6; x: Db_x vx
+ * -> g1, Df_x vz -> g5, Df_x vy -> g6; memory variables 0, 1, 2 of the
+ * axis, staggers 0, 1, 1) each through pml_corr, added into the component's
+ * correction in that order, starting from 0.  Returns the mask of the
+ * components that received a correction (bit k: g_{k+1}). */
+static inline int CAT(pml_strain_corr, SUF)(const orc_shot *S, int64_t i, int iz, int iy,
+                                            int ix, REAL ga[6])
+{
+    int mask = 0;
+    const int nz = S->nz, ny = S->ny, nx = S->nx, nc = S->nc, L = S->cpml_width;
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    const REAL *vx = (const REAL *)S->v[0], *vy = (const REAL *)S->v[1],
+               *vz = (const REAL *)S->v[2];
+    REAL c[4];
+    for (int j = 0; j < 4; ++j) c[j] = (REAL)S->coeffs[j];
+    const REAL inv_h = (REAL)S->inv_h;
+    for (int k = 0; k < 6; ++k) ga[k] = 0;
+    if (pml_in(iz, nz, L)) {
+        REAL d0 = 0, d1 = 0, d2 = 0;
+        for (int j = 0; j < nc; ++j) {
+            if (iz >= j + 1 && iz < nz - j) d0 = FMA(c[j], vz[i + j * sz] - vz[i - (j + 1) * sz], d0);
+            if (iz >= j && iz < nz - j - 1) {
+                d1 = FMA(c[j], vy[i + (j + 1) * sz] - vy[i - j * sz], d1);
+                d2 = FMA(c[j], vx[i + (j + 1) * sz] - vx[i - j * sz], d2);
+            }
+        }
+        ga[2] = ga[2] + CAT(pml_corr, SUF)(S, 0, 0, 0, iz, iy, ix, d0 * inv_h);
+        ga[3] = ga[3] + CAT(pml_corr, SUF)(S, 0, 1, 1, iz, iy, ix, d1 * inv_h);
+        ga[4] = ga[4] + CAT(pml_corr, SUF)(S, 0, 2, 1, iz, iy, ix, d2 * inv_h);
+        mask |= 0x1c;
+    }
+    if (pml_in(iy, ny, L)) {
+        REAL d0 = 0, d1 = 0, d2 = 0;
+        for (int j = 0; j < nc; ++j) {
+            if (iy >= j + 1 && iy < ny - j) d0 = FMA(c[j], vy[i + j * sy] - vy[i - (j + 1) * sy], d0);
+            if (iy >= j && iy < ny - j - 1) {
+                d1 = FMA(c[j], vz[i + (j + 1) * sy] - vz[i - j * sy], d1);
+                d2 = FMA(c[j], vx[i + (j + 1) * sy] - vx[i - j * sy], d2);
+            }
+        }
+        ga[1] = ga[1] + CAT(pml_corr, SUF)(S, 1, 0, 0, iz, iy, ix, d0 * inv_h);
+        ga[3] = ga[3] + CAT(pml_corr, SUF)(S, 1, 1, 1, iz, iy, ix, d1 * inv_h);
+        ga[5] = ga[5] + CAT(pml_corr, SUF)(S, 1, 2, 1, iz, iy, ix, d2 * inv_h);
+        mask |= 0x2a;
+    }
+    if (pml_in(ix, nx, L)) {
+        REAL d0 = 0, d1 = 0, d2 = 0;
+        for (int j = 0; j < nc; ++j) {
+            if (ix >= j + 1 && ix < nx - j) d0 = FMA(c[j], vx[i + j] - vx[i - (j + 1)], d0);
+            if (ix >= j && ix < nx - j - 1) {
+                d1 = FMA(c[j], vz[i + (j + 1)] - vz[i - j], d1);
+                d2 = FMA(c[j], vy[i + (j + 1)] - vy[i - j], d2);
+            }
+        }
+        ga[0] = ga[0] + CAT(pml_corr, SUF)(S, 2, 0, 0, iz, iy, ix, d0 * inv_h);
+        ga[4] = ga[4] + CAT(pml_corr, SUF)(S, 2, 1, 1, iz, iy, ix, d1 * inv_h);
+        ga[5] = ga[5] + CAT(pml_corr, SUF)(S, 2, 2, 1, iz, iy, ix, d2 * inv_h);
+        mask |= 0x31;
+    }
+    return mask;
+}
+
+/* Acceleration corrections gv[3] of a CPML band cell (unscaled, like the
+ * plain accumulators): for each band axis in the order z, y, x, its three
+ * derivatives (z: D_z s5 -> ax, D_z s4 -> ay, D_z s3 -> az, staggers 0, 0,
+ * 1; y: D_y s6 -> ax, D_y s2 -> ay, D_y s4 -> az, staggers 0, 1, 0; x:
+ * D_x s1 -> ax, D_x s6 -> ay, D_x s5 -> az, staggers 1, 0, 0; memory
+ * variables 3, 4, 5 of the axis) through pml_corr. */
+static inline void CAT(pml_accel_corr, SUF)(const orc_shot *S, int64_t i, int iz, int iy,
+                                            int ix, REAL gv[3])
+{
+    const int nz = S->nz, ny = S->ny, nx = S->nx, nc = S->nc, L = S->cpml_width;
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    const REAL *s1 = (const REAL *)S->s[0], *s2 = (const REAL *)S->s[1],
+               *s3 = (const REAL *)S->s[2], *s4 = (const REAL *)S->s[3],
+               *s5 = (const REAL *)S->s[4], *s6 = (const REAL *)S->s[5];
+    REAL c[4];
+    for (int j = 0; j < 4; ++j) c[j] = (REAL)S->coeffs[j];
+    gv[0] = gv[1] = gv[2] = 0;
+    if (pml_in(iz, nz, L)) {
+        REAL e0 = 0, e1 = 0, e2 = 0;
+        for (int j = 0; j < nc; ++j) {
+            if (iz >= j + 1 && iz < nz - j) {
+                e0 = FMA(c[j], s5[i + j * sz] - s5[i - (j + 1) * sz], e0);
+                e1 = FMA(c[j], s4[i + j * sz] - s4[i - (j + 1) * sz], e1);
+            }
+            if (iz >= j && iz < nz - j - 1) e2 = FMA(c[j], s3[i + (j + 1) * sz] - s3[i - j * sz], e2);
+        }
+        gv[0] = gv[0] + CAT(pml_corr, SUF)(S, 0, 3, 0, iz, iy, ix, e0);
+        gv[1] = gv[1] + CAT(pml_corr, SUF)(S, 0, 4, 0, iz, iy, ix, e1);
+        gv[2] = gv[2] + CAT(pml_corr, SUF)(S, 0, 5, 1, iz, iy, ix, e2);
+    }
+    if (pml_in(iy, ny, L)) {
+        REAL e0 = 0, e1 = 0, e2 = 0;
+        for (int j = 0; j < nc; ++j) {
+            if (iy >= j + 1 && iy < ny - j) {
+                e0 = FMA(c[j], s6[i + j * sy] - s6[i - (j + 1) * sy], e0);
+                e2 = FMA(c[j], s4[i + j * sy] - s4[i - (j + 1) * sy], e2);
+            }
+            if (iy >= j && iy < ny - j - 1) e1 = FMA(c[j], s2[i + (j + 1) * sy] - s2[i - j * sy], e1);
+        }
+        gv[0] = gv[0] + CAT(pml_corr, SUF)(S, 1, 3, 0, iz, iy, ix, e0);
+        gv[1] = gv[1] + CAT(pml_corr, SUF)(S, 1, 4, 1, iz, iy, ix, e1);
+        gv[2] = gv[2] + CAT(pml_corr, SUF)(S, 1, 5, 0, iz, iy, ix, e2);
+    }
+    if (pml_in(ix, nx, L)) {
+        REAL e0 = 0, e1 = 0, e2 = 0;
+        for (int j = 0; j < nc; ++j) {
+            if (ix >= j && ix < nx - j - 1) e0 = FMA(c[j], s1[i + (j + 1)] - s1[i - j], e0);
+            if (ix >= j + 1 && ix < nx - j) {
+                e1 = FMA(c[j], s6[i + j] - s6[i - (j + 1)], e1);
+                e2 = FMA(c[j], s5[i + j] - s5[i - (j + 1)], e2);
+            }
+        }
+        gv[0] = gv[0] + CAT(pml_corr, SUF)(S, 2, 3, 1, iz, iy, ix, e0);
+        gv[1] = gv[1] + CAT(pml_corr, SUF)(S, 2, 4, 0, iz, iy, ix, e1);
+        gv[2] = gv[2] + CAT(pml_corr, SUF)(S, 2, 5, 0, iz, iy, ix, e2);
+    }
 }
 
 /* Strains g1..g6 at every cell, from v (sim/core/fdtd.py:1136-1164).
  * Each g is sum_j c_j * (difference) accumulated j-ascending with per-term
- * edge guards, then scaled by 1/h.  CPML band cells (no upstream
- * counterpart) accumulate each of the nine derivatives on its own, scale it
- * by 1/h, pass it through pml_d and add the two derivatives of a shear
- * strain in the order of the plain sums (g4: y then z, g5: x then z, g6: x
- * then y). */
+ * edge guards, then scaled by 1/h.  A CPML band cell (no upstream
+ * counterpart) then adds its corrections, g_k = g_k + ga_k for each
+ * component a band axis of the cell corrects (pml_strain_corr). */
 static void CAT(strains, SUF)(const orc_shot *S)
 {
     const int nz = S->nz, ny = S->ny, nx = S->nx, nc = S->nc;
@@ -56,41 +178,6 @@ static void CAT(strains, SUF)(const orc_shot *S)
             for (int iy = 0; iy < ny; ++iy)
                 for (int ix = 0; ix < nx; ++ix) {
                     const int64_t i = ((int64_t)iz * ny + iy) * nx + ix;
-                    if (pml_cell(S, iz, iy, ix)) {
-                        REAL d[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-                        for (int j = 0; j < nc; ++j) {
-                            const REAL cj = c[j];
-                            if (ix >= j + 1 && ix < nx - j)
-                                d[0] = FMA(cj, vx[i + j] - vx[i - (j + 1)], d[0]);
-                            if (iy >= j + 1 && iy < ny - j)
-                                d[1] = FMA(cj, vy[i + j * sy] - vy[i - (j + 1) * sy], d[1]);
-                            if (iz >= j + 1 && iz < nz - j)
-                                d[2] = FMA(cj, vz[i + j * sz] - vz[i - (j + 1) * sz], d[2]);
-                            if (iy >= j && iy < ny - j - 1)
-                                d[3] = FMA(cj, vz[i + (j + 1) * sy] - vz[i - j * sy], d[3]);
-                            if (iz >= j && iz < nz - j - 1)
-                                d[4] = FMA(cj, vy[i + (j + 1) * sz] - vy[i - j * sz], d[4]);
-                            if (ix >= j && ix < nx - j - 1)
-                                d[5] = FMA(cj, vz[i + (j + 1)] - vz[i - j], d[5]);
-                            if (iz >= j && iz < nz - j - 1)
-                                d[6] = FMA(cj, vx[i + (j + 1) * sz] - vx[i - j * sz], d[6]);
-                            if (ix >= j && ix < nx - j - 1)
-                                d[7] = FMA(cj, vy[i + (j + 1)] - vy[i - j], d[7]);
-                            if (iy >= j && iy < ny - j - 1)
-                                d[8] = FMA(cj, vx[i + (j + 1) * sy] - vx[i - j * sy], d[8]);
-                        }
-                        for (int t = 0; t < 9; ++t) d[t] = d[t] * inv_h;
-                        g[0 * ncell + i] = CAT(pml_d, SUF)(S, 2, 0, 0, iz, iy, ix, d[0]);
-                        g[1 * ncell + i] = CAT(pml_d, SUF)(S, 1, 0, 0, iz, iy, ix, d[1]);
-                        g[2 * ncell + i] = CAT(pml_d, SUF)(S, 0, 0, 0, iz, iy, ix, d[2]);
-                        g[3 * ncell + i] = CAT(pml_d, SUF)(S, 1, 1, 1, iz, iy, ix, d[3]) +
-                                           CAT(pml_d, SUF)(S, 0, 1, 1, iz, iy, ix, d[4]);
-                        g[4 * ncell + i] = CAT(pml_d, SUF)(S, 2, 1, 1, iz, iy, ix, d[5]) +
-                                           CAT(pml_d, SUF)(S, 0, 2, 1, iz, iy, ix, d[6]);
-                        g[5 * ncell + i] = CAT(pml_d, SUF)(S, 2, 2, 1, iz, iy, ix, d[7]) +
-                                           CAT(pml_d, SUF)(S, 1, 2, 1, iz, iy, ix, d[8]);
-                        continue;
-                    }
                     REAL a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0, a6 = 0;
                     for (int j = 0; j < nc; ++j) {
                         const REAL cj = c[j];
@@ -113,12 +200,15 @@ static void CAT(strains, SUF)(const orc_shot *S)
                         if (iy >= j && iy < ny - j - 1)
                             a6 = FMA(cj, vx[i + (j + 1) * sy] - vx[i - j * sy], a6);
                     }
-                    g[0 * ncell + i] = a1 * inv_h;
-                    g[1 * ncell + i] = a2 * inv_h;
-                    g[2 * ncell + i] = a3 * inv_h;
-                    g[3 * ncell + i] = a4 * inv_h;
-                    g[4 * ncell + i] = a5 * inv_h;
-                    g[5 * ncell + i] = a6 * inv_h;
+                    REAL a[6] = {a1 * inv_h, a2 * inv_h, a3 * inv_h,
+                                 a4 * inv_h, a5 * inv_h, a6 * inv_h};
+                    if (pml_cell(S, iz, iy, ix)) {
+                        REAL ga[6];
+                        const int mask = CAT(pml_strain_corr, SUF)(S, i, iz, iy, ix, ga);
+                        for (int k = 0; k < 6; ++k)
+                            if (mask >> k & 1) a[k] = a[k] + ga[k];
+                    }
+                    for (int k = 0; k < 6; ++k) g[k * ncell + i] = a[k];
                 }
         FP_EXIT
     }
@@ -206,10 +296,9 @@ static void CAT(source, SUF)(const orc_shot *S, REAL wn)
 
 /* Velocity half-step (sim/core/fdtd.py:1211-1244): j-interleaved sums in
  * x, y, z tap order; face density 0.5 (rho[m] + rho[m(+1)]) or rho[m] on the
- * last slice; v <- (v + dt/h * a / rho_face) * damp.  CPML band cells (no
- * upstream counterpart): each of the nine stress derivatives accumulated on
- * its own, scaled by 1/h, through pml_d (psi components 3..5), summed x, y, z
- * per component; v <- (v + dt * a / rho_face) * damp. */
+ * last slice; v <- (v + dt/h * a / rho_face) * damp.  A CPML band cell (no
+ * upstream counterpart) adds its corrections to the sums first: a = a + gv
+ * (pml_accel_corr). */
 static void CAT(velocity, SUF)(const orc_shot *S)
 {
     const int nz = S->nz, ny = S->ny, nx = S->nx, nc = S->nc;
@@ -233,50 +322,6 @@ static void CAT(velocity, SUF)(const orc_shot *S)
             for (int iy = 0; iy < ny; ++iy)
                 for (int ix = 0; ix < nx; ++ix) {
                     const int64_t i = ((int64_t)iz * ny + iy) * nx + ix;
-                    if (pml_cell(S, iz, iy, ix)) {
-                        REAL e[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-                        for (int j = 0; j < nc; ++j) {
-                            const REAL cj = c[j];
-                            if (ix >= j && ix < nx - j - 1)
-                                e[0] = FMA(cj, s1[i + (j + 1)] - s1[i - j], e[0]);
-                            if (ix >= j + 1 && ix < nx - j) {
-                                e[3] = FMA(cj, s6[i + j] - s6[i - (j + 1)], e[3]);
-                                e[6] = FMA(cj, s5[i + j] - s5[i - (j + 1)], e[6]);
-                            }
-                            if (iy >= j && iy < ny - j - 1)
-                                e[4] = FMA(cj, s2[i + (j + 1) * sy] - s2[i - j * sy], e[4]);
-                            if (iy >= j + 1 && iy < ny - j) {
-                                e[1] = FMA(cj, s6[i + j * sy] - s6[i - (j + 1) * sy], e[1]);
-                                e[7] = FMA(cj, s4[i + j * sy] - s4[i - (j + 1) * sy], e[7]);
-                            }
-                            if (iz >= j && iz < nz - j - 1)
-                                e[8] = FMA(cj, s3[i + (j + 1) * sz] - s3[i - j * sz], e[8]);
-                            if (iz >= j + 1 && iz < nz - j) {
-                                e[2] = FMA(cj, s5[i + j * sz] - s5[i - (j + 1) * sz], e[2]);
-                                e[5] = FMA(cj, s4[i + j * sz] - s4[i - (j + 1) * sz], e[5]);
-                            }
-                        }
-                        const REAL inv_h = (REAL)S->inv_h, dt = (REAL)S->dt;
-                        for (int t = 0; t < 9; ++t) e[t] = e[t] * inv_h;
-                        const REAL ax = CAT(pml_d, SUF)(S, 2, 3, 1, iz, iy, ix, e[0]) +
-                                        CAT(pml_d, SUF)(S, 1, 3, 0, iz, iy, ix, e[1]) +
-                                        CAT(pml_d, SUF)(S, 0, 3, 0, iz, iy, ix, e[2]);
-                        const REAL ay = CAT(pml_d, SUF)(S, 2, 4, 0, iz, iy, ix, e[3]) +
-                                        CAT(pml_d, SUF)(S, 1, 4, 1, iz, iy, ix, e[4]) +
-                                        CAT(pml_d, SUF)(S, 0, 4, 0, iz, iy, ix, e[5]);
-                        const REAL az = CAT(pml_d, SUF)(S, 2, 5, 0, iz, iy, ix, e[6]) +
-                                        CAT(pml_d, SUF)(S, 1, 5, 0, iz, iy, ix, e[7]) +
-                                        CAT(pml_d, SUF)(S, 0, 5, 1, iz, iy, ix, e[8]);
-                        const REAL r0 = rho[mats[i]];
-                        const REAL rvx = ix < nx - 1 ? half * (r0 + rho[mats[i + 1]]) : r0;
-                        const REAL rvy = iy < ny - 1 ? half * (r0 + rho[mats[i + sy]]) : r0;
-                        const REAL rvz = iz < nz - 1 ? half * (r0 + rho[mats[i + sz]]) : r0;
-                        const REAL dmp = damp ? damp[i] : (REAL)1;
-                        vx[i] = (vx[i] + dt * ax / rvx) * dmp;
-                        vy[i] = (vy[i] + dt * ay / rvy) * dmp;
-                        vz[i] = (vz[i] + dt * az / rvz) * dmp;
-                        continue;
-                    }
                     REAL ax = 0, ay = 0, az = 0;
                     for (int j = 0; j < nc; ++j) {
                         const REAL cj = c[j];
@@ -298,6 +343,13 @@ static void CAT(velocity, SUF)(const orc_shot *S)
                             ax = FMA(cj, s5[i + j * sz] - s5[i - (j + 1) * sz], ax);
                             ay = FMA(cj, s4[i + j * sz] - s4[i - (j + 1) * sz], ay);
                         }
+                    }
+                    if (pml_cell(S, iz, iy, ix)) {
+                        REAL gv[3];
+                        CAT(pml_accel_corr, SUF)(S, i, iz, iy, ix, gv);
+                        ax = ax + gv[0];
+                        ay = ay + gv[1];
+                        az = az + gv[2];
                     }
                     const REAL r0 = rho[mats[i]];
                     const REAL rvx = ix < nx - 1 ? half * (r0 + rho[mats[i + 1]]) : r0;
@@ -338,14 +390,12 @@ static void CAT(record, SUF)(const orc_shot *S, REAL *row)
 }
 
 /* Steps n0 .. n1-1 of the hot loop (sim/core/fdtd.py:1786-1815):
- * stress, source, velocity, record.  traces is (nt, n_rec), row n written.
- * Step n reads the CPML memory variables from buffer n & 1. */
+ * stress, source, velocity, record.  traces is (nt, n_rec), row n written. */
 void CAT(orc_run, SUF)(const orc_shot *S, int n0, int n1, const double *wavelet,
                        REAL *traces)
 {
     orc_shot Sn = *S;
     for (int n = n0; n < n1; ++n) {
-        Sn.cpml_parity = n & 1;
         CAT(strains, SUF)(&Sn);
         CAT(stress, SUF)(&Sn);
         CAT(source, SUF)(&Sn, (REAL)wavelet[n]);

@@ -52,7 +52,7 @@ class _Shot(C.Structure):
         ("n_rec", C.c_int32), ("rec_ptr", C.c_void_p), ("rec_idx", C.c_void_p),
         ("rec_w", C.c_void_p),
         ("record_mode", C.c_int32),
-        ("cpml_width", C.c_int32), ("cpml_parity", C.c_int32),
+        ("cpml_width", C.c_int32), ("cpml_unused", C.c_int32),
         ("cpml_prof", C.c_void_p * 3), ("cpml_psi", C.c_void_p * 3),
     ]
 
@@ -146,7 +146,8 @@ class OracleShot:
         st.rec_idx = self.rec_idx.ctypes.data
         st.rec_w = self.rec_w.ctypes.data
         # CPML: (width, [prof_z, prof_y, prof_x]) with prof real (2, 3, n);
-        # memory variables (2 buffers, 6 components, band volume) per axis
+        # memory variables (6 components, band volume) per axis, advanced in
+        # place
         self.psi = []
         self.prof = []
         if cpml is not None:
@@ -156,7 +157,7 @@ class OracleShot:
             st.cpml_width = int(width)
             for a in range(3):
                 self.prof.append(np.ascontiguousarray(prof[a], r))
-                self.psi.append(np.zeros((2, 6, vols[a]), r))
+                self.psi.append(np.zeros((6, vols[a]), r))
                 st.cpml_prof[a] = self.prof[a].ctypes.data
                 st.cpml_psi[a] = self.psi[a].ctypes.data
         st.record_mode = RECORD_MODES[record]
@@ -172,10 +173,6 @@ class OracleShot:
         getattr(lib, "orc_run_" + self.suf)(C.byref(self._st), int(n0), int(n1),
                                             wav.ctypes.data, traces.ctypes.data)
         return traces
-
-    def set_step(self, n):
-        """Parity of step n picks the CPML buffer stress()/velocity() read."""
-        self._st.cpml_parity = int(n) & 1
 
     def stress(self):
         getattr(_load(), "orc_stress_" + self.suf)(C.byref(self._st))

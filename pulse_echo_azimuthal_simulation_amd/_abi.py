@@ -16,7 +16,7 @@ PE_KERNEL_DIRECT, PE_KERNEL_TILED = 0, 1
 PE_HIST_VELOCITY, PE_HIST_STRAIN = 0, 1
 RECORD_MODES = {"pressure": 0, "vx": 1, "vy": 2, "vz": 3}
 KERNELS = {"direct": PE_KERNEL_DIRECT, "tiled": PE_KERNEL_TILED}
-ABI_VERSION = 8
+ABI_VERSION = 9
 
 EXPORTS = ("pe_abi_version", "pe_last_error", "pe_check_shot", "pe_stress", "pe_source",
            "pe_velocity", "pe_record", "pe_run", "pe_run_timed", "pe_tile_info", "pe_run_store",
@@ -51,9 +51,10 @@ class PeShot(C.Structure):
         ("tile", C.c_int32 * 2),
         ("zchunk", C.c_int32 * 2),
         ("cpml_width", C.c_int32),
-        ("cpml_step", C.c_int32),
+        ("cpml_reserved", C.c_int32),
         ("cpml_prof", C.c_void_p * 3),
         ("cpml_psi", C.c_void_p * 3),
+        ("cpml_corr", C.c_void_p),
     ]
 
 
@@ -75,7 +76,7 @@ class PeBatch(C.Structure):
         ("n_shot", C.c_int32),
         ("shots", C.c_void_p), ("wavelet", C.c_void_p), ("traces", C.c_void_p),
         ("nt", C.c_void_p), ("table", C.c_void_p), ("step", C.c_void_p),
-        ("nb", C.c_int64 * 2),
+        ("nb", C.c_int64 * 3),
     ]
 
 

@@ -8,8 +8,9 @@ timed over a few launches with HIP events on the launch stream, and the
 fastest is kept per (grid shape, precision) for the life of the process.
 
 Timing runs on the shot's own fields while they are still zero and before
-any source is applied: both half-steps map zero fields to zero fields, so
-tuning leaves the shot's state untouched.
+any source is applied: both half-steps (CPML band pre-passes included) map
+zero fields and memory variables to zero, so tuning leaves the shot's state
+untouched.
 """
 from __future__ import annotations
 
@@ -39,12 +40,6 @@ def _time_half(shot, half, reps):
     return a.elapsed_time(b) / reps
 
 
-# Variants whose tile shape has a CPML band instantiation (the library runs
-# variant 0 for any other on a CPML shot): stress variant 0 only, velocity
-# variants 0, 2 and 3 in float32 (64x8, 32x8, 64x4), variant 0 in float64.
-CPML_VARIANTS = {"f32": {0: (0,), 1: (0, 2, 3)}, "f64": {0: (0,), 1: (0,)}}
-
-
 def tune(shot, reps=5, zchunks=ZCHUNKS, verbose=False):
     """Pick tile variants and z-chunk lengths for ``shot`` (a DeviceShot with
     zero fields); sets them in its descriptor and returns
@@ -60,10 +55,7 @@ def tune(shot, reps=5, zchunks=ZCHUNKS, verbose=False):
     best = {}
     for h in (0, 1):
         variants = _abi.tile_variants(prec, h)
-        cands = range(len(variants))
-        if shot.desc.cpml_width > 0:
-            cands = CPML_VARIANTS[prec][h]
-        for v in cands:
+        for v in range(len(variants)):
             for zc in zchunks:
                 shot.desc.tile[h] = v
                 shot.desc.zchunk[h] = zc

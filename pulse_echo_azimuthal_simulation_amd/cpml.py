@@ -18,8 +18,12 @@ with, at normalised depth xi in (0, 1] into the band (xi = 0 elsewhere),
     a = d * (b - 1) / (kappa * (d + kappa * alpha))      (0 where d = 0)
 
 (the CFS-PML recursion of Roden & Gedney 2000, in the form of Komatitsch &
-Martin 2007, Geophysics 72(5) SM155).  Depth is measured from the last
-interior point: on the low side xi = (L - p) / L for p < L, on the high
+Martin 2007, Geophysics 72(5) SM155).  A band cell is computed as the plain
+cell plus corrections: each derivative along a band axis contributes
+corr = dd * (1/kappa - 1) + psi' to its strain or acceleration, so that
+dd + corr = D; the memory variables advance in place, once per step.
+
+Depth is measured from the last interior point: on the low side xi = (L - p) / L for p < L, on the high
 side xi = (p - (n - 1 - L)) / L for p > n - 1 - L, where p = i for integer
 points and i + 1/2 for half points (the two staggers of the grid), capped
 at 1.  The kernels keep memory variables on a band of L + 1 cells at each

@@ -70,7 +70,6 @@ int run_impl(const pe_shot *p, int32_t n0, int32_t n1, const double *wavelet, vo
     const bool rec_before = p->record_mode == PE_RECORD_PRESSURE;
     const size_t ncell = (size_t)p->nz * p->ny * p->nx;
     for (int32_t n = n0; n < n1; ++n) {
-        s.pml_par = n & 1;
         hipError_t e;
         if (vhist && hist_kind == 0) {
             e = pe::store_velocity<R>(s, (R *)vhist + (size_t)(n - n0) * 3 * ncell, st);
@@ -120,7 +119,6 @@ int run_timed_impl(const pe_shot *p, int32_t n0, int32_t n1, const double *wavel
         const int nb = (n1 - b0) < kBatch ? (n1 - b0) : kBatch;
         for (int b = 0; b < nb && rc == PE_OK; ++b) {
             const int32_t n = b0 + b;
-            s.pml_par = n & 1;
             (void)hipEventRecord(ev[b][0], st);
             hipError_t e = direct ? pe::launch_stress_direct<R>(s, st) : pe::launch_stress_tiled<R>(s, p->tile[0], p->zchunk[0], st);
             (void)hipEventRecord(ev[b][1], st);
@@ -153,17 +151,25 @@ int run_timed_impl(const pe_shot *p, int32_t n0, int32_t n1, const double *wavel
 }
 
 // Layout of a batch table: entries (256-byte aligned block), then the
-// workgroup -> shot maps of the stress and velocity launches.
+// workgroup -> shot maps of the stress and velocity launches and of the
+// CPML band pre-passes (nb[2] blocks, 0 without CPML shots).
 struct BatchLayout {
-    size_t entry_bytes, blk_s_off, blk_v_off, total;
-    int64_t nb[2];
+    size_t entry_bytes, blk_s_off, blk_v_off, blk_p_off, total;
+    int64_t nb[3];
 };
+
+int64_t pml_blocks(const pe_shot *p)
+{
+    if (p->cpml_width <= 0) return 0;
+    const long long cells = pe::pml_band_cells(p->nz, p->ny, p->nx, p->cpml_width);
+    return (cells + pe::kPmlBlock - 1) / pe::kPmlBlock;
+}
 
 int batch_layout(const pe_shot *shots, int32_t n, BatchLayout *L)
 {
     if (!shots || n < 1) return fail(PE_EINVAL, "a batch needs at least one shot");
     const int prec = shots[0].precision;
-    L->nb[0] = L->nb[1] = 0;
+    L->nb[0] = L->nb[1] = L->nb[2] = 0;
     for (int32_t i = 0; i < n; ++i) {
         const pe_shot *p = &shots[i];
         int rc = pe_check_shot(p);
@@ -171,9 +177,7 @@ int batch_layout(const pe_shot *shots, int32_t n, BatchLayout *L)
         if (p->precision != prec) return fail(PE_EINVAL, "shot %d: precision differs from shot 0", i);
         if (p->kernel != PE_KERNEL_TILED)
             return fail(PE_EINVAL, "shot %d: batches run the tiled kernels only", i);
-        if (p->cpml_width > 0 && (shots[0].tile[0] != 0 || shots[0].tile[1] != 0))
-            return fail(PE_EINVAL, "shot %d: a batch with CPML shots runs tile variant 0 "
-                                   "(shot 0 has %d, %d)", i, shots[0].tile[0], shots[0].tile[1]);
+        L->nb[2] += pml_blocks(p);
         for (int h = 0; h < 2; ++h) {
             int tx, ty, zl;
             const int nb = pe::tiled_geometry(prec, h, shots[0].tile[h], p->zchunk[h], p->nz, p->ny,
@@ -182,13 +186,15 @@ int batch_layout(const pe_shot *shots, int32_t n, BatchLayout *L)
             L->nb[h] += nb;
         }
     }
-    if (L->nb[0] >= ((int64_t)1 << 31) || L->nb[1] >= ((int64_t)1 << 31))
+    if (L->nb[0] >= ((int64_t)1 << 31) || L->nb[1] >= ((int64_t)1 << 31) ||
+        L->nb[2] >= ((int64_t)1 << 31))
         return fail(PE_ERANGE, "batch exceeds 2^31 workgroups");
     const size_t esz = prec == PE_F32 ? sizeof(pe::BatchEntry<float>) : sizeof(pe::BatchEntry<double>);
     L->entry_bytes = (esz * n + 255) / 256 * 256;
     L->blk_s_off = L->entry_bytes;
     L->blk_v_off = L->blk_s_off + ((size_t)L->nb[0] * 4 + 255) / 256 * 256;
-    L->total = L->blk_v_off + (size_t)L->nb[1] * 4;
+    L->blk_p_off = L->blk_v_off + ((size_t)L->nb[1] * 4 + 255) / 256 * 256;
+    L->total = L->blk_p_off + (size_t)L->nb[2] * 4;
     return PE_OK;
 }
 
@@ -199,15 +205,17 @@ int batch_setup_impl(pe_batch *b, const BatchLayout &L)
     auto *ent = reinterpret_cast<pe::BatchEntry<R> *>(host.data());
     auto *blk_s = reinterpret_cast<int32_t *>(host.data() + L.blk_s_off);
     auto *blk_v = reinterpret_cast<int32_t *>(host.data() + L.blk_v_off);
-    int64_t off[2] = {0, 0};
+    auto *blk_p = reinterpret_cast<int32_t *>(host.data() + L.blk_p_off);
+    int64_t off[3] = {0, 0, 0};
     for (int32_t i = 0; i < b->n_shot; ++i) {
         const pe_shot *p = &b->shots[i];
         pe::BatchEntry<R> &e = ent[i];
         e.S = pe::make_shot<R>(p);
-        e.S.pml_par = 0;
-        e.S_odd = e.S;
-        e.S_odd.pml_par = 1;
         e.nt = b->nt[i];
+        const int64_t np = pml_blocks(p);
+        e.blk_off[2] = (int)off[2];
+        for (int64_t k = 0; k < np; ++k) blk_p[off[2] + k] = i;
+        off[2] += np;
         for (int h = 0; h < 2; ++h) {
             const int nb = pe::tiled_geometry(p->precision, h, b->shots[0].tile[h], p->zchunk[h],
                                               p->nz, p->ny, p->nx, &e.tiles_x[h], &e.tiles_y[h],
@@ -235,6 +243,7 @@ int batch_setup_impl(pe_batch *b, const BatchLayout &L)
     if (e != hipSuccess) return hip_fail(e, "batch step reset");
     b->nb[0] = L.nb[0];
     b->nb[1] = L.nb[1];
+    b->nb[2] = L.nb[2];
     return PE_OK;
 }
 
@@ -244,22 +253,30 @@ int batch_step_impl(const pe_batch *b, const BatchLayout &L, int32_t steps, hipS
     const auto *tab = reinterpret_cast<const pe::BatchEntry<R> *>(b->table);
     const auto *blk_s = reinterpret_cast<const int32_t *>((const char *)b->table + L.blk_s_off);
     const auto *blk_v = reinterpret_cast<const int32_t *>((const char *)b->table + L.blk_v_off);
-    const int vs = b->shots[0].tile[0], vv = b->shots[0].tile[1];
+    const auto *blk_p = reinterpret_cast<const int32_t *>((const char *)b->table + L.blk_p_off);
+    int vs = b->shots[0].tile[0];
+    const int vv = b->shots[0].tile[1];
     int max_mat = 1, max_pml = 0;
     for (int32_t i = 0; i < b->n_shot; ++i) {
         max_mat = std::max(max_mat, (int)b->shots[i].n_mat);
         max_pml = std::max(max_pml, (int)b->shots[i].cpml_width);
     }
+    const int np = (int)L.nb[2];
+    if (np > 0) {
+        // the one-barrier pipelined stress body has no CPML corrections: a
+        // batch with CPML shots runs the default tile instead
+        int tx, ty, pad, fr;
+        pe::tile_variants(b->shots[0].precision, 0, vs, &tx, &ty, &pad, &fr);
+        if (fr == 2) vs = 0;
+    }
     for (int32_t k = 0; k < steps; ++k) {
-        hipError_t e = max_pml > 0
-            ? pe::launch_stress_batched_pml<R>(tab, blk_s, (int)L.nb[0], b->step, st, max_pml)
-            : pe::launch_stress_batched<R>(vs, tab, blk_s, (int)L.nb[0], b->step, st);
+        hipError_t e = pe::launch_pml_corr_batched<R>(0, tab, blk_p, np, b->step, st);
+        if (e == hipSuccess)
+            e = pe::launch_stress_batched<R>(vs, tab, blk_s, (int)L.nb[0], b->step, st, np > 0);
         if (e == hipSuccess) e = pe::launch_source_record_batched<R>(tab, b->n_shot, b->step, 0, st);
         if (e == hipSuccess)
-            e = max_pml > 0 ? pe::launch_velocity_batched_pml<R>(tab, blk_v, (int)L.nb[1], b->step,
-                                                                 st, max_mat, max_pml)
-                            : pe::launch_velocity_batched<R>(vv, tab, blk_v, (int)L.nb[1], b->step,
-                                                             st, max_mat);
+            e = pe::launch_velocity_batched<R>(vv, tab, blk_v, (int)L.nb[1], b->step, st, max_mat,
+                                               max_pml);
         if (e == hipSuccess) e = pe::launch_source_record_batched<R>(tab, b->n_shot, b->step, 1, st);
         if (e != hipSuccess) return hip_fail(e, "batched step launch");
     }
@@ -373,6 +390,8 @@ int pe_check_shot(const pe_shot *p)
             if (!p->cpml_prof[a] || !p->cpml_psi[a])
                 return fail(PE_EINVAL, "CPML profile or memory variables of axis %d are NULL", a);
         }
+        if (p->kernel == PE_KERNEL_TILED && !p->cpml_corr)
+            return fail(PE_EINVAL, "a tiled CPML shot needs cpml_corr (6 x ncell)");
     }
     return PE_OK;
 }
@@ -658,7 +677,7 @@ int pe_batch_step(const pe_batch *b, int32_t steps, void *stream)
     BatchLayout L;
     int rc = batch_check(b, &L);
     if (rc) return rc;
-    if (b->nb[0] != L.nb[0] || b->nb[1] != L.nb[1])
+    if (b->nb[0] != L.nb[0] || b->nb[1] != L.nb[1] || b->nb[2] != L.nb[2])
         return fail(PE_EINVAL, "batch changed since pe_batch_setup");
     if (steps < 0) return fail(PE_EINVAL, "negative step count");
     return b->shots[0].precision == PE_F32

@@ -31,13 +31,14 @@ struct Shot {
     R *v[3];
     R *s[6];
     R *g;
-    // CPML (no upstream counterpart): band half-width (0 = none), the psi
-    // buffer read this step (the other one is written), per-axis (z, y, x)
-    // profiles real[2 staggers][b, a, 1/kappa][n] and memory variables
-    // real[2 buffers][6][band volume].
-    int pml, pml_par;
+    // CPML (no upstream counterpart): band half-width (0 = none), per-axis
+    // (z, y, x) profiles real[2 staggers][b, a, 1/kappa][n], memory
+    // variables real[6][band volume] (advanced in place) and, for the tiled
+    // kernels, the corrections real[6][ncell] the band pre-pass writes.
+    int pml;
     const R *prof[3];
     R *psi[3];
+    R *corr;
 };
 
 template <typename R>
@@ -62,11 +63,11 @@ inline Shot<R> make_shot(const pe_shot *p)
     for (int k = 0; k < 6; ++k) s.s[k] = (R *)p->s[k];
     s.g = (R *)p->g;
     s.pml = p->cpml_width;
-    s.pml_par = p->cpml_step & 1;
     for (int a = 0; a < 3; ++a) {
         s.prof[a] = (const R *)p->cpml_prof[a];
         s.psi[a] = (R *)p->cpml_psi[a];
     }
+    s.corr = (R *)p->cpml_corr;
     return s;
 }
 
@@ -76,14 +77,14 @@ __device__ __forceinline__ double fma_r(double a, double b, double c) { return _
 // ------------------------------------------------------------------ CPML
 // Convolutional PML (no upstream counterpart).  Axis A (0 z, 1 y, 2 x) of
 // length n has a band of L+1 cells at each end: i <= L or i >= n-1-L, band
-// index i or i - (n - 2L - 2).  A cell in the band of any axis takes the
-// band path of the stencils: each derivative accumulated on its own, scaled
-// by 1/h, and along a band axis replaced by
-//   psi' = fma(b, psi, a * dd),  D = fma(dd, 1/kappa, psi')
-// with the profile of its stagger ST (0 integer point, 1 half point) and
-// memory variable K (0..2 strains, 3..5 accelerations).  psi is read from
-// buffer pml_par and written to the other, so a value read twice in one
-// step (the ring columns of neighbouring stress tiles) is the same.
+// index i or i - (n - 2L - 2).  A cell in the band of any axis is the plain
+// cell plus corrections: for each band axis (in the order z, y, x) its three
+// derivatives dd are taken on their own (the plain sums' taps and per-term
+// guards; strains scaled by 1/h, accelerations unscaled), memory variable K
+// of the cell is advanced in place, psi' = fma(b, psi, a * dd), with the
+// profile of the derivative's stagger ST (0 integer point, 1 half point),
+// and corr = fma(dd, 1/kappa - 1, psi') is added into the correction of the
+// component the derivative belongs to.  Then dd/kappa + psi' = dd + corr.
 __device__ __forceinline__ bool pml_in(int i, int n, int L) { return i <= L || i >= n - 1 - L; }
 
 template <typename R>
@@ -94,12 +95,11 @@ __device__ __forceinline__ bool pml_cell(const Shot<R> &S, int iz, int iy, int i
 }
 
 template <int A, int K, int ST, typename R>
-__device__ __forceinline__ R pml_d(const Shot<R> &S, int iz, int iy, int ix, R dd, bool write)
+__device__ __forceinline__ R pml_corr(const Shot<R> &S, int iz, int iy, int ix, R dd)
 {
     const int L = S.pml, W = 2 * L + 2;
     const int n = A == 0 ? S.nz : (A == 1 ? S.ny : S.nx);
     const int ia = A == 0 ? iz : (A == 1 ? iy : ix);
-    if (!pml_in(ia, n, L)) return dd;
     const int bi = ia <= L ? ia : ia - (n - W);
     size_t vol, f;
     if constexpr (A == 0) {
@@ -113,11 +113,197 @@ __device__ __forceinline__ R pml_d(const Shot<R> &S, int iz, int iy, int ix, R d
         f = ((size_t)iz * S.ny + iy) * W + bi;
     }
     const R *pr = S.prof[A] + (size_t)ST * 3 * n;
-    f += (size_t)K * vol;
-    R *psi = S.psi[A];
-    const R p = fma_r(pr[ia], psi[(size_t)S.pml_par * 6 * vol + f], pr[n + ia] * dd);
-    if (write) psi[(size_t)(1 - S.pml_par) * 6 * vol + f] = p;
-    return fma_r(dd, pr[2 * n + ia], p);
+    R *psi = S.psi[A] + (size_t)K * vol + f;
+    const R p = fma_r(pr[ia], *psi, pr[n + ia] * dd);
+    *psi = p;
+    return fma_r(dd, pr[2 * n + ia] - (R)1, p);
+}
+
+// Components of the strains a cell's bands correct (bit k: g_{k+1}): the z
+// band g3, g4, g5; the y band g2, g4, g6; the x band g1, g5, g6.
+constexpr int kPmlMaskZ = 0x1c, kPmlMaskY = 0x2a, kPmlMaskX = 0x31;
+
+template <typename R>
+__device__ __forceinline__ int pml_strain_mask(const Shot<R> &S, int iz, int iy, int ix)
+{
+    const int L = S.pml;
+    return (pml_in(iz, S.nz, L) ? kPmlMaskZ : 0) | (pml_in(iy, S.ny, L) ? kPmlMaskY : 0) |
+           (pml_in(ix, S.nx, L) ? kPmlMaskX : 0);
+}
+
+// Strain corrections ga[6] of the band cell (iz, iy, ix) at flat index i,
+// from v: z band: Db_z vz -> g3, Df_z vy -> g4, Df_z vx -> g5; y band:
+// Db_y vy -> g2, Df_y vz -> g4, Df_y vx -> g6; x band: Db_x vx -> g1,
+// Df_x vz -> g5, Df_x vy -> g6 (memory variables 0, 1, 2; staggers 0, 1, 1).
+// Only the components of pml_strain_mask are corrected (added to the plain
+// strain); the others are left 0.
+template <typename R>
+__device__ __forceinline__ void pml_strain_corr(const Shot<R> &S, int i, int iz, int iy, int ix,
+                                                R (&ga)[6])
+{
+    const int nz = S.nz, ny = S.ny, nx = S.nx, L = S.pml;
+    const int sy = nx, sz = nx * ny;
+    const R *__restrict__ vx = S.v[0];
+    const R *__restrict__ vy = S.v[1];
+    const R *__restrict__ vz = S.v[2];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) ga[k] = 0;
+    if (pml_in(iz, nz, L)) {
+        R d0 = 0, d1 = 0, d2 = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j >= S.nc) break;
+            const R cj = S.c[j];
+            if (iz >= j + 1 && iz < nz - j) d0 = fma_r(cj, vz[i + j * sz] - vz[i - (j + 1) * sz], d0);
+            if (iz >= j && iz < nz - j - 1) {
+                d1 = fma_r(cj, vy[i + (j + 1) * sz] - vy[i - j * sz], d1);
+                d2 = fma_r(cj, vx[i + (j + 1) * sz] - vx[i - j * sz], d2);
+            }
+        }
+        ga[2] = ga[2] + pml_corr<0, 0, 0>(S, iz, iy, ix, d0 * S.inv_h);
+        ga[3] = ga[3] + pml_corr<0, 1, 1>(S, iz, iy, ix, d1 * S.inv_h);
+        ga[4] = ga[4] + pml_corr<0, 2, 1>(S, iz, iy, ix, d2 * S.inv_h);
+    }
+    if (pml_in(iy, ny, L)) {
+        R d0 = 0, d1 = 0, d2 = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j >= S.nc) break;
+            const R cj = S.c[j];
+            if (iy >= j + 1 && iy < ny - j) d0 = fma_r(cj, vy[i + j * sy] - vy[i - (j + 1) * sy], d0);
+            if (iy >= j && iy < ny - j - 1) {
+                d1 = fma_r(cj, vz[i + (j + 1) * sy] - vz[i - j * sy], d1);
+                d2 = fma_r(cj, vx[i + (j + 1) * sy] - vx[i - j * sy], d2);
+            }
+        }
+        ga[1] = ga[1] + pml_corr<1, 0, 0>(S, iz, iy, ix, d0 * S.inv_h);
+        ga[3] = ga[3] + pml_corr<1, 1, 1>(S, iz, iy, ix, d1 * S.inv_h);
+        ga[5] = ga[5] + pml_corr<1, 2, 1>(S, iz, iy, ix, d2 * S.inv_h);
+    }
+    if (pml_in(ix, nx, L)) {
+        R d0 = 0, d1 = 0, d2 = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j >= S.nc) break;
+            const R cj = S.c[j];
+            if (ix >= j + 1 && ix < nx - j) d0 = fma_r(cj, vx[i + j] - vx[i - (j + 1)], d0);
+            if (ix >= j && ix < nx - j - 1) {
+                d1 = fma_r(cj, vz[i + (j + 1)] - vz[i - j], d1);
+                d2 = fma_r(cj, vy[i + (j + 1)] - vy[i - j], d2);
+            }
+        }
+        ga[0] = ga[0] + pml_corr<2, 0, 0>(S, iz, iy, ix, d0 * S.inv_h);
+        ga[4] = ga[4] + pml_corr<2, 1, 1>(S, iz, iy, ix, d1 * S.inv_h);
+        ga[5] = ga[5] + pml_corr<2, 2, 1>(S, iz, iy, ix, d2 * S.inv_h);
+    }
+}
+
+// Acceleration corrections gv[3] (unscaled, like the plain sums) of a band
+// cell, from s: z band: D_z s5 -> ax, D_z s4 -> ay, D_z s3 -> az (staggers
+// 0, 0, 1); y band: D_y s6 -> ax, D_y s2 -> ay, D_y s4 -> az (0, 1, 0); x
+// band: D_x s1 -> ax, D_x s6 -> ay, D_x s5 -> az (1, 0, 0); memory
+// variables 3, 4, 5.
+template <typename R>
+__device__ __forceinline__ void pml_accel_corr(const Shot<R> &S, int i, int iz, int iy, int ix,
+                                               R (&gv)[3])
+{
+    const int nz = S.nz, ny = S.ny, nx = S.nx, L = S.pml;
+    const int sy = nx, sz = nx * ny;
+    const R *__restrict__ s1 = S.s[0];
+    const R *__restrict__ s2 = S.s[1];
+    const R *__restrict__ s3 = S.s[2];
+    const R *__restrict__ s4 = S.s[3];
+    const R *__restrict__ s5 = S.s[4];
+    const R *__restrict__ s6 = S.s[5];
+    gv[0] = gv[1] = gv[2] = 0;
+    if (pml_in(iz, nz, L)) {
+        R e0 = 0, e1 = 0, e2 = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j >= S.nc) break;
+            const R cj = S.c[j];
+            if (iz >= j + 1 && iz < nz - j) {
+                e0 = fma_r(cj, s5[i + j * sz] - s5[i - (j + 1) * sz], e0);
+                e1 = fma_r(cj, s4[i + j * sz] - s4[i - (j + 1) * sz], e1);
+            }
+            if (iz >= j && iz < nz - j - 1) e2 = fma_r(cj, s3[i + (j + 1) * sz] - s3[i - j * sz], e2);
+        }
+        gv[0] = gv[0] + pml_corr<0, 3, 0>(S, iz, iy, ix, e0);
+        gv[1] = gv[1] + pml_corr<0, 4, 0>(S, iz, iy, ix, e1);
+        gv[2] = gv[2] + pml_corr<0, 5, 1>(S, iz, iy, ix, e2);
+    }
+    if (pml_in(iy, ny, L)) {
+        R e0 = 0, e1 = 0, e2 = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j >= S.nc) break;
+            const R cj = S.c[j];
+            if (iy >= j + 1 && iy < ny - j) {
+                e0 = fma_r(cj, s6[i + j * sy] - s6[i - (j + 1) * sy], e0);
+                e2 = fma_r(cj, s4[i + j * sy] - s4[i - (j + 1) * sy], e2);
+            }
+            if (iy >= j && iy < ny - j - 1) e1 = fma_r(cj, s2[i + (j + 1) * sy] - s2[i - j * sy], e1);
+        }
+        gv[0] = gv[0] + pml_corr<1, 3, 0>(S, iz, iy, ix, e0);
+        gv[1] = gv[1] + pml_corr<1, 4, 1>(S, iz, iy, ix, e1);
+        gv[2] = gv[2] + pml_corr<1, 5, 0>(S, iz, iy, ix, e2);
+    }
+    if (pml_in(ix, nx, L)) {
+        R e0 = 0, e1 = 0, e2 = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j >= S.nc) break;
+            const R cj = S.c[j];
+            if (ix >= j && ix < nx - j - 1) e0 = fma_r(cj, s1[i + (j + 1)] - s1[i - j], e0);
+            if (ix >= j + 1 && ix < nx - j) {
+                e1 = fma_r(cj, s6[i + j] - s6[i - (j + 1)], e1);
+                e2 = fma_r(cj, s5[i + j] - s5[i - (j + 1)], e2);
+            }
+        }
+        gv[0] = gv[0] + pml_corr<2, 3, 1>(S, iz, iy, ix, e0);
+        gv[1] = gv[1] + pml_corr<2, 4, 0>(S, iz, iy, ix, e1);
+        gv[2] = gv[2] + pml_corr<2, 5, 0>(S, iz, iy, ix, e2);
+    }
+}
+
+// The band cells of a grid enumerated as one index range (the pre-passes of
+// the tiled kernels run one thread per band cell): first the 2L+2 z-band
+// planes, then, plane by plane over the middle planes, the 2L+2 y-band rows
+// and the x-band cells of the middle rows.  Every axis holds >= 2L+2 cells.
+__host__ __device__ __forceinline__ long long pml_band_cells(int nz, int ny, int nx, int L)
+{
+    const long long W = 2 * (long long)L + 2;
+    const long long ring = (long long)ny * nx - ((long long)ny - W) * ((long long)nx - W);
+    return W * ny * nx + ((long long)nz - W) * ring;
+}
+
+__device__ __forceinline__ void pml_band_cell(int b, int nz, int ny, int nx, int L, int &iz,
+                                              int &iy, int &ix)
+{
+    const int W = 2 * L + 2;
+    const int plane = ny * nx;
+    if (b < W * plane) {
+        const int p = b / plane, r = b - p * plane;
+        iz = p <= L ? p : p + (nz - W);
+        iy = r / nx;
+        ix = r - iy * nx;
+        return;
+    }
+    b -= W * plane;
+    const int ring = plane - (ny - W) * (nx - W);
+    const int m = b / ring;
+    int r = b - m * ring;
+    iz = L + 1 + m;
+    if (r < W * nx) {
+        const int q = r / nx;
+        iy = q <= L ? q : q + (ny - W);
+        ix = r - q * nx;
+    } else {
+        r -= W * nx;
+        const int q = r / W, c = r - q * W;
+        iy = L + 1 + q;
+        ix = c <= L ? c : c + (nx - W);
+    }
 }
 
 // Workgroup barrier for LDS hand-offs only.  __syncthreads() also fences
@@ -180,10 +366,11 @@ __device__ __forceinline__ R moved_direct(const R *__restrict__ gJ, int i, int i
 // first workgroup in the batched launch, and its source / receiver arrays.
 template <typename R>
 struct BatchEntry {
-    Shot<R> S;          // the shot (CPML memory-variable buffer parity 0: even steps)
-    Shot<R> S_odd;      // the same with parity 1 (odd steps)
+    Shot<R> S;
     int nt;
-    int tiles_x[2], tiles_y[2], zc_len[2], blk_off[2];
+    // [0] stress, [1] velocity; [2] the CPML band pre-passes (blocks of
+    // kPmlBlock band cells; 0 without a CPML)
+    int tiles_x[2], tiles_y[2], zc_len[2], blk_off[3];
     int n_src, n_rec, mode;
     const int64_t *src_idx;
     const R *src_w;
@@ -218,21 +405,22 @@ int tile_variants(int precision, int half, int variant, int *tx, int *ty, int *p
 // such variant).
 int tiled_geometry(int precision, int half, int variant, int zchunk, int nz, int ny, int nx,
                    int *tiles_x, int *tiles_y, int *zc_len);
+// cp: some shot of the batch has a CPML (the kernel adds band corrections).
 template <typename R>
 hipError_t launch_stress_batched(int variant, const BatchEntry<R> *tab, const int32_t *blk, int nb,
-                                 const int32_t *step, hipStream_t st);
+                                 const int32_t *step, hipStream_t st, bool cp);
+// max_pml: the widest CPML band of the batch (0: no CPML shot).
 template <typename R>
 hipError_t launch_velocity_batched(int variant, const BatchEntry<R> *tab, const int32_t *blk,
-                                   int nb, const int32_t *step, hipStream_t st, int max_mat);
-// The same for a batch with CPML shots (tile variant 0; max_pml: the widest
-// band of the batch).
+                                   int nb, const int32_t *step, hipStream_t st, int max_mat,
+                                   int max_pml);
+// CPML band pre-pass of a batch (strain corrections from v, before the
+// stress kernel) over the band cells of every CPML shot; blk maps a block to
+// its shot.
+constexpr int kPmlBlock = 256;
 template <typename R>
-hipError_t launch_stress_batched_pml(const BatchEntry<R> *tab, const int32_t *blk, int nb,
-                                     const int32_t *step, hipStream_t st, int max_pml);
-template <typename R>
-hipError_t launch_velocity_batched_pml(const BatchEntry<R> *tab, const int32_t *blk, int nb,
-                                       const int32_t *step, hipStream_t st, int max_mat,
-                                       int max_pml);
+hipError_t launch_pml_corr_batched(int half, const BatchEntry<R> *tab, const int32_t *blk, int nb,
+                                   const int32_t *step, hipStream_t st);
 // Source / record of every active shot (phase 0, after the stress
 // half-step) and velocity-mode record plus step advance (phase 1, after the
 // velocity half-step); one workgroup per shot.

@@ -19,99 +19,7 @@ namespace {
 
 constexpr int kBlock = 256;
 
-// CPML band cell of the strain kernel: the nine derivatives on their own,
-// through pml_d, shear pairs added in the order of the plain sums.
-template <typename R>
-__device__ __forceinline__ void pml_strains_cell(const Shot<R> &S, int i, int iz,
-                                                            int iy, int ix)
-{
-    const int nz = S.nz, ny = S.ny, nx = S.nx, sy = nx, sz = nx * ny;
-    const R *__restrict__ vx = S.v[0];
-    const R *__restrict__ vy = S.v[1];
-    const R *__restrict__ vz = S.v[2];
-    R d[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        if (j >= S.nc) break;
-        const R cj = S.c[j];
-        if (ix >= j + 1 && ix < nx - j) d[0] = fma_r(cj, vx[i + j] - vx[i - (j + 1)], d[0]);
-        if (iy >= j + 1 && iy < ny - j) d[1] = fma_r(cj, vy[i + j * sy] - vy[i - (j + 1) * sy], d[1]);
-        if (iz >= j + 1 && iz < nz - j) d[2] = fma_r(cj, vz[i + j * sz] - vz[i - (j + 1) * sz], d[2]);
-        if (iy >= j && iy < ny - j - 1) d[3] = fma_r(cj, vz[i + (j + 1) * sy] - vz[i - j * sy], d[3]);
-        if (iz >= j && iz < nz - j - 1) d[4] = fma_r(cj, vy[i + (j + 1) * sz] - vy[i - j * sz], d[4]);
-        if (ix >= j && ix < nx - j - 1) d[5] = fma_r(cj, vz[i + (j + 1)] - vz[i - j], d[5]);
-        if (iz >= j && iz < nz - j - 1) d[6] = fma_r(cj, vx[i + (j + 1) * sz] - vx[i - j * sz], d[6]);
-        if (ix >= j && ix < nx - j - 1) d[7] = fma_r(cj, vy[i + (j + 1)] - vy[i - j], d[7]);
-        if (iy >= j && iy < ny - j - 1) d[8] = fma_r(cj, vx[i + (j + 1) * sy] - vx[i - j * sy], d[8]);
-    }
-#pragma unroll
-    for (int t = 0; t < 9; ++t) d[t] = d[t] * S.inv_h;
-    R *__restrict__ g = S.g;
-    const size_t n = (size_t)nz * sz;
-    g[0 * n + i] = pml_d<2, 0, 0>(S, iz, iy, ix, d[0], true);
-    g[1 * n + i] = pml_d<1, 0, 0>(S, iz, iy, ix, d[1], true);
-    g[2 * n + i] = pml_d<0, 0, 0>(S, iz, iy, ix, d[2], true);
-    g[3 * n + i] = pml_d<1, 1, 1>(S, iz, iy, ix, d[3], true) + pml_d<0, 1, 1>(S, iz, iy, ix, d[4], true);
-    g[4 * n + i] = pml_d<2, 1, 1>(S, iz, iy, ix, d[5], true) + pml_d<0, 2, 1>(S, iz, iy, ix, d[6], true);
-    g[5 * n + i] = pml_d<2, 2, 1>(S, iz, iy, ix, d[7], true) + pml_d<1, 2, 1>(S, iz, iy, ix, d[8], true);
-}
-
-// CPML band cell of the velocity kernel: the nine stress derivatives on
-// their own through pml_d (memory variables 3..5), summed x, y, z per
-// component; v <- (v + dt * a / rho_face) * damp.
-template <typename R>
-__device__ __forceinline__ void pml_velocity_cell(const Shot<R> &S, int i, int iz,
-                                                             int iy, int ix)
-{
-    const int nz = S.nz, ny = S.ny, nx = S.nx, sy = nx, sz = nx * ny;
-    const R *__restrict__ s1 = S.s[0];
-    const R *__restrict__ s2 = S.s[1];
-    const R *__restrict__ s3 = S.s[2];
-    const R *__restrict__ s4 = S.s[3];
-    const R *__restrict__ s5 = S.s[4];
-    const R *__restrict__ s6 = S.s[5];
-    R e[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        if (j >= S.nc) break;
-        const R cj = S.c[j];
-        if (ix >= j && ix < nx - j - 1) e[0] = fma_r(cj, s1[i + (j + 1)] - s1[i - j], e[0]);
-        if (ix >= j + 1 && ix < nx - j) {
-            e[3] = fma_r(cj, s6[i + j] - s6[i - (j + 1)], e[3]);
-            e[6] = fma_r(cj, s5[i + j] - s5[i - (j + 1)], e[6]);
-        }
-        if (iy >= j && iy < ny - j - 1) e[4] = fma_r(cj, s2[i + (j + 1) * sy] - s2[i - j * sy], e[4]);
-        if (iy >= j + 1 && iy < ny - j) {
-            e[1] = fma_r(cj, s6[i + j * sy] - s6[i - (j + 1) * sy], e[1]);
-            e[7] = fma_r(cj, s4[i + j * sy] - s4[i - (j + 1) * sy], e[7]);
-        }
-        if (iz >= j && iz < nz - j - 1) e[8] = fma_r(cj, s3[i + (j + 1) * sz] - s3[i - j * sz], e[8]);
-        if (iz >= j + 1 && iz < nz - j) {
-            e[2] = fma_r(cj, s5[i + j * sz] - s5[i - (j + 1) * sz], e[2]);
-            e[5] = fma_r(cj, s4[i + j * sz] - s4[i - (j + 1) * sz], e[5]);
-        }
-    }
-#pragma unroll
-    for (int t = 0; t < 9; ++t) e[t] = e[t] * S.inv_h;
-    const R ax = pml_d<2, 3, 1>(S, iz, iy, ix, e[0], true) + pml_d<1, 3, 0>(S, iz, iy, ix, e[1], true) +
-                 pml_d<0, 3, 0>(S, iz, iy, ix, e[2], true);
-    const R ay = pml_d<2, 4, 0>(S, iz, iy, ix, e[3], true) + pml_d<1, 4, 1>(S, iz, iy, ix, e[4], true) +
-                 pml_d<0, 4, 0>(S, iz, iy, ix, e[5], true);
-    const R az = pml_d<2, 5, 0>(S, iz, iy, ix, e[6], true) + pml_d<1, 5, 0>(S, iz, iy, ix, e[7], true) +
-                 pml_d<0, 5, 1>(S, iz, iy, ix, e[8], true);
-    const R *__restrict__ rho = S.rho;
-    const int32_t *__restrict__ mats = S.mats;
-    const R r0 = rho[mats[i]];
-    const R rvx = ix < nx - 1 ? (R)0.5 * (r0 + rho[mats[i + 1]]) : r0;
-    const R rvy = iy < ny - 1 ? (R)0.5 * (r0 + rho[mats[i + sy]]) : r0;
-    const R rvz = iz < nz - 1 ? (R)0.5 * (r0 + rho[mats[i + sz]]) : r0;
-    const R dmp = S.damp ? S.damp[i] : (R)1;
-    S.v[0][i] = (S.v[0][i] + S.dt * ax / rvx) * dmp;
-    S.v[1][i] = (S.v[1][i] + S.dt * ay / rvy) * dmp;
-    S.v[2][i] = (S.v[2][i] + S.dt * az / rvz) * dmp;
-}
-
-// PML: the shot has a CPML boundary (band cells take pml_strains_cell).
+// PML: the shot has a CPML boundary (band cells add pml_strain_corr).
 template <typename R, bool PML = false>
 __global__ __launch_bounds__(kBlock) void k_strain_direct(Shot<R> S)
 {
@@ -124,12 +32,6 @@ __global__ __launch_bounds__(kBlock) void k_strain_direct(Shot<R> S)
     const R *__restrict__ vx = S.v[0];
     const R *__restrict__ vy = S.v[1];
     const R *__restrict__ vz = S.v[2];
-    if constexpr (PML) {
-        if (pml_cell(S, iz, iy, ix)) {
-            pml_strains_cell(S, i, iz, iy, ix);
-            return;
-        }
-    }
     R a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0, a6 = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -145,14 +47,21 @@ __global__ __launch_bounds__(kBlock) void k_strain_direct(Shot<R> S)
         if (ix >= j && ix < nx - j - 1) a6 = fma_r(cj, vy[i + (j + 1)] - vy[i - j], a6);
         if (iy >= j && iy < ny - j - 1) a6 = fma_r(cj, vx[i + (j + 1) * sy] - vx[i - j * sy], a6);
     }
+    R a[6] = {a1 * S.inv_h, a2 * S.inv_h, a3 * S.inv_h, a4 * S.inv_h, a5 * S.inv_h, a6 * S.inv_h};
+    if constexpr (PML) {
+        if (pml_cell(S, iz, iy, ix)) {
+            R ga[6];
+            pml_strain_corr(S, i, iz, iy, ix, ga);
+            const int mask = pml_strain_mask(S, iz, iy, ix);
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                if (mask >> k & 1) a[k] = a[k] + ga[k];
+        }
+    }
     R *__restrict__ g = S.g;
     const size_t n = (size_t)ncell;
-    g[0 * n + i] = a1 * S.inv_h;
-    g[1 * n + i] = a2 * S.inv_h;
-    g[2 * n + i] = a3 * S.inv_h;
-    g[3 * n + i] = a4 * S.inv_h;
-    g[4 * n + i] = a5 * S.inv_h;
-    g[5 * n + i] = a6 * S.inv_h;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) g[k * n + i] = a[k];
 }
 
 // Stress component I at position P = kVP[I]: acc = sum over active J of
@@ -230,12 +139,6 @@ __global__ __launch_bounds__(kBlock) void k_velocity_direct(Shot<R> S)
     const R *__restrict__ s4 = S.s[3];
     const R *__restrict__ s5 = S.s[4];
     const R *__restrict__ s6 = S.s[5];
-    if constexpr (PML) {
-        if (pml_cell(S, iz, iy, ix)) {
-            pml_velocity_cell(S, i, iz, iy, ix);
-            return;
-        }
-    }
     R ax = 0, ay = 0, az = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -255,6 +158,15 @@ __global__ __launch_bounds__(kBlock) void k_velocity_direct(Shot<R> S)
         if (iz >= j + 1 && iz < nz - j) {
             ax = fma_r(cj, s5[i + j * sz] - s5[i - (j + 1) * sz], ax);
             ay = fma_r(cj, s4[i + j * sz] - s4[i - (j + 1) * sz], ay);
+        }
+    }
+    if constexpr (PML) {
+        if (pml_cell(S, iz, iy, ix)) {
+            R gv[3];
+            pml_accel_corr(S, i, iz, iy, ix, gv);
+            ax = ax + gv[0];
+            ay = ay + gv[1];
+            az = az + gv[2];
         }
     }
     const R *__restrict__ rho = S.rho;

@@ -19,8 +19,6 @@
 // consecutive workgroup ids to the 8 XCDs round robin; the id is remapped so
 // that each XCD receives a contiguous run of neighbouring tiles and their
 // shared halo planes stay in that XCD's L2.
-#include <mutex>
-
 #include "tiled_common.hpp"
 
 namespace pe {
@@ -32,24 +30,23 @@ using namespace tiled;
 // Body of the stress kernel for logical tile t (tile x fastest, then tile
 // y, then z-chunk) of shot S.  STORE_G: also write the strains g1..g6 of the
 // tile's own columns into S.g (6 x ncell, the history the adjoint's
-// stiffness gradient reads), each plane once.  FR: see StressTile.  PM: the
-// CPML band kernel of the tile (band_kernel; 0 = no band path compiled), band
-// columns take strains_col_pml.
+// stiffness gradient reads), each plane once.  FR: see StressTile.  CP:
+// the shot may have a CPML; the strains of its band cells (tile, ring
+// columns and chunk-edge planes alike) then get the corrections the band
+// pre-pass wrote into S.corr, loaded after the step's first barrier (loaded
+// with the step's prefetches they would not fit the register budget).
 template <typename R, int TX, int TY, int PAD, bool STORE_G = false, bool FR = false,
-          int PM = 0>
+          bool CP = false>
 __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x, int tiles_y,
                                                   int zc_len, int t, int zlo, int zhi)
 {
     using T = StressTile<R, TX, TY, PAD, FR>;
-    constexpr bool PML = PM != 0;
     constexpr int GX = T::GX, GY = T::GY, NG = T::NG, VX = T::VX, VY = T::VY, NV = T::NV;
     constexpr int NT = T::NT, NF = T::NF, WX = T::WX;
     extern __shared__ unsigned char lds_raw[];
     R *vt = reinterpret_cast<R *>(lds_raw);       // [3][VY][VX]
     R *gt = vt + 3 * NV;                           // [6][3][NT]: strains by thread
     R *fr = gt + 6 * 3 * NT;                       // FR: [4][NF][3][NT] frame ring
-    R *bp = fr + T::NRING;                         // PML: band profiles (pml_profiles_to_lds)
-    if constexpr (PML) pml_profiles_to_lds(S, bp, NT);
 
     const int nz = S.nz, ny = S.ny, nx = S.nx;
     const int sy = nx, sz = nx * ny;
@@ -66,7 +63,10 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
     const int gx = gact ? tid % GX : 0, gy = gact ? tid / GX : 0;
     const int ix = x0 - 1 + gx, iy = y0 - 1 + gy;                  // column of this thread
     const bool col_in = gact && ix >= 0 && ix < nx && iy >= 0 && iy < ny;
-    const bool pml_xy = ((PM & 4) && pml_in(ix, nx, S.pml)) || ((PM & 2) && pml_in(iy, ny, S.pml));
+    const int L = CP ? S.pml : 0;
+    const bool band_xy = L > 0 && col_in && (pml_in(ix, nx, L) || pml_in(iy, ny, L));
+    const int mask_xy = L > 0 ? (pml_in(iy, ny, L) ? kPmlMaskY : 0) |
+                                    (pml_in(ix, nx, L) ? kPmlMaskX : 0) : 0;
     const bool core = col_in && gx >= 1 && gx <= TX && gy >= 1 && gy <= TY;
     const int col = clampi(iy, 0, ny - 1) * sy + clampi(ix, 0, nx - 1);
     // LDS slot of this column in each v plane tile; idle threads write the
@@ -180,14 +180,6 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
         rx[8] = px;
         ry[8] = py;
         rz[8] = pz;
-        // CPML band: this plane's memory variables, loaded ahead of the
-        // step's prefetches
-        R ps[9];
-        bool band = false;
-        if constexpr (PML) {
-            band = pml_xy || ((PM & 1) && pml_in(zc, nz, S.pml));
-            if (band && col_in && plane_in) pml_psi_load<PM>(S, zc, iy, ix, 0, ps);
-        }
         // (1) plane zc of v into LDS: own column from the ring, frame from
         //     the delay line
         vt[0 * NV + vc] = rx[4];
@@ -259,21 +251,22 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
         const int slot = (zc + 3) % 3;
         {
             R a[6] = {0, 0, 0, 0, 0, 0};
-            if (band) {
-                // CPML band: only the owner of the cell advances its memory
-                // variables (ring columns and chunk-edge planes are also
-                // computed by neighbouring workgroups)
-                if constexpr (PML)
-                    if (col_in && plane_in)
-                        strains_col_pml<R, VX, PM>(S, bp, ps, vt + vc, vt + NV + vc, vt + 2 * NV + vc,
-                                               rx, ry, rz, ix, iy, zc,
-                                               core && zc >= z0 && zc < z1, a);
-            } else if (inner_xy && zc >= 4 && zc <= nz - 5)
+            if (inner_xy && zc >= 4 && zc <= nz - 5)
                 strains_col<false, R, VX>(S, vt + vc, vt + NV + vc, vt + 2 * NV + vc, rx, ry, rz,
                                           ix, iy, zc, a);
             else if (col_in && plane_in)
                 strains_col<true, R, VX>(S, vt + vc, vt + NV + vc, vt + 2 * NV + vc, rx, ry, rz,
                                          ix, iy, zc, a);
+            // CPML band cell: this plane's strain corrections (the components
+            // its bands correct)
+            if (CP && L > 0 && col_in && plane_in && (band_xy || pml_in(zc, nz, L))) {
+                const int mask = mask_xy | (pml_in(zc, nz, L) ? kPmlMaskZ : 0);
+                const size_t n = (size_t)nz * sz;
+                const int ig = zc * sz + col;
+#pragma unroll
+                for (int k = 0; k < 6; ++k)
+                    if (mask >> k & 1) a[k] = a[k] + ldg(S.corr + k * n, ig);
+            }
             R *gs = gt + slot * NT + tid;
 #pragma unroll
             for (int k = 0; k < 6; ++k) gs[k * 3 * NT] = a[k];
@@ -551,63 +544,19 @@ __device__ __forceinline__ void accel_col(const Shot<R> &S, const R *p1, const R
     }
 }
 
-// Accelerations of one column at a CPML band cell (see pml_d): the taps of
-// accel_col per derivative, scaled by 1/h, through pml_t (memory variables
-// 3..5 in ps); summed x, y, z per component.  Every velocity cell has one
-// owner.  M: see pml_psi_load.
-template <typename R, int VX, int M = 7>
-__device__ __forceinline__ void accel_col_pml(const Shot<R> &S, const R *bp, const R (&ps)[9],
-                                              const R *p1, const R *p2, const R *p4,
-                                              const R *p5, const R *p6, const R (&r3)[9],
-                                              const R (&r4)[9], const R (&r5)[9], int ix, int iy,
-                                              int z, R &ax, R &ay, R &az)
-{
-    const int nz = S.nz, ny = S.ny, nx = S.nx;
-    R e[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        if (j >= S.nc) break;
-        const R cj = S.c[j];
-        if (ix >= j && ix < nx - j - 1) e[0] = fma_r(cj, p1[j + 1] - p1[-j], e[0]);
-        if (ix >= j + 1 && ix < nx - j) {
-            e[3] = fma_r(cj, p6[j] - p6[-(j + 1)], e[3]);
-            e[6] = fma_r(cj, p5[j] - p5[-(j + 1)], e[6]);
-        }
-        if (iy >= j && iy < ny - j - 1) e[4] = fma_r(cj, p2[(j + 1) * VX] - p2[-j * VX], e[4]);
-        if (iy >= j + 1 && iy < ny - j) {
-            e[1] = fma_r(cj, p6[j * VX] - p6[-(j + 1) * VX], e[1]);
-            e[7] = fma_r(cj, p4[j * VX] - p4[-(j + 1) * VX], e[7]);
-        }
-        if (z >= j && z < nz - j - 1) e[8] = fma_r(cj, r3[5 + j] - r3[4 - j], e[8]);
-        if (z >= j + 1 && z < nz - j) {
-            e[2] = fma_r(cj, r5[4 + j] - r5[3 - j], e[2]);
-            e[5] = fma_r(cj, r4[4 + j] - r4[3 - j], e[5]);
-        }
-    }
-#pragma unroll
-    for (int t = 0; t < 9; ++t) e[t] = e[t] * S.inv_h;
-    ax = pml_t<2, 3, 1, M>(S, bp, z, iy, ix, e[0], ps, true) +
-         pml_t<1, 3, 0, M>(S, bp, z, iy, ix, e[1], ps, true) +
-         pml_t<0, 3, 0, M>(S, bp, z, iy, ix, e[2], ps, true);
-    ay = pml_t<2, 4, 0, M>(S, bp, z, iy, ix, e[3], ps, true) +
-         pml_t<1, 4, 1, M>(S, bp, z, iy, ix, e[4], ps, true) +
-         pml_t<0, 4, 0, M>(S, bp, z, iy, ix, e[5], ps, true);
-    az = pml_t<2, 5, 0, M>(S, bp, z, iy, ix, e[6], ps, true) +
-         pml_t<1, 5, 0, M>(S, bp, z, iy, ix, e[7], ps, true) +
-         pml_t<0, 5, 1, M>(S, bp, z, iy, ix, e[8], ps, true);
-}
-
 // RHO_LDS: the density table sits in LDS after the five plane tiles, so the
 // face-density lookups of the output stage wait on LDS only.  A global
 // lookup there would make the wave wait for every prefetch issued before it
 // (vector memory counters retire in order).
-// PM: the CPML band kernel of the tile, as in stress_tiled_body.
-template <typename R, int TX, int TY, bool RHO_LDS = false, int PM = 0>
+// CP: the shot may have a CPML.  Each velocity cell has one owner thread,
+// which for a band cell forms the acceleration corrections itself (the taps
+// of its band axes from the same LDS planes and z rings, profiles from an
+// LDS table) and advances its memory variables 3..5 in place.
+template <typename R, int TX, int TY, bool RHO_LDS = false, bool CP = false>
 __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_x, int tiles_y,
                                                     int zc_len, int t, int zlo, int zhi)
 {
     using T = VelTile<R, TX, TY>;
-    constexpr bool PML = PM != 0;
     constexpr int VX = T::VX, NV = T::NV, NT = T::NT, NF = (T::NFRAME + NT - 1) / NT;
     extern __shared__ unsigned char lds_raw[];
     R *st = reinterpret_cast<R *>(lds_raw);   // [5][VY][VX]: s1, s2, s4, s5, s6
@@ -622,9 +571,10 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
         for (int k = threadIdx.x; k < S.n_mat; k += NT) rl[k] = S.rho[k];
         rho_tab = rl;                   // ordered by the loop's first barrier
     }
-    // PML: band profiles after the density table
+    // CP: band profiles after the density table (same ordering)
     R *bp = st + 5 * NV + (RHO_LDS ? S.n_mat : 0);
-    if constexpr (PML) pml_profiles_to_lds(S, bp, NT);
+    if constexpr (CP)
+        if (S.pml > 0) pml_profiles_to_lds(S, bp, NT);
 
     const int nz = S.nz, ny = S.ny, nx = S.nx;
     const int sy = nx, sz = nx * ny;
@@ -637,7 +587,10 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
     const int lx = tid % TX, ly = tid / TX;
     const int ix = x0 + lx, iy = y0 + ly;
     const bool col_in = ix < nx && iy < ny;
-    const bool pml_xy = ((PM & 4) && pml_in(ix, nx, S.pml)) || ((PM & 2) && pml_in(iy, ny, S.pml));
+    const int L = CP ? S.pml : 0, W = 2 * L + 2;
+    const bool bx = L > 0 && col_in && pml_in(ix, nx, L);
+    const bool by = L > 0 && col_in && pml_in(iy, ny, L);
+    const int bix = ix <= L ? ix : ix - (nx - W), biy = iy <= L ? iy : iy - (ny - W);
     const int col = clampi(iy, 0, ny - 1) * sy + clampi(ix, 0, nx - 1);
     const int vc = (ly + 4) * VX + (lx + 4);
 
@@ -737,13 +690,31 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
         r3[8] = p3;
         r4[8] = p4;
         r5[8] = p5;
-        // CPML band: this plane's memory variables, loaded ahead of the
-        // step's prefetches
+        // CPML band cell: memory variables 3..5 of its band axes, loaded
+        // ahead of the step's prefetches
+        const bool bz = CP && L > 0 && col_in && pml_in(z, nz, L);
         R ps[9];
-        bool band = false;
-        if constexpr (PML) {
-            band = pml_xy || ((PM & 1) && pml_in(z, nz, S.pml));
-            if (band && col_in) pml_psi_load<PM>(S, z, iy, ix, 3, ps);
+        size_t fz = 0, fy = 0, fx = 0, vz_ = 0, vy_ = 0, vx_ = 0;
+        if constexpr (CP) {
+            if (bz) {
+                const int biz = z <= L ? z : z - (nz - W);
+                vz_ = (size_t)W * ny * nx;
+                fz = ((size_t)biz * ny + iy) * nx + ix;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) ps[k] = S.psi[0][(3 + k) * vz_ + fz];
+            }
+            if (by) {
+                vy_ = (size_t)nz * W * nx;
+                fy = ((size_t)z * W + biy) * nx + ix;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) ps[3 + k] = S.psi[1][(3 + k) * vy_ + fy];
+            }
+            if (bx) {
+                vx_ = (size_t)nz * ny * W;
+                fx = ((size_t)z * ny + iy) * W + bix;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) ps[6 + k] = S.psi[2][(3 + k) * vx_ + fx];
+            }
         }
         // (1) plane z of s1, s2, s4, s5, s6 into LDS
         t1[vc] = c1;
@@ -772,20 +743,80 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
 
         if (col_in) {
             R ax = 0, ay = 0, az = 0;
-            // a CPML band cell's accelerations are in units of 1/h already
-            R tstep = S.dt_h;
-            if (band) {
-                if constexpr (PML) {
-                    accel_col_pml<R, VX, PM>(S, bp, ps, t1 + vc, t2 + vc, t4 + vc, t5 + vc, t6 + vc,
-                                         r3, r4, r5, ix, iy, z, ax, ay, az);
-                    tstep = S.dt;
-                }
-            } else if (inner_xy && z >= 4 && z <= nz - 5)
+            if (inner_xy && z >= 4 && z <= nz - 5)
                 accel_col<false, R, VX>(S, t1 + vc, t2 + vc, t4 + vc, t5 + vc, t6 + vc, r3, r4, r5,
                                         ix, iy, z, ax, ay, az);
             else
                 accel_col<true, R, VX>(S, t1 + vc, t2 + vc, t4 + vc, t5 + vc, t6 + vc, r3, r4, r5,
                                        ix, iy, z, ax, ay, az);
+            if constexpr (CP) {
+                if (bz || by || bx) {
+                    // corrections of the band axes z, y, x (pml_accel_corr)
+                    const R *p1 = t1 + vc, *p2 = t2 + vc, *p4 = t4 + vc, *p5 = t5 + vc,
+                            *p6 = t6 + vc;
+                    R g0 = 0, g1 = 0, g2 = 0;
+                    if (bz) {
+                        const int biz = z <= L ? z : z - (nz - W);
+                        R e0 = 0, e1 = 0, e2 = 0;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            if (j >= S.nc) break;
+                            const R cj = S.c[j];
+                            if (z >= j + 1 && z < nz - j) {
+                                e0 = fma_r(cj, r5[4 + j] - r5[3 - j], e0);
+                                e1 = fma_r(cj, r4[4 + j] - r4[3 - j], e1);
+                            }
+                            if (z >= j && z < nz - j - 1) e2 = fma_r(cj, r3[5 + j] - r3[4 - j], e2);
+                        }
+                        g0 = g0 + pml_corr_lds<0, 0>(bp, W, biz, ps[0], e0);
+                        g1 = g1 + pml_corr_lds<0, 0>(bp, W, biz, ps[1], e1);
+                        g2 = g2 + pml_corr_lds<0, 1>(bp, W, biz, ps[2], e2);
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) S.psi[0][(3 + k) * vz_ + fz] = ps[k];
+                    }
+                    if (by) {
+                        R e0 = 0, e1 = 0, e2 = 0;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            if (j >= S.nc) break;
+                            const R cj = S.c[j];
+                            if (iy >= j + 1 && iy < ny - j) {
+                                e0 = fma_r(cj, p6[j * VX] - p6[-(j + 1) * VX], e0);
+                                e2 = fma_r(cj, p4[j * VX] - p4[-(j + 1) * VX], e2);
+                            }
+                            if (iy >= j && iy < ny - j - 1)
+                                e1 = fma_r(cj, p2[(j + 1) * VX] - p2[-j * VX], e1);
+                        }
+                        g0 = g0 + pml_corr_lds<1, 0>(bp, W, biy, ps[3], e0);
+                        g1 = g1 + pml_corr_lds<1, 1>(bp, W, biy, ps[4], e1);
+                        g2 = g2 + pml_corr_lds<1, 0>(bp, W, biy, ps[5], e2);
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) S.psi[1][(3 + k) * vy_ + fy] = ps[3 + k];
+                    }
+                    if (bx) {
+                        R e0 = 0, e1 = 0, e2 = 0;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            if (j >= S.nc) break;
+                            const R cj = S.c[j];
+                            if (ix >= j && ix < nx - j - 1) e0 = fma_r(cj, p1[j + 1] - p1[-j], e0);
+                            if (ix >= j + 1 && ix < nx - j) {
+                                e1 = fma_r(cj, p6[j] - p6[-(j + 1)], e1);
+                                e2 = fma_r(cj, p5[j] - p5[-(j + 1)], e2);
+                            }
+                        }
+                        g0 = g0 + pml_corr_lds<2, 1>(bp, W, bix, ps[6], e0);
+                        g1 = g1 + pml_corr_lds<2, 0>(bp, W, bix, ps[7], e1);
+                        g2 = g2 + pml_corr_lds<2, 0>(bp, W, bix, ps[8], e2);
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) S.psi[2][(3 + k) * vx_ + fx] = ps[6 + k];
+                    }
+                    ax = ax + g0;
+                    ay = ay + g1;
+                    az = az + g2;
+                }
+            }
+            const R tstep = S.dt_h;
             const R *rho = rho_tab;
             const R r0 = rho[m0];
             const R rvx = ix < nx - 1 ? (R)0.5 * (r0 + rho[mxp]) : r0;
@@ -802,81 +833,36 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
     // its load (measured 0.481 -> 0.457 ms at 400x400x200, 64x8 tiles; the
     // kernel stays within the registers of its occupancy)
     int z = z0;
-    for (; z + 1 < z1; z += 2) {
-        step(z);
-        step(z + 1);
+    // (with a CPML only the 4-wave tiles, whose register bound is twice the
+    // 8-wave tiles', have room for the second step of the unrolled loop)
+    if constexpr (!CP || TX * TY <= 256) {
+        for (; z + 1 < z1; z += 2) {
+            step(z);
+            step(z + 1);
+        }
     }
-    if (z < z1) step(z);
+    for (; z < z1; ++z) step(z);
 }
 
 // ------------------------------------------------------------- launchers
-// Single-shot kernels.  With a CPML boundary each tile is given to one of
-// three kernels by the bands it meets: 0 (none: the plain kernel), 1 (a z
-// band only) and 7 (an x or y band, maybe also a z band); the template
-// argument PM is the mask of axes whose memory variables the kernel compiles
-// (bit 0 z, bit 1 y, bit 2 x).  Each kernel exits for the tiles of the
-// others, so launches of the same geometry over the same planes split the
-// tiles between them.  Kernel 7 may use twice the registers (its band path
-// needs them); kernel 1 keeps the plain kernel's register bound, and so its
-// occupancy.  (Kernels for the x and y bands apart, 3 and 5, were measured
-// slower: their launches run too few tiles to fill the device.)
-inline __device__ int band_kernel(bool x, bool y, bool z)
-{
-    return x || y ? 7 : (z ? 1 : 0);
-}
-
-template <typename R, int TX, int TY>
-__device__ __forceinline__ int stress_band_kernel(const Shot<R> &S, int t, int tiles_x,
-                                                  int tiles_y, int zc_len, int zlo, int zhi)
-{
-    // strain columns x0-1 .. x0+TX, y0-1 .. y0+TY; planes z0-1 .. z1
-    const int L = S.pml;
-    const int x0 = (t % tiles_x) * TX, y0 = ((t / tiles_x) % tiles_y) * TY;
-    const int z0 = zlo + (t / (tiles_x * tiles_y)) * zc_len, z1 = min(zhi, z0 + zc_len);
-    return band_kernel(x0 - 1 <= L || x0 + TX >= S.nx - 1 - L,
-                       y0 - 1 <= L || y0 + TY >= S.ny - 1 - L, z0 - 1 <= L || z1 >= S.nz - 1 - L);
-}
-
-template <typename R, int TX, int TY>
-__device__ __forceinline__ int velocity_band_kernel(const Shot<R> &S, int t, int tiles_x,
-                                                    int tiles_y, int zc_len, int zlo, int zhi)
-{
-    const int L = S.pml;
-    const int x0 = (t % tiles_x) * TX, y0 = ((t / tiles_x) % tiles_y) * TY;
-    const int z0 = zlo + (t / (tiles_x * tiles_y)) * zc_len, z1 = min(zhi, z0 + zc_len);
-    return band_kernel(x0 <= L || x0 + TX - 1 >= S.nx - 1 - L,
-                       y0 <= L || y0 + TY - 1 >= S.ny - 1 - L, z0 <= L || z1 - 1 >= S.nz - 1 - L);
-}
-
-// Register bound of band kernel PM: the plain kernel's (MinWaves) but for
-// kernel 7.
-template <typename R, int NT, int PM> struct ClassWaves {
-    static constexpr int v = PM == 7 ? 1 : MinWaves<R, NT>::v;
-};
-
-template <typename R, int TX, int TY, int PAD, bool STORE_G = false, bool FR = false, int PM = 0>
+template <typename R, int TX, int TY, int PAD, bool STORE_G = false, bool FR = false,
+          bool CP = false>
 __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
-                             (ClassWaves<R, StressTile<R, TX, TY, PAD>::NT, PM>::v)) void
+                             (MinWaves<R, StressTile<R, TX, TY, PAD>::NT>::v)) void
 k_stress_tiled(Shot<R> S, int tiles_x, int tiles_y, int zc_len, int zlo, int zhi)
 {
     const int nb = tiles_x * tiles_y * ((zhi - zlo + zc_len - 1) / zc_len);
     const int t = xcd_remap(blockIdx.x, nb);
-    if (S.pml > 0 &&
-        stress_band_kernel<R, TX, TY>(S, t, tiles_x, tiles_y, zc_len, zlo, zhi) != PM)
-        return;
-    stress_tiled_body<R, TX, TY, PAD, STORE_G, FR, PM>(S, tiles_x, tiles_y, zc_len, t, zlo, zhi);
+    stress_tiled_body<R, TX, TY, PAD, STORE_G, FR, CP>(S, tiles_x, tiles_y, zc_len, t, zlo, zhi);
 }
 
-template <typename R, int TX, int TY, bool RHO_LDS = false, int PM = 0>
-__global__ __launch_bounds__(TX *TY, (ClassWaves<R, TX * TY, PM>::v)) void k_velocity_tiled(
+template <typename R, int TX, int TY, bool RHO_LDS = false, bool CP = false>
+__global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_velocity_tiled(
     Shot<R> S, int tiles_x, int tiles_y, int zc_len, int zlo, int zhi)
 {
     const int nb = tiles_x * tiles_y * ((zhi - zlo + zc_len - 1) / zc_len);
     const int t = xcd_remap(blockIdx.x, nb);
-    if (S.pml > 0 &&
-        velocity_band_kernel<R, TX, TY>(S, t, tiles_x, tiles_y, zc_len, zlo, zhi) != PM)
-        return;
-    velocity_tiled_body<R, TX, TY, RHO_LDS, PM>(S, tiles_x, tiles_y, zc_len, t, zlo, zhi);
+    velocity_tiled_body<R, TX, TY, RHO_LDS, CP>(S, tiles_x, tiles_y, zc_len, t, zlo, zhi);
 }
 
 template <typename R, int TX, int TY>
@@ -888,37 +874,118 @@ k_stress_pipe(Shot<R> S, int tiles_x, int tiles_y, int zc_len, int zlo, int zhi)
     stress_pipe_body<R, TX, TY>(S, tiles_x, tiles_y, zc_len, xcd_remap(blockIdx.x, nb), zlo, zhi);
 }
 
+// CPML band pre-pass: one thread per band cell (pml_band_cell order), which
+// advances the cell's memory variables and writes its corrections into
+// S.corr -- HALF 0 the six strain corrections (from v, before the stress
+// kernel), HALF 1 the three acceleration corrections (from s, before the
+// velocity kernel).  Each memory variable has one owner, so it is advanced
+// in place.
+// pml_strain_corr with every load of an axis issued before its arithmetic:
+// the taps at clamped indices (a tap outside the grid only feeds a term the
+// edge guards drop), the memory variables and the profile values, so that a
+// band cell waits for its loads once per axis.  Same arithmetic.
+template <int A, typename R>
+__device__ __forceinline__ void pml_strain_axis(const Shot<R> &S, int i, int iz, int iy, int ix,
+                                                R (&ga)[6])
+{
+    constexpr int K0 = A == 0 ? 2 : (A == 1 ? 1 : 0);     // Db component
+    constexpr int K1 = A == 0 ? 3 : (A == 1 ? 3 : 4);     // first Df component
+    constexpr int K2 = A == 0 ? 4 : 5;                    // second Df component
+    const int n = A == 0 ? S.nz : (A == 1 ? S.ny : S.nx);
+    const int ia = A == 0 ? iz : (A == 1 ? iy : ix);
+    const int st = A == 0 ? S.nx * S.ny : (A == 1 ? S.nx : 1);
+    // Db on the component along A, Df on the other two (z: vz | vy, vx;
+    // y: vy | vz, vx; x: vx | vz, vy)
+    const R *__restrict__ fb = S.v[A == 0 ? 2 : (A == 1 ? 1 : 0)];
+    const R *__restrict__ f1 = S.v[A == 2 ? 2 : (A == 1 ? 2 : 1)];
+    const R *__restrict__ f2 = S.v[A == 2 ? 1 : 0];
+    R tb[8], t1[8], t2[8];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        // Db taps ia+j, ia-(j+1); Df taps ia+j+1, ia-j
+        const int pb = min(max(ia + j, 0), n - 1), mb = min(max(ia - (j + 1), 0), n - 1);
+        const int pf = min(max(ia + j + 1, 0), n - 1), mf = min(max(ia - j, 0), n - 1);
+        tb[2 * j] = fb[i + (pb - ia) * st];
+        tb[2 * j + 1] = fb[i + (mb - ia) * st];
+        t1[2 * j] = f1[i + (pf - ia) * st];
+        t1[2 * j + 1] = f1[i + (mf - ia) * st];
+        t2[2 * j] = f2[i + (pf - ia) * st];
+        t2[2 * j + 1] = f2[i + (mf - ia) * st];
+    }
+    R d0 = 0, d1 = 0, d2 = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (j >= S.nc) break;
+        const R cj = S.c[j];
+        if (ia >= j + 1 && ia < n - j) d0 = fma_r(cj, tb[2 * j] - tb[2 * j + 1], d0);
+        if (ia >= j && ia < n - j - 1) {
+            d1 = fma_r(cj, t1[2 * j] - t1[2 * j + 1], d1);
+            d2 = fma_r(cj, t2[2 * j] - t2[2 * j + 1], d2);
+        }
+    }
+    ga[K0] = ga[K0] + pml_corr<A, 0, 0>(S, iz, iy, ix, d0 * S.inv_h);
+    ga[K1] = ga[K1] + pml_corr<A, 1, 1>(S, iz, iy, ix, d1 * S.inv_h);
+    ga[K2] = ga[K2] + pml_corr<A, 2, 1>(S, iz, iy, ix, d2 * S.inv_h);
+}
+
+template <typename R>
+__device__ __forceinline__ void pml_strain_corr_hoisted(const Shot<R> &S, int i, int iz, int iy,
+                                                        int ix, R (&ga)[6])
+{
+    const int L = S.pml;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) ga[k] = 0;
+    if (pml_in(iz, S.nz, L)) pml_strain_axis<0>(S, i, iz, iy, ix, ga);
+    if (pml_in(iy, S.ny, L)) pml_strain_axis<1>(S, i, iz, iy, ix, ga);
+    if (pml_in(ix, S.nx, L)) pml_strain_axis<2>(S, i, iz, iy, ix, ga);
+}
+
+template <typename R, int HALF>
+__device__ __forceinline__ void pml_corr_body(const Shot<R> &S, int b)
+{
+    if (b >= pml_band_cells(S.nz, S.ny, S.nx, S.pml)) return;
+    int iz, iy, ix;
+    pml_band_cell(b, S.nz, S.ny, S.nx, S.pml, iz, iy, ix);
+    const int sz = S.nx * S.ny;
+    const int i = iz * sz + iy * S.nx + ix;
+    const size_t n = (size_t)S.nz * sz;
+    if constexpr (HALF == 0) {
+        R ga[6];
+        pml_strain_corr_hoisted(S, i, iz, iy, ix, ga);
+        const int mask = pml_strain_mask(S, iz, iy, ix);
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            if (mask >> k & 1) S.corr[k * n + i] = ga[k];
+    } else {
+        R gv[3];
+        pml_accel_corr(S, i, iz, iy, ix, gv);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) S.corr[k * n + i] = gv[k];
+    }
+}
+
+template <typename R, int HALF>
+__global__ __launch_bounds__(kPmlBlock) void k_pml_corr(Shot<R> S)
+{
+    pml_corr_body<R, HALF>(S, blockIdx.x * kPmlBlock + threadIdx.x);
+}
+
 // Multi-shot kernels: the workgroups of every shot of a batch in one launch.
 // blk_shot maps a (remapped) workgroup id to its shot; a shot whose step
-// counter has reached its step count exits at once.  A batch with CPML shots
-// runs the band kernels PM = 0, 1, 7 one after another over every
-// workgroup; a tile of a CPML shot runs in the kernel of its band class
-// (stress_band_kernel) and a sponge shot's tiles in kernel 0.  The memory
-// variables' buffer parity is that of the shot's own step counter (the entry
-// holds the shot once per parity, so the kernel reads it in place: a
-// modified local copy would be kept in scratch memory).
-template <typename R, int TX, int TY, int PAD, bool FR = false, int PM = 0>
+// counter has reached its step count exits at once.
+template <typename R, int TX, int TY, int PAD, bool FR = false, bool CP = false>
 __global__ __launch_bounds__((StressTile<R, TX, TY, PAD>::NT),
-                             (ClassWaves<R, StressTile<R, TX, TY, PAD>::NT, PM>::v)) void
+                             (MinWaves<R, StressTile<R, TX, TY, PAD>::NT>::v)) void
 k_stress_batched(const BatchEntry<R> *__restrict__ tab, const int32_t *__restrict__ blk_shot,
                  int nb_total, const int32_t *__restrict__ step)
 {
     const int g = xcd_remap(blockIdx.x, nb_total);
     const int s = blk_shot[g];
     const BatchEntry<R> *e = tab + s;
-    const int n = step[s];
-    if (n >= e->nt) return;
-    const Shot<R> &S = (PM != 0 && (n & 1)) ? e->S_odd : e->S;
-    const int t = g - e->blk_off[0];
-    if (S.pml > 0) {
-        if (stress_band_kernel<R, TX, TY>(S, t, e->tiles_x[0], e->tiles_y[0], e->zc_len[0], 0,
-                                          S.nz) != PM)
-            return;
-    } else if (PM != 0) {
-        return;
-    }
-    stress_tiled_body<R, TX, TY, PAD, false, FR, PM>(S, e->tiles_x[0], e->tiles_y[0],
-                                                     e->zc_len[0], t, 0, S.nz);
+    if (step[s] >= e->nt) return;
+    const Shot<R> &S = e->S;
+    stress_tiled_body<R, TX, TY, PAD, false, FR, CP>(S, e->tiles_x[0], e->tiles_y[0],
+                                                     e->zc_len[0], g - e->blk_off[0], 0, S.nz);
 }
 
 template <typename R, int TX, int TY>
@@ -936,125 +1003,75 @@ k_stress_pipe_batched(const BatchEntry<R> *__restrict__ tab, const int32_t *__re
                                 0, S.nz);
 }
 
-template <typename R, int TX, int TY, bool RHO_LDS = false, int PM = 0>
-__global__ __launch_bounds__(TX *TY, (ClassWaves<R, TX * TY, PM>::v)) void k_velocity_batched(
+template <typename R, int TX, int TY, bool RHO_LDS = false, bool CP = false>
+__global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_velocity_batched(
     const BatchEntry<R> *__restrict__ tab, const int32_t *__restrict__ blk_shot, int nb_total,
     const int32_t *__restrict__ step)
 {
     const int g = xcd_remap(blockIdx.x, nb_total);
     const int s = blk_shot[g];
     const BatchEntry<R> *e = tab + s;
-    const int n = step[s];
-    if (n >= e->nt) return;
-    const Shot<R> &S = (PM != 0 && (n & 1)) ? e->S_odd : e->S;
-    const int t = g - e->blk_off[1];
-    if (S.pml > 0) {
-        if (velocity_band_kernel<R, TX, TY>(S, t, e->tiles_x[1], e->tiles_y[1], e->zc_len[1], 0,
-                                            S.nz) != PM)
-            return;
-    } else if (PM != 0) {
-        return;
-    }
-    velocity_tiled_body<R, TX, TY, RHO_LDS, PM>(S, e->tiles_x[1], e->tiles_y[1], e->zc_len[1], t,
-                                                0, S.nz);
+    if (step[s] >= e->nt) return;
+    const Shot<R> &S = e->S;
+    velocity_tiled_body<R, TX, TY, RHO_LDS, CP>(S, e->tiles_x[1], e->tiles_y[1], e->zc_len[1],
+                                                g - e->blk_off[1], 0, S.nz);
 }
 
-// A second stream per device (default priority, created once) on which a
-// CPML half-step runs its full band kernel (tiles in an x or y band) while
-// the plain and z-band launches run on the caller's stream; the sets of
-// tiles write disjoint cells.  Calls that
-// overlap on one device share it: a caller holds `use` from the fork record
-// to the join wait, so host threads enqueueing at the same time cannot
-// interleave their fork/join records (the lock covers enqueueing only).
-struct SideStream {
-    hipStream_t s = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
-    std::mutex use;
-};
-
-inline hipError_t side_stream(SideStream **out)
+// Band pre-pass of a batch: blk_shot maps a block to its CPML shot, whose
+// blocks start at blk_off[2].
+template <typename R, int HALF>
+__global__ __launch_bounds__(kPmlBlock) void k_pml_corr_batched(
+    const BatchEntry<R> *__restrict__ tab, const int32_t *__restrict__ blk_shot,
+    const int32_t *__restrict__ step)
 {
-    static std::mutex mu;
-    static SideStream per_dev[64];
-    int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e != hipSuccess) return e;
-    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
-    std::lock_guard<std::mutex> lock(mu);
-    SideStream &ss = per_dev[dev];
-    if (!ss.s) {
-        e = hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&ss.join, hipEventDisableTiming);
-        if (e != hipSuccess) return e;
-    }
-    *out = &ss;
-    return hipSuccess;
+    const int s = blk_shot[blockIdx.x];
+    const BatchEntry<R> *e = tab + s;
+    if (step[s] >= e->nt) return;
+    pml_corr_body<R, HALF>(e->S, (blockIdx.x - e->blk_off[2]) * kPmlBlock + threadIdx.x);
 }
 
-// CPML z ranges: the middle planes [m0, m1), whose tiles lie in a band only
-// through x or y, and the two slabs that contain the z bands.  A stress tile
-// of planes z0..z1-1 computes strains on z0-1 .. z1, so its middle range is
-// [L+2, nz-L-2); a velocity tile touches only its own planes: [L+1, nz-L-1).
-inline void pml_zranges(int nz, int L, int half, int &m0, int &m1)
+template <typename R>
+hipError_t pml_corr_launch(const Shot<R> &s, int half, hipStream_t st)
 {
-    m0 = half == 0 ? L + 2 : L + 1;
-    m1 = half == 0 ? nz - L - 2 : nz - L - 1;
-    if (m1 <= m0) m0 = m1 = nz;                  // no band-free middle
+    const long long cells = pml_band_cells(s.nz, s.ny, s.nx, s.pml);
+    const int nb = (int)((cells + kPmlBlock - 1) / kPmlBlock);
+    if (half == 0)
+        hipLaunchKernelGGL((k_pml_corr<R, 0>), dim3(nb), dim3(kPmlBlock), 0, st, s);
+    else
+        hipLaunchKernelGGL((k_pml_corr<R, 1>), dim3(nb), dim3(kPmlBlock), 0, st, s);
+    return hipGetLastError();
 }
 
-template <typename R, int TX, int TY, int PAD, bool FR = false, bool PML = false>
+template <typename R, int TX, int TY, int PAD, bool FR = false>
 hipError_t stress_launch(const Shot<R> &s, int zchunk, hipStream_t st, bool store_g = false)
 {
     using T = StressTile<R, TX, TY, PAD, FR>;
     const int tiles_x = (s.nx + TX - 1) / TX, tiles_y = (s.ny + TY - 1) / TY;
-    auto run = [&](auto kern, size_t lds, int lo, int hi) -> hipError_t {
-        if (hi <= lo) return hipSuccess;
-        const int zl = chunk_len(hi - lo, tiles_x * tiles_y, zchunk);
-        const int nb = tiles_x * tiles_y * ((hi - lo + zl - 1) / zl);
-        hipError_t e = allow_lds(kern, lds);
+    auto run = [&](auto kern) -> hipError_t {
+        const int zl = chunk_len(s.nz, tiles_x * tiles_y, zchunk);
+        const int nb = tiles_x * tiles_y * ((s.nz + zl - 1) / zl);
+        hipError_t e = allow_lds(kern, T::LDS);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(kern, dim3(nb), dim3(T::NT), lds, st, s, tiles_x, tiles_y, zl, lo, hi);
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(T::NT), T::LDS, st, s, tiles_x, tiles_y, zl, 0,
+                           s.nz);
         return hipGetLastError();
     };
-    if constexpr (PML) {
+    if (s.pml > 0) {
         if (store_g) return hipErrorInvalidValue;   // the strain history serves the adjoint only
-        const size_t lds_pml = T::LDS + sizeof(R) * 18 * (2 * (size_t)s.pml + 2);
-        int m0, m1;
-        pml_zranges(s.nz, s.pml, 0, m0, m1);
-        // the caller's stream takes the band-free middle tiles (kernel 0)
-        // and then the z-only tiles of the two slabs that hold the z bands
-        // (kernel 1); concurrently the side stream takes the tiles in an x
-        // or y band (kernel 7) in one launch over every plane
-        const int lo_end = m0 < m1 ? m0 : s.nz;
-        SideStream *ss = nullptr;
-        hipError_t e = side_stream(&ss);
+        hipError_t e = pml_corr_launch(s, 0, st);
         if (e != hipSuccess) return e;
-        std::lock_guard<std::mutex> hold(ss->use);
-        e = hipEventRecord(ss->fork, st);
-        if (e == hipSuccess) e = hipStreamWaitEvent(ss->s, ss->fork, 0);
-        if (e == hipSuccess) e = run(k_stress_tiled<R, TX, TY, PAD, false, FR, 0>, T::LDS, m0, m1);
-        if (e == hipSuccess) e = run(k_stress_tiled<R, TX, TY, PAD, false, FR, 1>, lds_pml, 0, lo_end);
-        if (e == hipSuccess && m0 < m1)
-            e = run(k_stress_tiled<R, TX, TY, PAD, false, FR, 1>, lds_pml, m1, s.nz);
-        const hipStream_t main = st;
-        st = ss->s;
-        if (e == hipSuccess) e = run(k_stress_tiled<R, TX, TY, PAD, false, FR, 7>, lds_pml, 0, s.nz);
-        st = main;
-        if (e == hipSuccess) e = hipEventRecord(ss->join, ss->s);
-        if (e == hipSuccess) e = hipStreamWaitEvent(st, ss->join, 0);
-        return e;
+        return run(k_stress_tiled<R, TX, TY, PAD, false, FR, true>);
     }
-    if (store_g) return run(k_stress_tiled<R, TX, TY, PAD, true, FR>, T::LDS, 0, s.nz);
-    return run(k_stress_tiled<R, TX, TY, PAD, false, FR>, T::LDS, 0, s.nz);
+    if (store_g) return run(k_stress_tiled<R, TX, TY, PAD, true, FR>);
+    return run(k_stress_tiled<R, TX, TY, PAD, false, FR>);
 }
 
-// Pipelined stress tile (stress_pipe_body); the strain-history store runs
-// the plain kernel of the same tile.
+// Pipelined stress tile (stress_pipe_body); the strain-history store and
+// CPML shots run the plain kernel of the same tile.
 template <typename R, int TX, int TY>
 hipError_t stress_pipe_launch(const Shot<R> &s, int zchunk, hipStream_t st, bool store_g)
 {
-    if (store_g) return stress_launch<R, TX, TY, 0>(s, zchunk, st, true);
+    if (store_g || s.pml > 0) return stress_launch<R, TX, TY, 0>(s, zchunk, st, store_g);
     using T = StressTile<R, TX, TY, 0>;
     const int tiles_x = (s.nx + TX - 1) / TX, tiles_y = (s.ny + TY - 1) / TY;
     const int zl = chunk_len(s.nz, tiles_x * tiles_y, zchunk);
@@ -1067,68 +1084,46 @@ hipError_t stress_pipe_launch(const Shot<R> &s, int zchunk, hipStream_t st, bool
     return hipGetLastError();
 }
 
-template <typename R, int TX, int TY, bool PML = false>
+template <typename R, int TX, int TY>
 hipError_t velocity_launch(const Shot<R> &s, int zchunk, hipStream_t st)
 {
     using T = VelTile<R, TX, TY>;
     const int tiles_x = (s.nx + TX - 1) / TX, tiles_y = (s.ny + TY - 1) / TY;
-    auto run = [&](auto kern, size_t lds, int lo, int hi) -> hipError_t {
-        if (hi <= lo) return hipSuccess;
-        const int zl = chunk_len(hi - lo, tiles_x * tiles_y, zchunk);
-        const int nb = tiles_x * tiles_y * ((hi - lo + zl - 1) / zl);
-        hipError_t e = allow_lds(kern, lds);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(kern, dim3(nb), dim3(T::NT), lds, st, s, tiles_x, tiles_y, zl, lo, hi);
-        return hipGetLastError();
-    };
     const size_t rho_bytes = sizeof(R) * (size_t)s.n_mat;
     const bool rho_lds = rho_bytes <= (size_t)kRhoLdsBytes;
-    const size_t lds = T::LDS + (rho_lds ? rho_bytes : 0);
-    if constexpr (PML) {
-        const size_t lds_pml = lds + sizeof(R) * 18 * (2 * (size_t)s.pml + 2);
-        int m0, m1;
-        pml_zranges(s.nz, s.pml, 1, m0, m1);
-        const int lo_end = m0 < m1 ? m0 : s.nz;
-        // the launch sequence of stress_launch: kernels 0 and 1 on the
-        // caller's stream, kernel 7 on the side stream
-        auto seq = [&](auto k0, auto k1, auto k7) -> hipError_t {
-            SideStream *ss = nullptr;
-            hipError_t e = side_stream(&ss);
-            if (e != hipSuccess) return e;
-            std::lock_guard<std::mutex> hold(ss->use);
-            e = hipEventRecord(ss->fork, st);
-            if (e == hipSuccess) e = hipStreamWaitEvent(ss->s, ss->fork, 0);
-            if (e == hipSuccess) e = run(k0, lds, m0, m1);
-            if (e == hipSuccess) e = run(k1, lds_pml, 0, lo_end);
-            if (e == hipSuccess && m0 < m1) e = run(k1, lds_pml, m1, s.nz);
-            const hipStream_t main = st;
-            st = ss->s;
-            if (e == hipSuccess) e = run(k7, lds_pml, 0, s.nz);
-            st = main;
-            if (e == hipSuccess) e = hipEventRecord(ss->join, ss->s);
-            if (e == hipSuccess) e = hipStreamWaitEvent(st, ss->join, 0);
-            return e;
-        };
-        if (rho_lds)
-            return seq(k_velocity_tiled<R, TX, TY, true, 0>, k_velocity_tiled<R, TX, TY, true, 1>,
-                       k_velocity_tiled<R, TX, TY, true, 7>);
-        return seq(k_velocity_tiled<R, TX, TY, false, 0>, k_velocity_tiled<R, TX, TY, false, 1>,
-                   k_velocity_tiled<R, TX, TY, false, 7>);
+    // CPML: the band profile table after the densities
+    const size_t lds = T::LDS + (rho_lds ? rho_bytes : 0) +
+                       (s.pml > 0 ? sizeof(R) * 18 * (2 * (size_t)s.pml + 2) : 0);
+    auto run = [&](auto kern) -> hipError_t {
+        const int zl = chunk_len(s.nz, tiles_x * tiles_y, zchunk);
+        const int nb = tiles_x * tiles_y * ((s.nz + zl - 1) / zl);
+        hipError_t e = allow_lds(kern, lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(T::NT), lds, st, s, tiles_x, tiles_y, zl, 0,
+                           s.nz);
+        return hipGetLastError();
+    };
+    if (s.pml > 0) {
+        if (rho_lds) return run(k_velocity_tiled<R, TX, TY, true, true>);
+        return run(k_velocity_tiled<R, TX, TY, false, true>);
     }
-    if (rho_lds) return run(k_velocity_tiled<R, TX, TY, true>, lds, 0, s.nz);
-    return run(k_velocity_tiled<R, TX, TY>, lds, 0, s.nz);
+    if (rho_lds) return run(k_velocity_tiled<R, TX, TY, true>);
+    return run(k_velocity_tiled<R, TX, TY>);
 }
 
 template <typename R, int TX, int TY, int PAD, bool FR = false>
 hipError_t stress_batched_launch(const BatchEntry<R> *tab, const int32_t *blk, int nb,
-                                 const int32_t *step, hipStream_t st)
+                                 const int32_t *step, hipStream_t st, bool cp)
 {
     using T = StressTile<R, TX, TY, PAD, FR>;
-    hipError_t e = allow_lds(k_stress_batched<R, TX, TY, PAD, FR>, T::LDS);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_stress_batched<R, TX, TY, PAD, FR>), dim3(nb), dim3(T::NT), T::LDS, st,
-                       tab, blk, nb, step);
-    return hipGetLastError();
+    auto go = [&](auto kern) -> hipError_t {
+        hipError_t e = allow_lds(kern, T::LDS);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(T::NT), T::LDS, st, tab, blk, nb, step);
+        return hipGetLastError();
+    };
+    return cp ? go(k_stress_batched<R, TX, TY, PAD, FR, true>)
+              : go(k_stress_batched<R, TX, TY, PAD, FR, false>);
 }
 
 template <typename R, int TX, int TY>
@@ -1144,88 +1139,29 @@ hipError_t stress_pipe_batched_launch(const BatchEntry<R> *tab, const int32_t *b
     return hipGetLastError();
 }
 
+// max_pml: the widest CPML band of the batch (sizes the LDS profile table;
+// 0 = no CPML shot).
 template <typename R, int TX, int TY>
 hipError_t velocity_batched_launch(const BatchEntry<R> *tab, const int32_t *blk, int nb,
-                                   const int32_t *step, hipStream_t st, int max_mat)
-{
-    using T = VelTile<R, TX, TY>;
-    const size_t rho_bytes = sizeof(R) * (size_t)max_mat;
-    if (rho_bytes <= (size_t)kRhoLdsBytes) {
-        const size_t lds = T::LDS + rho_bytes;
-        hipError_t e = allow_lds(k_velocity_batched<R, TX, TY, true>, lds);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_velocity_batched<R, TX, TY, true>), dim3(nb), dim3(T::NT), lds, st,
-                           tab, blk, nb, step);
-        return hipGetLastError();
-    }
-    hipError_t e = allow_lds(k_velocity_batched<R, TX, TY>, T::LDS);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_velocity_batched<R, TX, TY>), dim3(nb), dim3(T::NT), T::LDS, st, tab,
-                       blk, nb, step);
-    return hipGetLastError();
-}
-
-// CPML batches: the three band kernels over every workgroup, variant-0
-// tiles; max_pml sizes the LDS band-profile table for the widest band.  As
-// for a single shot, kernels 0 and 1 run on the caller's stream and kernel 7
-// concurrently on the side stream (fork / join by events, which a graph
-// capture of the caller's stream follows).
-template <typename R, int NT, typename K0, typename K1, typename K7>
-hipError_t batched_pml_fork(K0 k0, K1 k1, K7 k7, size_t lds, size_t lds_pml,
-                            const BatchEntry<R> *tab, const int32_t *blk, int nb,
-                            const int32_t *step, hipStream_t st)
-{
-    auto go = [&](auto kern, size_t l, hipStream_t q) -> hipError_t {
-        hipError_t e = allow_lds(kern, l);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(kern, dim3(nb), dim3(NT), l, q, tab, blk, nb, step);
-        return hipGetLastError();
-    };
-    SideStream *ss = nullptr;
-    hipError_t e = side_stream(&ss);
-    if (e != hipSuccess) return e;
-    std::lock_guard<std::mutex> hold(ss->use);
-    e = hipEventRecord(ss->fork, st);
-    if (e == hipSuccess) e = hipStreamWaitEvent(ss->s, ss->fork, 0);
-    if (e == hipSuccess) e = go(k0, lds, st);
-    if (e == hipSuccess) e = go(k1, lds_pml, st);
-    if (e == hipSuccess) e = go(k7, lds_pml, ss->s);
-    if (e == hipSuccess) e = hipEventRecord(ss->join, ss->s);
-    if (e == hipSuccess) e = hipStreamWaitEvent(st, ss->join, 0);
-    return e;
-}
-
-template <typename R, int TX, int TY, int PAD>
-hipError_t stress_batched_pml_launch(const BatchEntry<R> *tab, const int32_t *blk, int nb,
-                                     const int32_t *step, hipStream_t st, int max_pml)
-{
-    using T = StressTile<R, TX, TY, PAD>;
-    const size_t lds_pml = T::LDS + sizeof(R) * 18 * (2 * (size_t)max_pml + 2);
-    return batched_pml_fork<R, T::NT>(k_stress_batched<R, TX, TY, PAD, false, 0>,
-                                      k_stress_batched<R, TX, TY, PAD, false, 1>,
-                                      k_stress_batched<R, TX, TY, PAD, false, 7>, T::LDS, lds_pml,
-                                      tab, blk, nb, step, st);
-}
-
-template <typename R, int TX, int TY>
-hipError_t velocity_batched_pml_launch(const BatchEntry<R> *tab, const int32_t *blk, int nb,
-                                       const int32_t *step, hipStream_t st, int max_mat,
-                                       int max_pml)
+                                   const int32_t *step, hipStream_t st, int max_mat, int max_pml)
 {
     using T = VelTile<R, TX, TY>;
     const size_t rho_bytes = sizeof(R) * (size_t)max_mat;
     const bool rho_lds = rho_bytes <= (size_t)kRhoLdsBytes;
-    const size_t lds = T::LDS + (rho_lds ? rho_bytes : 0);
-    const size_t lds_pml = lds + sizeof(R) * 18 * (2 * (size_t)max_pml + 2);
+    const bool cp = max_pml > 0;
+    const size_t lds = T::LDS + (rho_lds ? rho_bytes : 0) +
+                       (cp ? sizeof(R) * 18 * (2 * (size_t)max_pml + 2) : 0);
+    auto go = [&](auto kern) -> hipError_t {
+        hipError_t e = allow_lds(kern, lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(T::NT), lds, st, tab, blk, nb, step);
+        return hipGetLastError();
+    };
     if (rho_lds)
-        return batched_pml_fork<R, T::NT>(k_velocity_batched<R, TX, TY, true, 0>,
-                                          k_velocity_batched<R, TX, TY, true, 1>,
-                                          k_velocity_batched<R, TX, TY, true, 7>, lds, lds_pml,
-                                          tab, blk, nb, step, st);
-    return batched_pml_fork<R, T::NT>(k_velocity_batched<R, TX, TY, false, 0>,
-                                      k_velocity_batched<R, TX, TY, false, 1>,
-                                      k_velocity_batched<R, TX, TY, false, 7>, lds, lds_pml, tab,
-                                      blk, nb, step, st);
+        return cp ? go(k_velocity_batched<R, TX, TY, true, true>)
+                  : go(k_velocity_batched<R, TX, TY, true, false>);
+    return cp ? go(k_velocity_batched<R, TX, TY, false, true>)
+              : go(k_velocity_batched<R, TX, TY, false, false>);
 }
 
 // Tile variants (TX, TY, LDS row padding); variant 0 is the default.
@@ -1278,106 +1214,91 @@ int tiled_geometry(int precision, int half, int variant, int zchunk, int nz, int
 
 template <>
 hipError_t launch_stress_batched<float>(int v, const BatchEntry<float> *tab, const int32_t *blk,
-                                        int nb, const int32_t *step, hipStream_t st)
+                                        int nb, const int32_t *step, hipStream_t st, bool cp)
 {
     switch (v) {
-    case 1: return stress_batched_launch<float, 62, 6, 0>(tab, blk, nb, step, st);
-    case 2: return stress_batched_launch<float, 30, 14, 24>(tab, blk, nb, step, st);
-    case 3: return stress_batched_launch<float, 32, 16, 24>(tab, blk, nb, step, st);
-    case 4: return stress_batched_launch<float, 32, 8, 24>(tab, blk, nb, step, st);
-    case 5: return stress_batched_launch<float, 16, 16, 0>(tab, blk, nb, step, st);
-    case 6: return stress_batched_launch<float, 32, 16, 0>(tab, blk, nb, step, st);
-    case 7: return stress_batched_launch<float, 30, 14, 24, true>(tab, blk, nb, step, st);
-    case 8: return stress_batched_launch<float, 32, 13, 24, true>(tab, blk, nb, step, st);
-    case 9: return stress_batched_launch<float, 30, 14, 0, true>(tab, blk, nb, step, st);
-    case 10: return stress_batched_launch<float, 62, 14, 0>(tab, blk, nb, step, st);
-    case 11: return stress_batched_launch<float, 62, 14, 0, true>(tab, blk, nb, step, st);
+    case 1: return stress_batched_launch<float, 62, 6, 0>(tab, blk, nb, step, st, cp);
+    case 2: return stress_batched_launch<float, 30, 14, 24>(tab, blk, nb, step, st, cp);
+    case 3: return stress_batched_launch<float, 32, 16, 24>(tab, blk, nb, step, st, cp);
+    case 4: return stress_batched_launch<float, 32, 8, 24>(tab, blk, nb, step, st, cp);
+    case 5: return stress_batched_launch<float, 16, 16, 0>(tab, blk, nb, step, st, cp);
+    case 6: return stress_batched_launch<float, 32, 16, 0>(tab, blk, nb, step, st, cp);
+    case 7: return stress_batched_launch<float, 30, 14, 24, true>(tab, blk, nb, step, st, cp);
+    case 8: return stress_batched_launch<float, 32, 13, 24, true>(tab, blk, nb, step, st, cp);
+    case 9: return stress_batched_launch<float, 30, 14, 0, true>(tab, blk, nb, step, st, cp);
+    case 10: return stress_batched_launch<float, 62, 14, 0>(tab, blk, nb, step, st, cp);
+    case 11: return stress_batched_launch<float, 62, 14, 0, true>(tab, blk, nb, step, st, cp);
     case 12: return stress_pipe_batched_launch<float, 32, 13>(tab, blk, nb, step, st);
     case 13: return stress_pipe_batched_launch<float, 30, 14>(tab, blk, nb, step, st);
-    default: return stress_batched_launch<float, 32, 13, 24>(tab, blk, nb, step, st);
+    default: return stress_batched_launch<float, 32, 13, 24>(tab, blk, nb, step, st, cp);
     }
 }
 
 template <>
 hipError_t launch_stress_batched<double>(int v, const BatchEntry<double> *tab, const int32_t *blk,
-                                         int nb, const int32_t *step, hipStream_t st)
+                                         int nb, const int32_t *step, hipStream_t st, bool cp)
 {
     switch (v) {
-    case 1: return stress_batched_launch<double, 32, 4, 0>(tab, blk, nb, step, st);
-    case 2: return stress_batched_launch<double, 16, 8, 0>(tab, blk, nb, step, st);
-    case 3: return stress_batched_launch<double, 32, 6, 24>(tab, blk, nb, step, st);
+    case 1: return stress_batched_launch<double, 32, 4, 0>(tab, blk, nb, step, st, cp);
+    case 2: return stress_batched_launch<double, 16, 8, 0>(tab, blk, nb, step, st, cp);
+    case 3: return stress_batched_launch<double, 32, 6, 24>(tab, blk, nb, step, st, cp);
     case 4: return stress_pipe_batched_launch<double, 32, 6>(tab, blk, nb, step, st);
-    default: return stress_batched_launch<double, 32, 6, 0>(tab, blk, nb, step, st);
+    default: return stress_batched_launch<double, 32, 6, 0>(tab, blk, nb, step, st, cp);
     }
 }
 
 template <>
 hipError_t launch_velocity_batched<float>(int v, const BatchEntry<float> *tab, const int32_t *blk,
                                           int nb, const int32_t *step, hipStream_t st,
-                                          int max_mat)
+                                          int max_mat, int cp)
 {
     switch (v) {
-    case 1: return velocity_batched_launch<float, 32, 16>(tab, blk, nb, step, st, max_mat);
-    case 2: return velocity_batched_launch<float, 32, 8>(tab, blk, nb, step, st, max_mat);
-    case 3: return velocity_batched_launch<float, 64, 4>(tab, blk, nb, step, st, max_mat);
-    case 4: return velocity_batched_launch<float, 128, 4>(tab, blk, nb, step, st, max_mat);
-    case 5: return velocity_batched_launch<float, 32, 32>(tab, blk, nb, step, st, max_mat);
-    default: return velocity_batched_launch<float, 64, 8>(tab, blk, nb, step, st, max_mat);
+    case 1: return velocity_batched_launch<float, 32, 16>(tab, blk, nb, step, st, max_mat, cp);
+    case 2: return velocity_batched_launch<float, 32, 8>(tab, blk, nb, step, st, max_mat, cp);
+    case 3: return velocity_batched_launch<float, 64, 4>(tab, blk, nb, step, st, max_mat, cp);
+    case 4: return velocity_batched_launch<float, 128, 4>(tab, blk, nb, step, st, max_mat, cp);
+    case 5: return velocity_batched_launch<float, 32, 32>(tab, blk, nb, step, st, max_mat, cp);
+    default: return velocity_batched_launch<float, 64, 8>(tab, blk, nb, step, st, max_mat, cp);
     }
 }
 
 template <>
 hipError_t launch_velocity_batched<double>(int v, const BatchEntry<double> *tab,
                                            const int32_t *blk, int nb, const int32_t *step,
-                                           hipStream_t st, int max_mat)
+                                           hipStream_t st, int max_mat, int cp)
 {
     switch (v) {
-    case 1: return velocity_batched_launch<double, 64, 4>(tab, blk, nb, step, st, max_mat);
-    case 2: return velocity_batched_launch<double, 32, 16>(tab, blk, nb, step, st, max_mat);
-    case 3: return velocity_batched_launch<double, 32, 4>(tab, blk, nb, step, st, max_mat);
-    default: return velocity_batched_launch<double, 32, 8>(tab, blk, nb, step, st, max_mat);
+    case 1: return velocity_batched_launch<double, 64, 4>(tab, blk, nb, step, st, max_mat, cp);
+    case 2: return velocity_batched_launch<double, 32, 16>(tab, blk, nb, step, st, max_mat, cp);
+    case 3: return velocity_batched_launch<double, 32, 4>(tab, blk, nb, step, st, max_mat, cp);
+    default: return velocity_batched_launch<double, 32, 8>(tab, blk, nb, step, st, max_mat, cp);
     }
 }
 
-template <>
-hipError_t launch_stress_batched_pml<float>(const BatchEntry<float> *tab, const int32_t *blk,
-                                            int nb, const int32_t *step, hipStream_t st,
-                                            int max_pml)
+template <typename R>
+hipError_t launch_pml_corr_batched(int half, const BatchEntry<R> *tab, const int32_t *blk, int nb,
+                                   const int32_t *step, hipStream_t st)
 {
-    return stress_batched_pml_launch<float, 32, 13, 24>(tab, blk, nb, step, st, max_pml);
+    if (nb <= 0) return hipSuccess;
+    if (half == 0)
+        hipLaunchKernelGGL((k_pml_corr_batched<R, 0>), dim3(nb), dim3(kPmlBlock), 0, st, tab, blk,
+                           step);
+    else
+        hipLaunchKernelGGL((k_pml_corr_batched<R, 1>), dim3(nb), dim3(kPmlBlock), 0, st, tab, blk,
+                           step);
+    return hipGetLastError();
 }
 
-template <>
-hipError_t launch_stress_batched_pml<double>(const BatchEntry<double> *tab, const int32_t *blk,
-                                             int nb, const int32_t *step, hipStream_t st,
-                                             int max_pml)
-{
-    return stress_batched_pml_launch<double, 32, 6, 0>(tab, blk, nb, step, st, max_pml);
-}
-
-template <>
-hipError_t launch_velocity_batched_pml<float>(const BatchEntry<float> *tab, const int32_t *blk,
-                                              int nb, const int32_t *step, hipStream_t st,
-                                              int max_mat, int max_pml)
-{
-    return velocity_batched_pml_launch<float, 64, 8>(tab, blk, nb, step, st, max_mat, max_pml);
-}
-
-template <>
-hipError_t launch_velocity_batched_pml<double>(const BatchEntry<double> *tab, const int32_t *blk,
-                                               int nb, const int32_t *step, hipStream_t st,
-                                               int max_mat, int max_pml)
-{
-    return velocity_batched_pml_launch<double, 32, 8>(tab, blk, nb, step, st, max_mat, max_pml);
-}
+template hipError_t launch_pml_corr_batched<float>(int, const BatchEntry<float> *, const int32_t *,
+                                                   int, const int32_t *, hipStream_t);
+template hipError_t launch_pml_corr_batched<double>(int, const BatchEntry<double> *,
+                                                    const int32_t *, int, const int32_t *,
+                                                    hipStream_t);
 
 template <>
 hipError_t launch_stress_tiled<float>(const Shot<float> &s, int v, int zc, hipStream_t st,
                                       bool g)
 {
-    // CPML shots run the variant-0 tile shape whatever the variant (6-wave
-    // band tiles at 168 VGPRs measured 1.5-1.8x slower)
-    if (s.pml > 0) return stress_launch<float, 32, 13, 24, false, true>(s, zc, st, g);
     switch (v) {
     case 1: return stress_launch<float, 62, 6, 0>(s, zc, st, g);
     case 2: return stress_launch<float, 30, 14, 24>(s, zc, st, g);
@@ -1400,7 +1321,6 @@ template <>
 hipError_t launch_stress_tiled<double>(const Shot<double> &s, int v, int zc, hipStream_t st,
                                        bool g)
 {
-    if (s.pml > 0) return stress_launch<double, 32, 6, 0, false, true>(s, zc, st, g);
     switch (v) {
     case 1: return stress_launch<double, 32, 4, 0>(s, zc, st, g);
     case 2: return stress_launch<double, 16, 8, 0>(s, zc, st, g);
@@ -1413,13 +1333,6 @@ hipError_t launch_stress_tiled<double>(const Shot<double> &s, int v, int zc, hip
 template <>
 hipError_t launch_velocity_tiled<float>(const Shot<float> &s, int v, int zc, hipStream_t st)
 {
-    // CPML shots: the band path is compiled for variants 0, 2 and 3 (the
-    // 4-wave tiles fit three band workgroups per CU); others run variant 0
-    if (s.pml > 0) {
-        if (v == 2) return velocity_launch<float, 32, 8, true>(s, zc, st);
-        if (v == 3) return velocity_launch<float, 64, 4, true>(s, zc, st);
-        return velocity_launch<float, 64, 8, true>(s, zc, st);
-    }
     switch (v) {
     case 1: return velocity_launch<float, 32, 16>(s, zc, st);
     case 2: return velocity_launch<float, 32, 8>(s, zc, st);
@@ -1433,7 +1346,6 @@ hipError_t launch_velocity_tiled<float>(const Shot<float> &s, int v, int zc, hip
 template <>
 hipError_t launch_velocity_tiled<double>(const Shot<double> &s, int v, int zc, hipStream_t st)
 {
-    if (s.pml > 0) return velocity_launch<double, 32, 8, true>(s, zc, st);
     switch (v) {
     case 1: return velocity_launch<double, 64, 4>(s, zc, st);
     case 2: return velocity_launch<double, 32, 16>(s, zc, st);

@@ -199,11 +199,9 @@ __device__ __forceinline__ void strains_col(const Shot<R> &S, const R *tx_, cons
     a[5] = a6 * S.inv_h;
 }
 
-// CPML in the tiled kernels (the recursion of pml_d).  The band part of the
-// profiles is copied to LDS at kernel start as bp[((a*2 + st)*3 + c)*W + bi]
-// (W = 2L+2 band indices per axis), and the memory variables of a cell are
-// loaded at the top of its plane's step, before that step's prefetches, so
-// the band path waits on no global load issued after them.
+// CPML profiles of the band cells in LDS, bp[((A*2 + st)*3 + c)*W + bi]
+// (axis A, stagger st, c = b, a, 1/kappa; W = 2L+2 band indices), copied at
+// kernel start by every thread of the workgroup.
 template <typename R>
 __device__ __forceinline__ void pml_profiles_to_lds(const Shot<R> &S, R *bp, int nthreads)
 {
@@ -219,128 +217,15 @@ __device__ __forceinline__ void pml_profiles_to_lds(const Shot<R> &S, R *bp, int
     }
 }
 
-// Flat offset of cell (iz, iy, ix) in one psi component of axis A (band
-// index bi along A) and that component's volume.  A component is smaller
-// than a field, so the offset fits the 32-bit addressing of ldg / stg; the
-// component bases are wave-uniform.
-template <int A, typename R>
-__device__ __forceinline__ int pml_flat(const Shot<R> &S, int iz, int iy, int ix, int bi,
-                                        size_t &vol)
+// pml_corr with the profile from the LDS table and the memory variable in a
+// register (advanced in place; the caller stores it).
+template <int A, int ST, typename R>
+__device__ __forceinline__ R pml_corr_lds(const R *bp, int W, int bi, R &psi, R dd)
 {
-    const int W = 2 * S.pml + 2;
-    if constexpr (A == 0) {
-        vol = (size_t)W * S.ny * S.nx;
-        return (bi * S.ny + iy) * S.nx + ix;
-    } else if constexpr (A == 1) {
-        vol = (size_t)S.nz * W * S.nx;
-        return (iz * W + bi) * S.nx + ix;
-    } else {
-        vol = (size_t)S.nz * S.ny * W;
-        return (iz * S.ny + iy) * W + bi;
-    }
-}
-
-template <int A, typename R>
-__device__ __forceinline__ int pml_bi(const Shot<R> &S, int iz, int iy, int ix)
-{
-    const int L = S.pml, W = 2 * L + 2;
-    const int n = A == 0 ? S.nz : (A == 1 ? S.ny : S.nx);
-    const int ia = A == 0 ? iz : (A == 1 ? iy : ix);
-    if (!pml_in(ia, n, L)) return -1;
-    return ia <= L ? ia : ia - (n - W);
-}
-
-// psi components K0..K0+2 of every band axis of a cell (read buffer):
-// ps[3A + k]; 0 along axes whose band the cell is not in.
-template <int A, typename R>
-__device__ __forceinline__ void pml_psi_load_axis(const Shot<R> &S, int iz, int iy, int ix, int K0,
-                                                  R (&ps)[9])
-{
-    const int bi = pml_bi<A>(S, iz, iy, ix);
-    if (bi < 0) {
-        ps[3 * A] = ps[3 * A + 1] = ps[3 * A + 2] = 0;
-        return;
-    }
-    size_t vol;
-    const int f = pml_flat<A>(S, iz, iy, ix, bi, vol);
-    const R *q = S.psi[A] + ((size_t)S.pml_par * 6 + K0) * vol;
-    ps[3 * A] = ldg(q, f);
-    ps[3 * A + 1] = ldg(q + vol, f);
-    ps[3 * A + 2] = ldg(q + 2 * vol, f);
-}
-
-// M: the axes whose band the caller's tile can meet (bit 0 z, bit 1 y,
-// bit 2 x); the memory variables of the other axes are compiled out.
-template <int M = 7, typename R>
-__device__ __forceinline__ void pml_psi_load(const Shot<R> &S, int iz, int iy, int ix, int K0,
-                                             R (&ps)[9])
-{
-    if constexpr (M & 1) pml_psi_load_axis<0>(S, iz, iy, ix, K0, ps);
-    else ps[0] = ps[1] = ps[2] = 0;
-    if constexpr (M & 2) pml_psi_load_axis<1>(S, iz, iy, ix, K0, ps);
-    else ps[3] = ps[4] = ps[5] = 0;
-    if constexpr (M & 4) pml_psi_load_axis<2>(S, iz, iy, ix, K0, ps);
-    else ps[6] = ps[7] = ps[8] = 0;
-}
-
-// pml_d with the profile from the LDS band table and psi given (ps[3A +
-// K % 3]); the new psi goes to the write buffer when `write`.
-template <int A, int K, int ST, int M = 7, typename R>
-__device__ __forceinline__ R pml_t(const Shot<R> &S, const R *bp, int iz, int iy, int ix, R dd,
-                                   const R (&ps)[9], bool write)
-{
-    if constexpr (!((M >> A) & 1)) return dd;   // not in this axis's band (see pml_psi_load)
-    const int bi = pml_bi<A>(S, iz, iy, ix);
-    if (bi < 0) return dd;
-    const int W = 2 * S.pml + 2;
     const R *pr = bp + ((A * 2 + ST) * 3) * W + bi;
-    const R p = fma_r(pr[0], ps[3 * A + K % 3], pr[W] * dd);
-    if (write) {
-        size_t vol;
-        const int f = pml_flat<A>(S, iz, iy, ix, bi, vol);
-        stg(S.psi[A] + ((size_t)(1 - S.pml_par) * 6 + K) * vol, f, p);
-    }
-    return fma_r(dd, pr[2 * W], p);
-}
-
-// Strains of one column at a CPML band cell (see pml_d): the edge-guarded
-// taps of strains_col accumulated per derivative, scaled by 1/h, through
-// pml_t; ps: the cell's psi (components 0..2); write: this thread owns the
-// cell's memory variables.  M: see pml_psi_load.
-template <typename R, int VX, int M = 7>
-__device__ __forceinline__ void strains_col_pml(const Shot<R> &S, const R *bp, const R (&ps)[9],
-                                                const R *tx_, const R *ty_, const R *tz_,
-                                                const R (&rx)[9], const R (&ry)[9],
-                                                const R (&rz)[9], int ix, int iy, int zc,
-                                                bool write, R (&a)[6])
-{
-    const int nz = S.nz, ny = S.ny, nx = S.nx;
-    R d[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        if (j >= S.nc) break;
-        const R cj = S.c[j];
-        if (ix >= j + 1 && ix < nx - j) d[0] = fma_r(cj, tx_[j] - tx_[-(j + 1)], d[0]);
-        if (iy >= j + 1 && iy < ny - j) d[1] = fma_r(cj, ty_[j * VX] - ty_[-(j + 1) * VX], d[1]);
-        if (zc >= j + 1 && zc < nz - j) d[2] = fma_r(cj, rz[4 + j] - rz[3 - j], d[2]);
-        if (iy >= j && iy < ny - j - 1) d[3] = fma_r(cj, tz_[(j + 1) * VX] - tz_[-j * VX], d[3]);
-        if (zc >= j && zc < nz - j - 1) d[4] = fma_r(cj, ry[5 + j] - ry[4 - j], d[4]);
-        if (ix >= j && ix < nx - j - 1) d[5] = fma_r(cj, tz_[j + 1] - tz_[-j], d[5]);
-        if (zc >= j && zc < nz - j - 1) d[6] = fma_r(cj, rx[5 + j] - rx[4 - j], d[6]);
-        if (ix >= j && ix < nx - j - 1) d[7] = fma_r(cj, ty_[j + 1] - ty_[-j], d[7]);
-        if (iy >= j && iy < ny - j - 1) d[8] = fma_r(cj, tx_[(j + 1) * VX] - tx_[-j * VX], d[8]);
-    }
-#pragma unroll
-    for (int t = 0; t < 9; ++t) d[t] = d[t] * S.inv_h;
-    a[0] = pml_t<2, 0, 0, M>(S, bp, zc, iy, ix, d[0], ps, write);
-    a[1] = pml_t<1, 0, 0, M>(S, bp, zc, iy, ix, d[1], ps, write);
-    a[2] = pml_t<0, 0, 0, M>(S, bp, zc, iy, ix, d[2], ps, write);
-    a[3] = pml_t<1, 1, 1, M>(S, bp, zc, iy, ix, d[3], ps, write) +
-           pml_t<0, 1, 1, M>(S, bp, zc, iy, ix, d[4], ps, write);
-    a[4] = pml_t<2, 1, 1, M>(S, bp, zc, iy, ix, d[5], ps, write) +
-           pml_t<0, 2, 1, M>(S, bp, zc, iy, ix, d[6], ps, write);
-    a[5] = pml_t<2, 2, 1, M>(S, bp, zc, iy, ix, d[7], ps, write) +
-           pml_t<1, 2, 1, M>(S, bp, zc, iy, ix, d[8], ps, write);
+    const R p = fma_r(pr[0], psi, pr[W] * dd);
+    psi = p;
+    return fma_r(dd, pr[2 * W] - (R)1, p);
 }
 
 // Stiffness rows are read in place.  A wave whose active lanes share one

@@ -35,7 +35,8 @@ class DeviceShot:
     holds their addresses.  Fields start at zero.  ``mats`` and ``damp`` may
     be given as device tensors (prepared on the GPU), else host arrays.
     ``cpml``: a ``cpml.Cpml`` for a CPML boundary (profiles on the device,
-    memory variables allocated here, zero), or None.
+    memory variables allocated here, zero; the tiled kernels also get a
+    6 x ncell buffer for the band corrections), or None.
     """
 
     def __init__(self, mats, Ct, rho_t, damp, coeffs, h, dt, src_idx, src_w,
@@ -124,9 +125,14 @@ class DeviceShot:
             d.cpml_width = int(cpml.width)
             for a in range(3):
                 self.cpml_prof.append(dev_real(cpml.prof[a]))
-                self.psi.append(torch.zeros((2, 6, int(cpml.vol[a])), dtype=self.real, device=dev))
+                self.psi.append(torch.zeros((6, int(cpml.vol[a])), dtype=self.real, device=dev))
                 d.cpml_prof[a] = self.cpml_prof[a].data_ptr()
                 d.cpml_psi[a] = self.psi[a].data_ptr()
+            if kernel == "tiled":
+                # band corrections the tiled kernels' pre-pass writes (band
+                # cells only are written and read)
+                self.cpml_corr = torch.empty((6,) + self.shape, dtype=self.real, device=dev)
+                d.cpml_corr = self.cpml_corr.data_ptr()
         self.desc = d
         self.lib = _abi.load()
         _abi.check(self.lib.pe_check_shot(C.byref(d)), "pe_check_shot")
@@ -141,11 +147,6 @@ class DeviceShot:
             f.zero_()
         if self.g is not None:
             self.g.zero_()
-
-    def set_step(self, n):
-        """Step index whose parity picks the CPML memory-variable buffer read by
-        ``stress()`` / ``velocity()`` (``run`` takes it from its own loop)."""
-        self.desc.cpml_step = int(n)
 
     def stress(self):
         _abi.check(self.lib.pe_stress(C.byref(self.desc), self._stream()), "pe_stress")

@@ -140,6 +140,9 @@ def test_cpml_is_refused_by_adjoint_and_history():
     for a in range(3):
         d.cpml_prof[a] = fake
         d.cpml_psi[a] = fake
+    assert lib.pe_check_shot(C.byref(d)) == _abi.PE_EINVAL
+    assert "cpml_corr" in lib.pe_last_error().decode()
+    d.cpml_corr = fake
     assert lib.pe_check_shot(C.byref(d)) == _abi.PE_OK
     adj = _abi.PeAdjoint()
     for k in range(3):
@@ -153,19 +156,29 @@ def test_cpml_is_refused_by_adjoint_and_history():
     assert "CPML" in lib.pe_last_error().decode()
 
 
-def test_batch_with_cpml_needs_tile_variant_zero():
-    """A batch accepts CPML shots with tile variant 0 and refuses others."""
+def test_batch_with_cpml_shots_sizes_the_band_prepass():
+    """A batch takes CPML shots with any tile variant; its table grows by one
+    int32 per pre-pass block (256 band cells) of every CPML shot."""
     lib = _abi.load()
     fake = 0x1000
     d = _desc(cpml_width=1)
     for a in range(3):
         d.cpml_prof[a] = fake
         d.cpml_psi[a] = fake
+    d.cpml_corr = fake
+    plain = (_abi.PeShot * 2)(_desc(), _desc())
     arr = (_abi.PeShot * 2)(d, _desc())
-    assert lib.pe_batch_table_bytes(arr, 2) > 0
+    n0 = lib.pe_batch_table_bytes(plain, 2)
+    n1 = lib.pe_batch_table_bytes(arr, 2)
+    nz, ny, nx, W = d.nz, d.ny, d.nx, 4
+    band = W * ny * nx + (nz - W) * (ny * nx - (ny - W) * (nx - W))
+    assert n1 == (n0 + 255) // 256 * 256 + 4 * ((band + 255) // 256)
     arr[0].tile[1] = 2
-    assert lib.pe_batch_table_bytes(arr, 2) == 0
-    assert "tile variant 0" in lib.pe_last_error().decode()
+    assert lib.pe_batch_table_bytes(arr, 2) > 0
+    d.cpml_corr = None
+    bad = (_abi.PeShot * 1)(d)
+    assert lib.pe_batch_table_bytes(bad, 1) == 0
+    assert "cpml_corr" in lib.pe_last_error().decode()
 
 
 def test_noalloc_batch_refuses_oversized_batch_before_device_work():
@@ -173,7 +186,7 @@ def test_noalloc_batch_refuses_oversized_batch_before_device_work():
     memory on the host, before it touches a device (runs without a GPU)."""
     lib = _abi.load()
     fake = 0x1000
-    n = 20000
+    n = 60000
     arr = (_abi.PeShot * n)(*([_desc(n_src=0)] * n))
     nbytes = lib.pe_batch_table_bytes(arr, n)
     assert nbytes > _abi.BATCH_SCRATCH_BYTES

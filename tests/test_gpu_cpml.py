@@ -1,8 +1,10 @@
 """GPU parity of the CPML boundary (through the C ABI) against the CPU
 oracle's band path: fields, traces and memory variables bit for bit in
-float32 and float64, for the direct and the tiled kernels, with z-chunks
-short enough that neighbouring workgroups share band planes and ring
-columns (whose memory variables only the owning workgroup advances).
+float32 and float64, for the direct kernels (corrections computed in line)
+and the tiled kernels (band pre-pass, corrections added by every tile
+variant), with z-chunks short enough that neighbouring workgroups share
+band planes and ring columns (which read the corrections the pre-pass wrote
+once).
 """
 from __future__ import annotations
 
@@ -61,61 +63,50 @@ def test_cpml_matches_oracle(torch_gpu, kernel, precision):
     assert H.rel_l2(tr, otr) <= 1e-6
 
 
-def _xy_band_free_tile(shape, L, TX, TY, half):
-    """Whether some TX x TY tile (stress: with its 1-column ring) lies clear
-    of the x and y bands: the tiles the plain and z band kernels take."""
-    nz, ny, nx = shape
-    r = 1 if half == 0 else 0
-    xs = [x0 for x0 in range(0, nx, TX) if x0 - r > L and x0 + TX - 1 + r < nx - 1 - L]
-    ys = [y0 for y0 in range(0, ny, TY) if y0 - r > L and y0 + TY - 1 + r < ny - 1 - L]
-    return bool(xs and ys)
-
-
 @pytest.mark.parametrize("precision", ["f32", "f64"])
-def test_cpml_every_band_kernel_matches_oracle(torch_gpu, precision):
-    """A grid on which each half-step gives tiles to all three tiled
-    kernels of a CPML shot: the plain kernel (tiles clear of every band),
-    the z band kernel (tiles in a z slab, clear of the x and y bands) and
-    the full band kernel; fields, traces and memory variables bit for bit."""
-    print("\n[gpu case] cpml every band kernel %s" % precision, flush=True)
+def test_cpml_every_tile_variant_matches_oracle(torch_gpu, precision):
+    """Every stress and velocity tile variant of the tiled kernels with a
+    CPML (tiles inside, across and clear of every band; 5-plane chunks)
+    gives the oracle's fields and memory variables bit for bit."""
     from oracle import oracle as O
     from pulse_echo_azimuthal_simulation_amd import _abi
     from pulse_echo_azimuthal_simulation_amd.solver import DeviceShot
     shape, L = (24, 40, 136), 4
-    for h in (0, 1):
-        tx, ty = _abi.tile_variants(precision, h)[0][:2]
-        assert _xy_band_free_tile(shape, L, tx, ty, h), (h, tx, ty)
-    assert shape[0] - (L + 2) > L + 2          # stress planes clear of the z bands exist
     inp, cp = _medium(shape=shape, seed=13, L=L)
-    nt = inp["nt"]
-    sh = DeviceShot(inp["mats"], inp["Ct"], inp["rho_t"], inp["damp"], inp["coeffs"], inp["h"],
-                    inp["dt"], inp["src"][0], inp["src"][1], *inp["rec"], precision=precision,
-                    kernel="tiled", zchunk=(5, 5), cpml=cp)
-    wav = sh.wavelet(inp["wavelet"])
-    tr = sh.new_traces(nt)
-    sh.run(wav, 0, nt, tr)
-    v, s = sh.fields()
-    psi = [p.cpu().numpy() for p in sh.psi]
+    nt = 24
     osh = O.OracleShot(inp["mats"], inp["Ct"], inp["rho_t"], inp["damp"], inp["coeffs"],
                        inp["h"], inp["dt"], inp["src"], inp["rec"], precision=precision,
                        cpml=(cp.width, cp.prof))
     otr = osh.run(inp["wavelet"], 0, nt)
     assert np.abs(otr).max() > 0
-    for k in range(3):
-        assert np.array_equal(v[k], osh.v[k]), "v%d" % k
-    for k in range(6):
-        assert np.array_equal(s[k], osh.s[k]), "s%d" % (k + 1)
-    for a in range(3):
-        assert np.array_equal(psi[a].reshape(osh.psi[a].shape), osh.psi[a]), "psi axis %d" % a
-    assert H.rel_l2(tr.cpu().numpy(), otr) <= 1e-6
+    nst = len(_abi.tile_variants(precision, 0))
+    nve = len(_abi.tile_variants(precision, 1))
+    for tile in [(v, 0) for v in range(nst)] + [(0, v) for v in range(1, nve)]:
+        print("\n[gpu case] cpml tile variants %s %s" % (precision, tile), flush=True)
+        sh = DeviceShot(inp["mats"], inp["Ct"], inp["rho_t"], inp["damp"], inp["coeffs"],
+                        inp["h"], inp["dt"], inp["src"][0], inp["src"][1], *inp["rec"],
+                        precision=precision, kernel="tiled", zchunk=(5, 5), cpml=cp, tile=tile)
+        wav = sh.wavelet(inp["wavelet"])
+        tr = sh.new_traces(nt)
+        sh.run(wav, 0, nt, tr)
+        v, s = sh.fields()
+        for k in range(3):
+            assert np.array_equal(v[k], osh.v[k]), ("v%d" % k, tile)
+        for k in range(6):
+            assert np.array_equal(s[k], osh.s[k]), ("s%d" % (k + 1), tile)
+        for a in range(3):
+            assert np.array_equal(sh.psi[a].cpu().numpy().reshape(osh.psi[a].shape),
+                                  osh.psi[a]), ("psi axis %d" % a, tile)
+        assert H.rel_l2(tr.cpu().numpy(), otr) <= 1e-6
+        del sh
 
 
 @pytest.mark.parametrize("precision", ["f32", "f64"])
 @pytest.mark.parametrize("nz", [10, 11])
 def test_cpml_thin_grid_without_band_free_middle(torch_gpu, nz, precision):
-    """nz = 2L+2 and 2L+3 with L = 4: no plane of the stress half-step is
-    clear of the z bands (one CPML launch over the whole grid), and the
-    velocity half-step has no middle plane (nz = 10) or one (nz = 11)."""
+    """nz = 2L+2 and 2L+3 with L = 4: every plane lies in a z band (nz = 10)
+    or one plane is clear of them (nz = 11); the band enumeration of the
+    pre-pass has no middle planes, or one."""
     print("\n[gpu case] cpml thin grid nz %d %s" % (nz, precision), flush=True)
     from oracle import oracle as O
     from pulse_echo_azimuthal_simulation_amd.solver import DeviceShot
@@ -140,43 +131,16 @@ def test_cpml_thin_grid_without_band_free_middle(torch_gpu, nz, precision):
     assert H.rel_l2(tr.cpu().numpy(), otr) <= 1e-6
 
 
-@pytest.mark.parametrize("vel_variant", [2, 3])
-def test_cpml_velocity_tile_variants_match_oracle(torch_gpu, vel_variant):
-    """The float32 velocity tiles with their own band instantiation (32x8,
-    64x4) give the oracle's fields bit for bit."""
-    print("\n[gpu case] cpml velocity variant %d" % vel_variant, flush=True)
-    from oracle import oracle as O
-    from pulse_echo_azimuthal_simulation_amd.solver import DeviceShot
-    inp, cp = _medium(shape=(23, 27, 70), seed=9)
-    nt = inp["nt"]
-    sh = DeviceShot(inp["mats"], inp["Ct"], inp["rho_t"], inp["damp"], inp["coeffs"], inp["h"],
-                    inp["dt"], inp["src"][0], inp["src"][1], *inp["rec"], precision="f32",
-                    kernel="tiled", zchunk=(6, 5), cpml=cp)
-    sh.desc.tile[1] = vel_variant
-    wav = sh.wavelet(inp["wavelet"])
-    tr = sh.new_traces(nt)
-    sh.run(wav, 0, nt, tr)
-    v, s = sh.fields()
-    osh = O.OracleShot(inp["mats"], inp["Ct"], inp["rho_t"], inp["damp"], inp["coeffs"],
-                       inp["h"], inp["dt"], inp["src"], inp["rec"], precision="f32",
-                       cpml=(cp.width, cp.prof))
-    otr = osh.run(inp["wavelet"], 0, nt)
-    for k in range(3):
-        assert np.array_equal(v[k], osh.v[k]), "v%d" % k
-    for k in range(6):
-        assert np.array_equal(s[k], osh.s[k]), "s%d" % (k + 1)
-    psi = [p.cpu().numpy() for p in sh.psi]
-    for a in range(3):
-        assert np.array_equal(psi[a].reshape(osh.psi[a].shape), osh.psi[a]), "psi axis %d" % a
-    assert H.rel_l2(tr.cpu().numpy(), otr) <= 1e-6
-
-
-@pytest.mark.parametrize("precision", ["f32", "f64"])
-def test_cpml_batch_equals_single_shots(torch_gpu, precision):
+@pytest.mark.parametrize("precision,tile", [("f32", (0, 0)), ("f64", (0, 0)), ("f32", (8, 2)),
+                                            ("f32", (12, 3))])
+def test_cpml_batch_equals_single_shots(torch_gpu, precision, tile):
     """Two CPML shots of different sizes, band widths and step counts and a
     sponge shot in one batch (graph replays of 7 steps): every trace, field
-    and memory variable bit-identical to that shot's own run."""
-    print("\n[gpu case] cpml batch %s" % precision, flush=True)
+    and memory variable bit-identical to that shot's own run.  Tile
+    variants: the defaults, the frame-ring stress tile with 32x8 velocity
+    tiles, and the one-barrier stress tile (which a batch with CPML shots
+    replaces by the default tile) with 64x4 velocity tiles."""
+    print("\n[gpu case] cpml batch %s %s" % (precision, tile), flush=True)
     torch = torch_gpu
     from pulse_echo_azimuthal_simulation_amd import sweep
     from pulse_echo_azimuthal_simulation_amd.solver import DeviceShot
@@ -185,12 +149,12 @@ def test_cpml_batch_equals_single_shots(torch_gpu, precision):
         inp, cp = _medium(shape=shape, seed=seed, L=L)
         sh = DeviceShot(inp["mats"], inp["Ct"], inp["rho_t"], inp["damp"], inp["coeffs"],
                         inp["h"], inp["dt"], inp["src"][0], inp["src"][1], *inp["rec"],
-                        precision=precision, kernel="tiled", cpml=cp)
+                        precision=precision, kernel="tiled", cpml=cp, tile=tile)
         items.append((sh, sh.wavelet(np.resize(inp["wavelet"], nt)), nt))
     inp = H.random_medium((20, 26, 31), n_grain=6, seed=1)
     sh = DeviceShot(inp["mats"], inp["Ct"], inp["rho_t"], inp["damp"], inp["coeffs"], inp["h"],
                     inp["dt"], inp["src"][0], inp["src"][1], *inp["rec"], precision=precision,
-                    kernel="tiled")
+                    kernel="tiled", tile=tile)
     items.append((sh, sh.wavelet(np.resize(inp["wavelet"], 37)), 37))
     singles = []
     for sh, wav, nt in items:

@@ -76,7 +76,7 @@ def test_noalloc_refuses_oversized_batch(torch_gpu):
     items = _shots([((12, 14, 16), 6, 10)])
     sh, wav, nt = items[0]
     lib = _abi.load()
-    n = 20000
+    n = 60000
     descs = (_abi.PeShot * n)(*([sh.desc] * n))
     nbytes = lib.pe_batch_table_bytes(descs, n)
     assert nbytes > _abi.BATCH_SCRATCH_BYTES
