@@ -145,6 +145,24 @@ int pe_run_store(const pe_shot *shot, int32_t n0, int32_t n1, const double *wave
 int pe_run_store_strains(const pe_shot *shot, int32_t n0, int32_t n1, const double *wavelet,
                          void *traces, void *ghist, void *stream);
 
+/* Dense-table twin of the step: the upstream forward_fused
+ * (sim/core/fdtd.py:831-921, kernels stress_acc / vel_update at
+ * sim/core/fdtd.py:392-583), the partner of the upstream's bit-equivalence
+ * gate (sim/core/validate_labels.py:66-100).  Stiffness as 36 per-cell
+ * planes Cflat[(6I+J)*ncell + i] (real; act_mask bit 6I+J: plane nonzero
+ * somewhere), face densities as three per-cell volumes rv[a]; per step:
+ * strains into shot->g and stress (no damping), source, velocity (no
+ * damping), then v and s times damp, then the record (post-damp).  Where
+ * damp == 1 the traces equal pe_run's on the matching label volume bit for
+ * bit.  The shot's mats / Ct / rho_t are not read; sponge boundary only. */
+typedef struct pe_dense {
+    const void *Cflat;           /* 36 x ncell real */
+    uint64_t act_mask;
+    const void *rv[3];           /* face densities x, y, z: ncell real each */
+} pe_dense;
+int pe_dense_run(const pe_shot *shot, const pe_dense *dense, int32_t n0, int32_t n1,
+                 const double *wavelet, void *traces, void *stream);
+
 /* Tile variants of the tiled kernels, for tuning.  half: 0 stress, 1
  * velocity.  Returns the number of variants at that precision (>= 1) and,
  * when info3 is not NULL, writes (tile x, tile y, flags) of `variant`:

@@ -24,7 +24,7 @@ EXPORTS = ("pe_abi_version", "pe_last_error", "pe_check_shot", "pe_stress", "pe_
            "pe_batch_table_bytes", "pe_batch_setup", "pe_batch_step", "pe_batch_graph",
            "pe_graph_launch", "pe_graph_destroy", "pe_prep_rotate", "pe_prep_edt_damp",
            "pe_prep_vmax", "pe_prep_label_power", "pe_run_store_strains",
-           "pe_batch_setup_noalloc")
+           "pe_batch_setup_noalloc", "pe_dense_run")
 
 #: bytes of library-owned device memory behind pe_batch_setup_noalloc
 BATCH_SCRATCH_BYTES = 8 * 1024 * 1024
@@ -80,6 +80,11 @@ class PeBatch(C.Structure):
     ]
 
 
+class PeDense(C.Structure):
+    """Mirror of struct pe_dense (include/pefdtd.h)."""
+    _fields_ = [("Cflat", C.c_void_p), ("act_mask", C.c_uint64), ("rv", C.c_void_p * 3)]
+
+
 class PeError(RuntimeError):
     """A libpefdtd entry point returned an error code."""
 
@@ -124,6 +129,8 @@ def load():
     lib.pe_batch_table_bytes.restype = C.c_int64
     lib.pe_batch_setup.argtypes = [B, C.c_void_p]
     lib.pe_batch_setup_noalloc.argtypes = [B, C.c_void_p]
+    lib.pe_dense_run.argtypes = [P, C.POINTER(PeDense), C.c_int32, C.c_int32, C.c_void_p,
+                                 C.c_void_p, C.c_void_p]
     lib.pe_batch_step.argtypes = [B, C.c_int32, C.c_void_p]
     lib.pe_batch_graph.argtypes = [B, C.c_int32, C.c_void_p, C.POINTER(C.c_void_p)]
     lib.pe_graph_launch.argtypes = [C.c_void_p, C.c_void_p]
@@ -144,7 +151,7 @@ def load():
                  "pe_adjoint_run", "pe_adjoint_gradient", "pe_batch_setup", "pe_batch_step",
                  "pe_batch_graph", "pe_graph_launch", "pe_graph_destroy", "pe_prep_rotate",
                  "pe_prep_edt_damp", "pe_prep_vmax", "pe_prep_label_power",
-                 "pe_run_store_strains", "pe_batch_setup_noalloc"):
+                 "pe_run_store_strains", "pe_batch_setup_noalloc", "pe_dense_run"):
         getattr(lib, name).restype = C.c_int
     ver = lib.pe_abi_version()
     if ver != ABI_VERSION:

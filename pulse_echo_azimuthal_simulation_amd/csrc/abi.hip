@@ -629,6 +629,36 @@ int pe_prep_vmax(const int32_t *mats, int32_t nz, int32_t ny, int32_t nx, const 
     return e == hipSuccess ? PE_OK : hip_fail(e, "vmax launch");
 }
 
+int pe_dense_run(const pe_shot *p, const pe_dense *d, int32_t n0, int32_t n1,
+                 const double *wavelet, void *traces, void *stream)
+{
+    int rc = pe_check_shot(p);
+    if (rc) return rc;
+    if (!d || !d->Cflat || !d->rv[0] || !d->rv[1] || !d->rv[2])
+        return fail(PE_EINVAL, "the dense twin needs Cflat and three face-density volumes");
+    if (!p->g) return fail(PE_EINVAL, "the dense twin needs the 6 x ncell strain scratch g");
+    if (p->cpml_width > 0) return fail(PE_EINVAL, "the dense twin runs the sponge boundary only");
+    if (n1 < n0 || n0 < 0) return fail(PE_EINVAL, "bad step range");
+    if (!wavelet && p->n_src > 0) return fail(PE_EINVAL, "wavelet is NULL");
+    hipStream_t st = (hipStream_t)stream;
+    auto go = [&](auto tag) -> int {
+        using R = decltype(tag);
+        const pe::Shot<R> s = pe::make_shot<R>(p);
+        for (int32_t n = n0; n < n1; ++n) {
+            hipError_t e = pe::launch_dense_half<R>(s, 0, d->Cflat, d->act_mask, d->rv, st);
+            if (e == hipSuccess)
+                e = pe::launch_source_record<R>(s, p, wavelet, n, true, (R *)traces, false, st);
+            if (e == hipSuccess) e = pe::launch_dense_half<R>(s, 1, d->Cflat, d->act_mask, d->rv, st);
+            if (e == hipSuccess) e = pe::launch_dense_half<R>(s, 2, d->Cflat, d->act_mask, d->rv, st);
+            if (e == hipSuccess && traces)
+                e = pe::launch_source_record<R>(s, p, wavelet, n, false, (R *)traces, true, st);
+            if (e != hipSuccess) return hip_fail(e, "dense step launch");
+        }
+        return PE_OK;
+    };
+    return p->precision == PE_F32 ? go(float(0)) : go(double(0));
+}
+
 int64_t pe_batch_table_bytes(const pe_shot *shots, int32_t n_shot)
 {
     BatchLayout L;

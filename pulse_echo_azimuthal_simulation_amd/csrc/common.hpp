@@ -421,6 +421,12 @@ constexpr int kPmlBlock = 256;
 template <typename R>
 hipError_t launch_pml_corr_batched(int half, const BatchEntry<R> *tab, const int32_t *blk, int nb,
                                    const int32_t *step, hipStream_t st);
+// Dense-table twin (kernels_direct.hip): half 0 = strains into s.g then the
+// stress from the 36 planes Cflat; 1 = velocity from the face-density
+// volumes rv; 2 = damping of all nine fields.
+template <typename R>
+hipError_t launch_dense_half(const Shot<R> &s, int half, const void *Cflat, uint64_t act,
+                             const void *const rv[3], hipStream_t st);
 // Source / record of every active shot (phase 0, after the stress
 // half-step) and velocity-mode record plus step advance (phase 1, after the
 // velocity half-step); one workgroup per shot.

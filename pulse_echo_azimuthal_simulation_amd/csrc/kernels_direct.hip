@@ -184,6 +184,101 @@ __global__ __launch_bounds__(kBlock) void k_velocity_direct(Shot<R> S)
     vz[i] = (vz[i] + S.dt_h * az / rvz) * dmp;
 }
 
+// ------------------------------------------------------------ dense twin
+// The dense-table form of the step (forward_fused, sim/core/fdtd.py:831-921,
+// kernels stress_acc and vel_update, sim/core/fdtd.py:392-583): stiffness
+// from 36 per-cell planes Cflat[(6I+J)*ncell + i], face densities from three
+// per-cell volumes, no damping inside the half-steps (k_damp_dense applies
+// it to all nine fields after the velocity update).  Same statement order as
+// the label kernels, so where damp == 1 the two forms are bit-identical (the
+// upstream's bit-equivalence gate, sim/core/validate_labels.py:66-100).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_stress_dense(Shot<R> S, const R *__restrict__ Cflat,
+                                                         uint64_t act)
+{
+    const int nz = S.nz, ny = S.ny, nx = S.nx;
+    const int sz = nx * ny;
+    const int ncell = sz * nz;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= ncell) return;
+    const int ix = i % nx, iy = (i / nx) % ny, iz = i / sz;
+    const size_t n = (size_t)ncell;
+    R q[6];
+#pragma unroll
+    for (int I = 0; I < 6; ++I) {
+        if (I == 0) gather_q<0>(S, q, i, iz, iy, ix);
+        if (I == 3) gather_q<1>(S, q, i, iz, iy, ix);
+        if (I == 4) gather_q<2>(S, q, i, iz, iy, ix);
+        if (I == 5) gather_q<3>(S, q, i, iz, iy, ix);
+        R acc = 0;
+        bool any = false;
+#pragma unroll
+        for (int J = 0; J < 6; ++J) {
+            if ((act >> (6 * I + J)) & 1ull) {
+                acc = fma_r(Cflat[(size_t)(6 * I + J) * n + i], q[J], acc);
+                any = true;
+            }
+        }
+        if (any) S.s[I][i] = fma_r(S.dt, acc, S.s[I][i]);
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_velocity_dense(Shot<R> S, const R *__restrict__ rvx,
+                                                           const R *__restrict__ rvy,
+                                                           const R *__restrict__ rvz)
+{
+    const int nz = S.nz, ny = S.ny, nx = S.nx;
+    const int sy = nx, sz = nx * ny;
+    const int ncell = sz * nz;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= ncell) return;
+    const int ix = i % nx, iy = (i / nx) % ny, iz = i / sz;
+    const R *__restrict__ s1 = S.s[0];
+    const R *__restrict__ s2 = S.s[1];
+    const R *__restrict__ s3 = S.s[2];
+    const R *__restrict__ s4 = S.s[3];
+    const R *__restrict__ s5 = S.s[4];
+    const R *__restrict__ s6 = S.s[5];
+    R ax = 0, ay = 0, az = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (j >= S.nc) break;
+        const R cj = S.c[j];
+        if (ix >= j && ix < nx - j - 1) ax = fma_r(cj, s1[i + (j + 1)] - s1[i - j], ax);
+        if (ix >= j + 1 && ix < nx - j) {
+            ay = fma_r(cj, s6[i + j] - s6[i - (j + 1)], ay);
+            az = fma_r(cj, s5[i + j] - s5[i - (j + 1)], az);
+        }
+        if (iy >= j && iy < ny - j - 1) ay = fma_r(cj, s2[i + (j + 1) * sy] - s2[i - j * sy], ay);
+        if (iy >= j + 1 && iy < ny - j) {
+            ax = fma_r(cj, s6[i + j * sy] - s6[i - (j + 1) * sy], ax);
+            az = fma_r(cj, s4[i + j * sy] - s4[i - (j + 1) * sy], az);
+        }
+        if (iz >= j && iz < nz - j - 1) az = fma_r(cj, s3[i + (j + 1) * sz] - s3[i - j * sz], az);
+        if (iz >= j + 1 && iz < nz - j) {
+            ax = fma_r(cj, s5[i + j * sz] - s5[i - (j + 1) * sz], ax);
+            ay = fma_r(cj, s4[i + j * sz] - s4[i - (j + 1) * sz], ay);
+        }
+    }
+    S.v[0][i] = S.v[0][i] + S.dt_h * ax / rvx[i];
+    S.v[1][i] = S.v[1][i] + S.dt_h * ay / rvy[i];
+    S.v[2][i] = S.v[2][i] + S.dt_h * az / rvz[i];
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_damp_dense(Shot<R> S)
+{
+    const int ncell = S.nx * S.ny * S.nz;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= ncell || !S.damp) return;
+    const R d = S.damp[i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) S.v[k][i] = S.v[k][i] * d;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) S.s[k][i] = S.s[k][i] * d;
+}
+
 // Source injection and receiver recording in one single-workgroup launch.
 // Phase 1 adds (real)wavelet[n] * w_k to s1..s3 at the (unique) footprint
 // cells (sim/core/fdtd.py:1798-1800).  Phase 2, after a barrier, forms each
@@ -334,6 +429,26 @@ hipError_t launch_velocity_direct(const Shot<R> &s, hipStream_t st)
 }
 
 template <typename R>
+hipError_t launch_dense_half(const Shot<R> &s, int half, const void *Cflat, uint64_t act,
+                             const void *const rv[3], hipStream_t st)
+{
+    const int ncell = s.nx * s.ny * s.nz;
+    const dim3 grid(blocks_for(ncell)), block(kBlock);
+    if (half == 0) {
+        hipLaunchKernelGGL(k_strain_direct<R>, grid, block, 0, st, s);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_stress_dense<R>, grid, block, 0, st, s, (const R *)Cflat, act);
+    } else if (half == 1) {
+        hipLaunchKernelGGL(k_velocity_dense<R>, grid, block, 0, st, s, (const R *)rv[0],
+                           (const R *)rv[1], (const R *)rv[2]);
+    } else {
+        hipLaunchKernelGGL(k_damp_dense<R>, grid, block, 0, st, s);
+    }
+    return hipGetLastError();
+}
+
+template <typename R>
 hipError_t launch_source_record(const Shot<R> &s, const pe_shot *p, const double *wavelet,
                                 int32_t n, bool do_source, R *traces, bool do_record,
                                 hipStream_t st)
@@ -360,6 +475,10 @@ template hipError_t launch_source_record_batched<float>(const BatchEntry<float> 
                                                         const int32_t *, int, hipStream_t);
 template hipError_t launch_source_record_batched<double>(const BatchEntry<double> *, int,
                                                          const int32_t *, int, hipStream_t);
+template hipError_t launch_dense_half<float>(const Shot<float> &, int, const void *, uint64_t,
+                                             const void *const[3], hipStream_t);
+template hipError_t launch_dense_half<double>(const Shot<double> &, int, const void *, uint64_t,
+                                              const void *const[3], hipStream_t);
 template hipError_t launch_stress_direct<float>(const Shot<float> &, hipStream_t);
 template hipError_t launch_stress_direct<double>(const Shot<double> &, hipStream_t);
 template hipError_t launch_strain_direct<float>(const Shot<float> &, hipStream_t);

@@ -144,3 +144,23 @@ def active_mask(Ct) -> int:
         if flags[k]:
             mask |= 1 << k
     return mask
+
+
+#: the 21 upper-triangle Voigt keys of a dense stiffness volume set
+KEYS21 = tuple("C%d%d" % (i, j) for i in range(1, 7) for j in range(i, 7))
+
+
+def polycrystal_stiffness_3d(labels, axes, c_couplant: float = 1500.0,
+                             rho_couplant: float = 300.0, material=ICE):
+    """Dense per-cell stiffness and density of a label volume (label -1 =
+    couplant, g = grain g): (dict of the 21 upper-triangle Voigt volumes
+    "Cij" (float64), rho (float64)), each grain's values those of its
+    material-table row (vendor/ringfwi/anisotropy.py:418-456)."""
+    labels = np.asarray(labels)
+    Ct, rho_t = material_tables(axes, c_couplant, rho_couplant, material, dtype=np.float64)
+    m = labels.astype(np.int64) + 1
+    C = {}
+    for key in KEYS21:
+        i, j = int(key[1]) - 1, int(key[2]) - 1
+        C[key] = Ct[:, 6 * i + j][m]
+    return C, rho_t[m]

@@ -13,12 +13,13 @@ lifted; ``compat_uint8=True`` restores the upstream refusal).
 from __future__ import annotations
 
 import ctypes as C
+import ctypes as _ctypes
 
 import numpy as np
 
 from . import _abi, host, materials
 
-__all__ = ["DeviceShot", "forward_fused_labels", "prepare_shot"]
+__all__ = ["DeviceShot", "forward_fused_labels", "forward_fused", "prepare_shot"]
 
 
 def _torch():
@@ -352,6 +353,90 @@ def forward_fused_labels(lab, axes, h, dt, nt, src_pts, wavelet, rec_groups,
         shot.run(wav, n, n1, traces)
         if progress is not None and n % progress_every == 0:
             shot.torch.cuda.synchronize(shot.device)
+            progress(n / nt)
+        n = n1
+    return traces.cpu().numpy()
+
+
+def forward_fused(C, rho, h, dt, nt, src_pts, wavelet, rec_groups, order=8, coeffs=None,
+                  sponge_width=0, sponge_strength=0.09, damp_mask=None, record="pressure",
+                  progress=None, *, precision="f32", progress_every=200):
+    """Dense-table twin of ``forward_fused_labels`` (the upstream
+    ``fdtd.forward_fused``, sim/core/fdtd.py:831-921): stiffness as per-cell
+    volumes ``C`` (a dict "Cij", i <= j, of volumes or scalars; C_ji = C_ij),
+    density as a volume ``rho``; the sponge and ``damp_mask`` damp v and s
+    after the velocity update and the traces are recorded after the damping.
+    Returns the (nt, n_rec) float32 traces (float64 with precision="f64").
+
+    It runs separate kernels (pe_dense_run) with the label kernels'
+    statement order, so on a label volume's dense expansion its traces equal
+    forward_fused_labels' bit for bit wherever the receivers sit at damp ==
+    1 -- the upstream's bit-equivalence gate (sim/core/validate_labels.py:
+    66-100).
+    """
+    torch = _torch()
+    if record not in _abi.RECORD_MODES:
+        raise ValueError(f"record must be one of {sorted(_abi.RECORD_MODES)}")
+    if coeffs is None:
+        coeffs = host.optimised_coeffs(order)
+    coeffs = np.asarray(coeffs, float)
+    if len(coeffs) > 4:
+        raise ValueError("forward_fused supports order <= 8 (at most 4 coefficients)")
+    rho_h = np.asarray(rho, float)
+    if rho_h.ndim != 3:
+        raise ValueError("rho must be a (nz, ny, nx) volume")
+    shape = rho_h.shape
+    nz, ny, nx = shape
+    nt = int(nt)
+    wavelet = np.asarray(wavelet, float)
+    if len(wavelet) < nt:
+        raise ValueError(f"wavelet has {len(wavelet)} samples, nt = {nt}")
+    real = np.float32 if precision == "f32" else np.float64
+    # the planes hold float32 values in both precisions, as the label path's
+    # tables do: precision="f64" is float64 arithmetic on them
+    Cflat = np.zeros((36,) + shape, np.float32)
+    act = 0
+    for I in range(1, 7):
+        for J in range(1, 7):
+            a = np.asarray(C["C%d%d" % (min(I, J), max(I, J))], float)
+            if not np.any(a):
+                continue
+            Cflat[(I - 1) * 6 + (J - 1)] = a if a.ndim == 3 else float(a)
+            act |= 1 << ((I - 1) * 6 + (J - 1))
+    rv = [host.face_density(rho_h, ax).astype(np.float32).astype(real) for ax in (2, 1, 0)]
+    damp = host.sponge_profile(shape, sponge_width, sponge_strength) if sponge_width > 0 else None
+    if damp_mask is not None:
+        dm = np.asarray(damp_mask, np.float32)
+        damp = dm if damp is None else damp * dm
+    src_idx = np.array([np.ravel_multi_index(tuple(int(a) for a in p), shape)
+                        for p, _ in src_pts], np.int64)
+    src_w = np.array([w for _, w in src_pts], float)
+    if len(np.unique(src_idx)) != len(src_idx):
+        raise ValueError("source footprint cells must be unique")
+    rec_ptr, rec_idx, rec_w = _csr(rec_groups, shape)
+    # the shot's label and table are not read by the dense kernels
+    sh = DeviceShot(np.zeros(shape, np.int32), np.zeros((1, 36)), np.ones(1), damp, coeffs, h,
+                    dt, src_idx, src_w, rec_ptr, rec_idx, rec_w, record=record,
+                    precision=precision, kernel="direct", act_mask=act)
+    dev = sh.device
+    Cd = torch.as_tensor(Cflat, device=dev).to(sh.real)
+    del Cflat
+    rvd = [torch.as_tensor(np.ascontiguousarray(r), device=dev) for r in rv]
+    d = _abi.PeDense()
+    d.Cflat = Cd.data_ptr()
+    d.act_mask = int(act)
+    for k in range(3):
+        d.rv[k] = rvd[k].data_ptr()
+    wav = sh.wavelet(wavelet[:nt])
+    traces = sh.new_traces(nt)
+    n = 0
+    while n < nt:
+        n1 = min(nt, n + 1 if n % progress_every == 0 else (n // progress_every + 1) * progress_every)
+        _abi.check(sh.lib.pe_dense_run(_ctypes.byref(sh.desc), _ctypes.byref(d), int(n), int(n1),
+                                       wav.data_ptr(), traces.data_ptr(), sh._stream()),
+                   "pe_dense_run")
+        if progress is not None and n % progress_every == 0:
+            torch.cuda.synchronize(dev)
             progress(n / nt)
         n = n1
     return traces.cpu().numpy()

@@ -225,3 +225,94 @@ def twoway_case(theta, ppw, homog, psi_a=0.0, psi_b=51.0):
     return dict(h=h, N=N, c=c, lab=lab, Ct=Ct, rho_t=rho_t, src=[((c, c, c), 1.0)],
                 groups=groups, expected=expected, dt_ref=float(ref["dt_" + key]),
                 dt_inputs=(mats_dt, Ct_dt, rho_dt), ref=ref)
+
+
+# ------------------------------------------------------------ full disk
+FULLDISK_FRACS = {"00": 0.0, "25": 0.25, "50": 0.5, "100": 1.0}
+
+
+@functools.lru_cache(maxsize=1)
+def fulldisk_setup():
+    """Grid and shot of the upstream 2 MHz full-disk reference traces
+    (sim/fw_checks/fw_reference_2mhz.py:52-78 with ladder.standard_build,
+    sim/model/ladder.py:68-73, and coda_convergence.build_grid,
+    sim/fw_checks/coda_convergence.py:40-53): the 100 mm x 35 mm disk of
+    100 grains (size_cv 0.35, Watson kappa 3.93 about x, seed 7) at
+    h = 3850 / 2 MHz / 6, padded by sponge 10 + 3 couplant cells; the
+    legacy single-start kh 1.6 coefficients (the archives' dt); dt from the
+    base fabric; couplant mask exp(-0.02 EDT); the +x rim probe of 6.35 mm."""
+    from pulse_echo_azimuthal_simulation_amd.disk_specimen import DiskSpecimen, solver_grid
+    h = C_REF / 2e6 / 6.0
+    build = DiskSpecimen(diameter_m=0.100, thickness_m=0.035, n_grains=100, size_cv=0.35,
+                         concentration=3.93, spatial_corr=0.0, fabric_axis=(1, 0, 0),
+                         seed=7).build(h, adjacency=False)
+    lab, nd, m = solver_grid(build, h, 10 + 3)
+    nz, nx = lab.shape[0], lab.shape[1]
+    co = host.optimised_coeffs(8, kh_max=1.6, multistart=False)
+    Ct, rho_t = materials.material_tables(build["axes"])
+    dt = host.safe_dt_labels(lab + 1, Ct, rho_t, h, co, safety=0.5)
+    nt = int(2.6 * nd * h / C_REF / dt)
+    dm = host.fluid_damp_mask(lab, 0.02)
+    cz, cy, cx = nz // 2, nx // 2, nx // 2
+    Rc = nd // 2
+    ixp = next(cx + k - 1 for k in range(Rc - 1, 0, -1) if lab[cz, cy, cx + k] >= 0)
+    er = max(int(6.35e-3 / 2 / h), 1)
+    pts = [(cz + dz, cy + dy, ixp) for dy in range(-er, er + 1) for dz in range(-er, er + 1)
+           if dy * dy + dz * dz <= er * er and lab[cz + dz, cy + dy, ixp] >= 0]
+    return dict(build=build, lab=lab, nd=nd, h=h, coeffs=co, dt=dt, nt=nt, damp_mask=dm,
+                pts=pts, w=1.0 / len(pts), wavelet=host.ricker(2e6, dt, nt))
+
+
+def fulldisk_fabrics(base_axes):
+    """The four fabric sets of ladder.standard_fabrics (sim/model/ladder.py:
+    88-101): a fraction 0 / 0.25 / 0.5 / 1 of the grains given random axes
+    from one default_rng(11) stream."""
+    rng = np.random.default_rng(11)
+    out = {}
+    for tag, frac in FULLDISK_FRACS.items():
+        ax = np.array(base_axes, float, copy=True)
+        if frac > 0:
+            kk = int(frac * len(ax))
+            idx = rng.choice(len(ax), kk, replace=False)
+            v = rng.normal(size=(kk, 3))
+            v /= np.linalg.norm(v, axis=1, keepdims=True)
+            ax[idx] = v
+        out[tag] = ax
+    return out
+
+
+def coda_level(x, dt):
+    """The ladder's coda metric (sim/fw_checks/fw_reference_2mhz.py:38-46):
+    envelope maximum over 22-42 us relative to the first-echo envelope E1
+    (within 2 us of the 200 mm round trip), in dB; returns (dB, E1)."""
+    from scipy.signal import hilbert
+    t1 = 2 * 0.100 / C_REF
+    e = np.abs(hilbert(x))
+    k0 = int(t1 / dt)
+    E1 = e[max(k0 - int(2e-6 / dt), 0):min(k0 + int(2e-6 / dt), len(e) - 1)].max()
+    lo, hi = int(22e-6 / dt), min(int(42e-6 / dt), len(e))
+    return 20 * np.log10(e[lo:hi].max() / E1 + 1e-30), E1
+
+
+def validate_setup(f0, build, ppw=6.0, sponge=10, fluid_damp=0.02, coeffs=None):
+    """sim/core/validate_labels.py:34-63: solver grid of a specimen build
+    (pad sponge + 3), material tables, stable dt (safety 0.5), record length
+    2.6 D / c, Ricker, couplant mask exp(-fluid_damp EDT) and the +x rim
+    probe footprint of a 6.35 mm element."""
+    from pulse_echo_azimuthal_simulation_amd.disk_specimen import solver_grid
+    h = C_REF / f0 / ppw
+    lab, nd, m = solver_grid(build, h, sponge + 3)
+    nz, nx = lab.shape[0], lab.shape[1]
+    co = host.optimised_coeffs(8) if coeffs is None else coeffs
+    Ct, rho_t = materials.material_tables(build["axes"])
+    dt = host.safe_dt_labels(lab + 1, Ct, rho_t, h, co, safety=0.5)
+    nt = int(2.6 * nd * h / C_REF / dt)
+    dm = host.fluid_damp_mask(lab, fluid_damp)
+    cz, cy, cx = nz // 2, nx // 2, nx // 2
+    Rc = nd // 2
+    ixp = next(cx + k - 1 for k in range(Rc - 1, 0, -1) if lab[cz, cy, cx + k] >= 0)
+    er = max(int(6.35e-3 / 2 / h), 1)
+    pts = [(cz + dz, cy + dy, ixp) for dy in range(-er, er + 1) for dz in range(-er, er + 1)
+           if dy * dy + dz * dz <= er * er and lab[cz + dz, cy + dy, ixp] >= 0]
+    return dict(lab=lab, nd=nd, h=h, coeffs=co, dt=dt, nt=nt, damp_mask=dm, pts=pts,
+                w=1.0 / len(pts), wavelet=host.ricker(f0, dt, nt), sponge=sponge)
