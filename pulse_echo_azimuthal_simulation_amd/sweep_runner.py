@@ -177,14 +177,16 @@ def save_azimuth(root, cfg, az, trace, dt, h):
 
 
 def build_specimen(cfg, h):
-    """The disk of the config: Voronoi grains (plain; the upstream's Laguerre
-    size calibration, size_cv and spatial_corr are not reproduced) with a
-    Watson fabric of the configured concentration about fabric_axis."""
-    from . import sweep
-    return sweep.disk_build(cfg["diameter_mm"] * 1e-3, cfg["thickness_mm"] * 1e-3, h,
-                            int(cfg["n_grains"]), int(cfg["seed"]),
-                            kappa=float(cfg["concentration"]),
-                            fabric_axis=tuple(cfg["fabric_axis"]))
+    """The disk of the config at spacing h (sweep_runner.py:290-303): the
+    Laguerre-Voronoi DiskSpecimen with the configured grain count, size CV,
+    Watson concentration about fabric_axis, spatial correlation and seed."""
+    from .disk_specimen import DiskSpecimen
+    sp = DiskSpecimen(diameter_m=cfg["diameter_mm"] * 1e-3, thickness_m=cfg["thickness_mm"] * 1e-3,
+                      n_grains=int(cfg["n_grains"]), size_cv=float(cfg["size_cv"]),
+                      concentration=float(cfg["concentration"]),
+                      spatial_corr=float(cfg["spatial_corr"]),
+                      fabric_axis=tuple(cfg["fabric_axis"]), seed=int(cfg["seed"]))
+    return sp.build(h, adjacency=False)
 
 
 def run(root, cfg, build=None, mem_budget_gb=None, graph_steps=50, log=print, should_stop=None):

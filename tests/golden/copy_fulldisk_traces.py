@@ -1,8 +1,15 @@
 """Copy the upstream's full-disk reference traces into tests/golden/reference/
-as data: sim/ref/fw_fabric{00,25,50,100}.npz (2 MHz label-path traces of
-ladder.standard_build() with the four fabric sets of ladder.standard_fabrics,
-written by sim/fw_checks/fw_reference_2mhz.py; keys trace, dt, f0, E1,
-level_db, frac, h, order, ppw, seed_specimen, seed_fabric, fluid_damp).
+as data:
+
+* sim/ref/fw_fabric{00,25,50,100}.npz -- 2 MHz label-path traces of
+  ladder.standard_build() with the four fabric sets of
+  ladder.standard_fabrics, written by sim/fw_checks/fw_reference_2mhz.py
+  (keys trace, dt, f0, E1, level_db, frac, h, order, ppw, seed_specimen,
+  seed_fabric, fluid_damp);
+* sim/ref/fw_az{17,31,47,62,76,90}.npz -- the same disk rotated by
+  rotation_test.rotated_grid, written by sim/pipeline/azimuth_sweep.py:
+  104-121 (keys trace, dt, f0, rot_deg, level_db = clean-window envelope
+  RMS over E1 in dB, fluid_damp).
 
     python tests/golden/copy_fulldisk_traces.py
 """
@@ -15,9 +22,11 @@ SRC = "/root/reference/sim/ref"
 
 
 def main():
-    for tag in ("00", "25", "50", "100"):
-        d = np.load(os.path.join(SRC, "fw_fabric%s.npz" % tag), allow_pickle=False)
-        np.savez_compressed(os.path.join(HERE, "reference", "fw_fabric%s.npz" % tag),
+    names = ["fw_fabric%s" % t for t in ("00", "25", "50", "100")]
+    names += ["fw_az%02d" % r for r in (17, 31, 47, 62, 76, 90)]
+    for name in names:
+        d = np.load(os.path.join(SRC, name + ".npz"), allow_pickle=False)
+        np.savez_compressed(os.path.join(HERE, "reference", name + ".npz"),
                             **{k: d[k] for k in d.files})
 
 

@@ -316,3 +316,33 @@ def validate_setup(f0, build, ppw=6.0, sponge=10, fluid_damp=0.02, coeffs=None):
            if dy * dy + dz * dz <= er * er and lab[cz + dz, cy + dy, ixp] >= 0]
     return dict(lab=lab, nd=nd, h=h, coeffs=co, dt=dt, nt=nt, damp_mask=dm, pts=pts,
                 w=1.0 / len(pts), wavelet=host.ricker(f0, dt, nt), sponge=sponge)
+
+
+def azimuth_setup(build, rot, h=C_REF / 2e6 / 6.0, sponge=10):
+    """The upstream's rotated-disk reference traces sim/ref/fw_az{rot}.npz
+    (sim/pipeline/azimuth_sweep.py:60-95): the build resampled by
+    rotation_test.rotated_grid at rot degrees, the legacy single-start kh 1.6
+    coefficients and the pre-'rigid2' axes convention axes @ R.T (c-axes
+    turned by +rot; rotation_test.py:68-76 records the later fix) -- both
+    fixed by the archives' stored dt, which this recipe reproduces -- then
+    the same dt, record, Ricker, couplant mask and rim probe as the
+    full-disk recipe."""
+    from pulse_echo_azimuthal_simulation_amd import sweep
+    lab, nd, m, _ = sweep.rotated_grid(build, h, sponge, rot)
+    a = np.radians(rot)
+    R = np.array([[np.cos(a), -np.sin(a), 0.0], [np.sin(a), np.cos(a), 0.0], [0.0, 0.0, 1.0]])
+    axes = np.asarray(build["axes"], float) @ R.T
+    nz, nx = lab.shape[0], lab.shape[1]
+    co = host.optimised_coeffs(8, kh_max=1.6, multistart=False)
+    Ct, rho_t = materials.material_tables(axes)
+    dt = host.safe_dt_labels(lab + 1, Ct, rho_t, h, co, safety=0.5)
+    nt = int(2.6 * nd * h / C_REF / dt)
+    cz, cy, cx = nz // 2, nx // 2, nx // 2
+    Rc = nd // 2
+    ixp = next(cx + k - 1 for k in range(Rc - 1, 0, -1) if lab[cz, cy, cx + k] >= 0)
+    er = max(int(6.35e-3 / 2 / h), 1)
+    pts = [(cz + dz, cy + dy, ixp) for dy in range(-er, er + 1) for dz in range(-er, er + 1)
+           if dy * dy + dz * dz <= er * er and lab[cz + dz, cy + dy, ixp] >= 0]
+    return dict(lab=lab, axes=axes, nd=nd, h=h, coeffs=co, dt=dt, nt=nt,
+                damp_mask=host.fluid_damp_mask(lab, 0.02), pts=pts, w=1.0 / len(pts),
+                wavelet=host.ricker(2e6, dt, nt), sponge=sponge)
