@@ -82,9 +82,13 @@ def schoenberg_muir(Cs, fs) -> np.ndarray:
 
 
 def _rotation_to_x3(normal) -> np.ndarray:
-    """Rotation R with R @ normal = e3 (Rodrigues)."""
+    """Rotation R with R @ normal = e3 (Rodrigues).  A normal already of
+    unit length to 1e-12 is used as given (the rounding of its components
+    carries into R, as in tilt_testbed.py:157-166)."""
     n = np.asarray(normal, float)
-    n = n / np.linalg.norm(n)
+    nn = np.linalg.norm(n)
+    if abs(nn - 1.0) > 1e-12:
+        n = n / nn
     e3 = np.array([0.0, 0.0, 1.0])
     v = np.cross(n, e3)
     s, c = np.linalg.norm(v), float(n @ e3)

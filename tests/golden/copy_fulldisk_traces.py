@@ -9,7 +9,10 @@ as data:
 * sim/ref/fw_az{17,31,47,62,76,90}.npz -- the same disk rotated by
   rotation_test.rotated_grid, written by sim/pipeline/azimuth_sweep.py:
   104-121 (keys trace, dt, f0, rot_deg, level_db = clean-window envelope
-  RMS over E1 in dB, fluid_damp).
+  RMS over E1 in dB, fluid_damp);
+* sim/results/tilt_table.npz -- Table 2 of the manuscript
+  (sim/runs/tilt_table_run.py): per treatment and ppw the mean absolute
+  tilt error (dB), the per-tilt errors and the echo amplitudes.
 
     python tests/golden/copy_fulldisk_traces.py
 """
@@ -24,9 +27,11 @@ SRC = "/root/reference/sim/ref"
 def main():
     names = ["fw_fabric%s" % t for t in ("00", "25", "50", "100")]
     names += ["fw_az%02d" % r for r in (17, 31, 47, 62, 76, 90)]
-    for name in names:
-        d = np.load(os.path.join(SRC, name + ".npz"), allow_pickle=False)
-        np.savez_compressed(os.path.join(HERE, "reference", name + ".npz"),
+    srcs = [os.path.join(SRC, n + ".npz") for n in names]
+    srcs.append("/root/reference/sim/results/tilt_table.npz")
+    for src in srcs:
+        d = np.load(src, allow_pickle=False)
+        np.savez_compressed(os.path.join(HERE, "reference", os.path.basename(src)),
                             **{k: d[k] for k in d.files})
 
 

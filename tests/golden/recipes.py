@@ -346,3 +346,63 @@ def azimuth_setup(build, rot, h=C_REF / 2e6 / 6.0, sponge=10):
     return dict(lab=lab, axes=axes, nd=nd, h=h, coeffs=co, dt=dt, nt=nt,
                 damp_mask=host.fluid_damp_mask(lab, 0.02), pts=pts, w=1.0 / len(pts),
                 wavelet=host.ricker(2e6, dt, nt), sponge=sponge)
+
+
+# ------------------------------------------------------------ tilt table
+TILTS = (0.0, 15.0, 22.5, 30.0, 45.0)
+
+
+def qp_speed_ice(psi):
+    """qP phase speed of ice at angle psi from the c-axis: the transversely
+    isotropic Christoffel root (the closed form of anisotropy.py:265-275)
+    rho v^2 = (A + sqrt(B^2 + 4 (C13 + C44)^2 s^2 c^2)) / 2 with
+    A = (C11 + C44) s^2 + (C33 + C44) c^2, B = (C11 - C44) s^2 - (C33 - C44) c^2."""
+    m = materials.ICE
+    s2, c2 = np.sin(psi) ** 2, np.cos(psi) ** 2
+    A = (m["C11"] + m["C44"]) * s2 + (m["C33"] + m["C44"]) * c2
+    B = (m["C11"] - m["C44"]) * s2 - (m["C33"] - m["C44"]) * c2
+    return float(np.sqrt(0.5 * (A + np.sqrt(B * B + 4 * (m["C13"] + m["C44"]) ** 2 * s2 * c2))
+                         / m["rho"]))
+
+
+def echo_amp(tr_bi, tr_hom, dt, d, v, half_us=0.5):
+    """Interface-echo amplitude of the A-B difference (tilt_testbed.py:
+    237-243): envelope maximum of bi - hom within +-half_us of the two-way
+    time 2 d / v plus the Ricker delay 1.2 / f0."""
+    from scipy.signal import hilbert
+    e = np.abs(hilbert(np.asarray(tr_bi, float) - np.asarray(tr_hom, float)))
+    fs = 1.0 / dt
+    i0 = int((2 * d / v + 1.2 / F0) * fs)
+    w = int(half_us * 1e-6 * fs)
+    return float(e[max(i0 - w, 0):i0 + w].max())
+
+
+def tilt_case(theta, ppw, treatment, homog, psi_a=0.0, psi_b=51.0):
+    """Inputs of one tilted-bicrystal run of the Table-2 testbed
+    (tilt_testbed.py:183-226, sim/runs/tilt_table_run.py:74-98) built with
+    the product's interface treatments: the twoway geometry, both c-axes
+    turned with the plane, the near-half volume fraction binned to 21
+    labels, the treatment's table (the homogeneous reference has grain A on
+    both sides) and the bicrystal's own dt for both runs."""
+    from pulse_echo_azimuthal_simulation_amd import treatments as T
+    g = twoway_geometry(theta, ppw)
+    R, nrm = g["R"], g["nrm"]
+    ca = R @ np.array([np.cos(np.radians(psi_a)), 0.0, np.sin(np.radians(psi_a))])
+    cb = R @ np.array([np.cos(np.radians(psi_b)), 0.0, np.sin(np.radians(psi_b))])
+    CA, CB = grain_C(ca), grain_C(cb)
+    lab = tilt_labels(theta, ppw)
+    Ct, rho_t = T.interface_tables(CA, CA if homog else CB, nrm, treatment, nbin=NBIN)
+    Cbi, rbi = T.interface_tables(CA, CB, nrm, treatment, nbin=NBIN)
+    dt = host.safe_dt_labels(lab + 1, Cbi, rbi, g["h"], coeffs_kh2(), safety=0.5)
+    nt = int(7.0e-6 / dt)
+    c = g["c"]
+    return dict(lab=lab, Ct=Ct, rho_t=rho_t, h=g["h"], dt=dt, nt=nt, d=g["d"],
+                wavelet=host.ricker(F0, dt, nt), src=[((c, c, c), 1.0)],
+                groups=[([(c, c, c)], np.array([1.0]))])
+
+
+@functools.lru_cache(maxsize=5)
+def tilt_labels(theta, ppw):
+    from pulse_echo_azimuthal_simulation_amd import treatments as T
+    g = twoway_geometry(theta, ppw)
+    return T.fraction_labels(T.fraction_field((g["N"],) * 3, g["h"], g["nrm"], g["x0"]), NBIN)
