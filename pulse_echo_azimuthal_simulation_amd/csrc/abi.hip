@@ -392,6 +392,17 @@ int pe_check_shot(const pe_shot *p)
         }
         if (p->kernel == PE_KERNEL_TILED && !p->cpml_corr)
             return fail(PE_EINVAL, "a tiled CPML shot needs cpml_corr (6 x ncell)");
+        // the tiled velocity kernel addresses the memory variables by 32-bit
+        // byte offsets
+        const int64_t W = 2 * (int64_t)p->cpml_width + 2;
+        const int64_t vol[3] = {W * p->ny * p->nx, (int64_t)p->nz * W * p->nx,
+                                (int64_t)p->nz * p->ny * W};
+        for (int a = 0; a < 3; ++a)
+            if (p->kernel == PE_KERNEL_TILED && 6 * vol[a] * rbytes >= ((int64_t)1 << 32))
+                return fail(PE_ERANGE,
+                            "CPML memory variables of axis %d take %lld bytes: the tiled "
+                            "kernels need them under 4 GiB; use PE_KERNEL_DIRECT",
+                            a, (long long)(6 * vol[a] * rbytes));
     }
     return PE_OK;
 }

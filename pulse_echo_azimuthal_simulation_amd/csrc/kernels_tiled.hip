@@ -247,6 +247,15 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
         }
         lds_barrier();
 
+        // The guarded paths' column and table inputs, re-read every step:
+        // left loop-invariant, the compiler hoists each per-lane edge mask
+        // and table flag they derive out of the loop, and the ~140 scalar
+        // registers that takes spill into vector-register lanes, whose
+        // reloads then cost the fast path dozens of instructions a step.
+        int gtid = tid;
+        uint64_t gact = S.act;
+        __asm__ volatile("" : "+v"(gtid), "+s"(gact));
+        const int gix = x0 - 1 + gtid % GX, giy = y0 - 1 + gtid / GX;
         // (2) strains of plane zc at every column of the tile and its ring
         const int slot = (zc + 3) % 3;
         {
@@ -256,7 +265,7 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
                                           ix, iy, zc, a);
             else if (col_in && plane_in)
                 strains_col<true, R, VX>(S, vt + vc, vt + NV + vc, vt + 2 * NV + vc, rx, ry, rz,
-                                         ix, iy, zc, a);
+                                         gix, giy, zc, a);
             // CPML band cell: this plane's strain corrections (the components
             // its bands correct)
             if (CP && L > 0 && col_in && plane_in && (band_xy || pml_in(zc, nz, L))) {
@@ -286,18 +295,19 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
         // (3) stress of plane zc - 1 from the strain planes zc-2, zc-1, zc
         if (out) {
             const int sl[3] = {(zc + 1) % 3, (zc + 2) % 3, slot};
-            const bool fast = full && inner_xy && zo >= 1 && zo <= nz - 2;
+            const bool fast = full && inner_xy && zo >= 1 && zo <= nz - 2;   // wave-uniform
             const int mu = __builtin_amdgcn_readfirstlane(m);
             if (fast && __all(m == mu)) {
                 // every lane of the wave in one grain: wave-uniform row
                 ConstRow<R> cm = (ConstRow<R>)(S.Ct + (size_t)mu * 36);
-                stress_cell<false, true, R, GX, NT>(S, gt, sl, tid, zo, iy, ix, cm, dmp, sv);
+                stress_cell<false, true, R, GX, NT>(S, gt, sl, tid, zo, iy, ix, cm, dmp, sv,
+                                                    gact);
             } else if (fast) {
                 stress_cell<false, true, R, GX, NT>(S, gt, sl, tid, zo, iy, ix,
-                                                    S.Ct + (size_t)m * 36, dmp, sv);
+                                                    S.Ct + (size_t)m * 36, dmp, sv, gact);
             } else {
-                stress_cell<true, false, R, GX, NT>(S, gt, sl, tid, zo, iy, ix,
-                                                    S.Ct + (size_t)m * 36, dmp, sv);
+                stress_cell<true, false, R, GX, NT>(S, gt, sl, tid, zo, giy, gix,
+                                                    S.Ct + (size_t)m * 36, dmp, sv, gact);
             }
 #pragma unroll
             for (int k = 0; k < 6; ++k) stg(S.s[k], io, sv[k]);
@@ -487,13 +497,15 @@ __device__ __forceinline__ void stress_pipe_body(const Shot<R> &S, int tiles_x, 
             if (fast && __all(m == mu)) {
                 ConstRow<R> cm = (ConstRow<R>)(S.Ct + (size_t)mu * 36);
                 stress_cell<false, true, R, GX, NT, ConstRow<R>, 4>(S, gt, sl, tid, zo, iy, ix, cm,
-                                                                     dmp, sv);
+                                                                     dmp, sv, S.act);
             } else if (fast) {
                 stress_cell<false, true, R, GX, NT, const R *, 4>(S, gt, sl, tid, zo, iy, ix,
-                                                                   S.Ct + (size_t)m * 36, dmp, sv);
+                                                                   S.Ct + (size_t)m * 36, dmp, sv,
+                                                                   S.act);
             } else {
                 stress_cell<true, false, R, GX, NT, const R *, 4>(S, gt, sl, tid, zo, iy, ix,
-                                                                   S.Ct + (size_t)m * 36, dmp, sv);
+                                                                   S.Ct + (size_t)m * 36, dmp, sv,
+                                                                   S.act);
             }
 #pragma unroll
             for (int k = 0; k < 6; ++k) stg(S.s[k], io, sv[k]);
@@ -588,9 +600,6 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
     const int ix = x0 + lx, iy = y0 + ly;
     const bool col_in = ix < nx && iy < ny;
     const int L = CP ? S.pml : 0, W = 2 * L + 2;
-    const bool bx = L > 0 && col_in && pml_in(ix, nx, L);
-    const bool by = L > 0 && col_in && pml_in(iy, ny, L);
-    const int bix = ix <= L ? ix : ix - (nx - W), biy = iy <= L ? iy : iy - (ny - W);
     const int col = clampi(iy, 0, ny - 1) * sy + clampi(ix, 0, nx - 1);
     const int vc = (ly + 4) * VX + (lx + 4);
 
@@ -681,6 +690,17 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
 
     auto step = [&](const int z) __attribute__((always_inline)) {
         const int i = z * sz + col;
+        // the guarded and band paths' column inputs, re-derived every step
+        // from a copy of the thread id the compiler cannot see through (else
+        // it hoists their per-lane masks out of the loop and spills them;
+        // see stress_tiled_body)
+        int gtid = tid;
+        __asm__ volatile("" : "+v"(gtid));
+        const int gix = x0 + gtid % TX, giy = y0 + gtid / TX;
+        const bool gcol = gix < nx && giy < ny;
+        const bool gbx = L > 0 && gcol && pml_in(gix, nx, L);
+        const bool gby = L > 0 && gcol && pml_in(giy, ny, L);
+        const int gbix = gix <= L ? gix : gix - (nx - W), gbiy = giy <= L ? giy : giy - (ny - W);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             r3[k] = r3[k + 1];
@@ -692,28 +712,30 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
         r5[8] = p5;
         // CPML band cell: memory variables 3..5 of its band axes, loaded
         // ahead of the step's prefetches
-        const bool bz = CP && L > 0 && col_in && pml_in(z, nz, L);
+        const bool bz = CP && L > 0 && gcol && pml_in(z, nz, L);
+        // (memory-variable offsets in 32 bits: the host caps each axis's
+        // 6 x band-volume array under 4 GiB)
         R ps[9];
-        size_t fz = 0, fy = 0, fx = 0, vz_ = 0, vy_ = 0, vx_ = 0;
+        int fz = 0, fy = 0, fx = 0, vz_ = 0, vy_ = 0, vx_ = 0;
         if constexpr (CP) {
             if (bz) {
                 const int biz = z <= L ? z : z - (nz - W);
-                vz_ = (size_t)W * ny * nx;
-                fz = ((size_t)biz * ny + iy) * nx + ix;
+                vz_ = W * ny * nx;
+                fz = (biz * ny + giy) * nx + gix;
 #pragma unroll
-                for (int k = 0; k < 3; ++k) ps[k] = S.psi[0][(3 + k) * vz_ + fz];
+                for (int k = 0; k < 3; ++k) ps[k] = ldg(S.psi[0], (3 + k) * vz_ + fz);
             }
-            if (by) {
-                vy_ = (size_t)nz * W * nx;
-                fy = ((size_t)z * W + biy) * nx + ix;
+            if (gby) {
+                vy_ = nz * W * nx;
+                fy = (z * W + gbiy) * nx + gix;
 #pragma unroll
-                for (int k = 0; k < 3; ++k) ps[3 + k] = S.psi[1][(3 + k) * vy_ + fy];
+                for (int k = 0; k < 3; ++k) ps[3 + k] = ldg(S.psi[1], (3 + k) * vy_ + fy);
             }
-            if (bx) {
-                vx_ = (size_t)nz * ny * W;
-                fx = ((size_t)z * ny + iy) * W + bix;
+            if (gbx) {
+                vx_ = nz * ny * W;
+                fx = (z * ny + giy) * W + gbix;
 #pragma unroll
-                for (int k = 0; k < 3; ++k) ps[6 + k] = S.psi[2][(3 + k) * vx_ + fx];
+                for (int k = 0; k < 3; ++k) ps[6 + k] = ldg(S.psi[2], (3 + k) * vx_ + fx);
             }
         }
         // (1) plane z of s1, s2, s4, s5, s6 into LDS
@@ -748,9 +770,9 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
                                         ix, iy, z, ax, ay, az);
             else
                 accel_col<true, R, VX>(S, t1 + vc, t2 + vc, t4 + vc, t5 + vc, t6 + vc, r3, r4, r5,
-                                       ix, iy, z, ax, ay, az);
+                                       gix, giy, z, ax, ay, az);
             if constexpr (CP) {
-                if (bz || by || bx) {
+                if (bz || gby || gbx) {
                     // corrections of the band axes z, y, x (pml_accel_corr)
                     const R *p1 = t1 + vc, *p2 = t2 + vc, *p4 = t4 + vc, *p5 = t5 + vc,
                             *p6 = t6 + vc;
@@ -772,44 +794,44 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
                         g1 = g1 + pml_corr_lds<0, 0>(bp, W, biz, ps[1], e1);
                         g2 = g2 + pml_corr_lds<0, 1>(bp, W, biz, ps[2], e2);
 #pragma unroll
-                        for (int k = 0; k < 3; ++k) S.psi[0][(3 + k) * vz_ + fz] = ps[k];
+                        for (int k = 0; k < 3; ++k) stg(S.psi[0], (3 + k) * vz_ + fz, ps[k]);
                     }
-                    if (by) {
+                    if (gby) {
                         R e0 = 0, e1 = 0, e2 = 0;
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
                             if (j >= S.nc) break;
                             const R cj = S.c[j];
-                            if (iy >= j + 1 && iy < ny - j) {
+                            if (giy >= j + 1 && giy < ny - j) {
                                 e0 = fma_r(cj, p6[j * VX] - p6[-(j + 1) * VX], e0);
                                 e2 = fma_r(cj, p4[j * VX] - p4[-(j + 1) * VX], e2);
                             }
-                            if (iy >= j && iy < ny - j - 1)
+                            if (giy >= j && giy < ny - j - 1)
                                 e1 = fma_r(cj, p2[(j + 1) * VX] - p2[-j * VX], e1);
                         }
-                        g0 = g0 + pml_corr_lds<1, 0>(bp, W, biy, ps[3], e0);
-                        g1 = g1 + pml_corr_lds<1, 1>(bp, W, biy, ps[4], e1);
-                        g2 = g2 + pml_corr_lds<1, 0>(bp, W, biy, ps[5], e2);
+                        g0 = g0 + pml_corr_lds<1, 0>(bp, W, gbiy, ps[3], e0);
+                        g1 = g1 + pml_corr_lds<1, 1>(bp, W, gbiy, ps[4], e1);
+                        g2 = g2 + pml_corr_lds<1, 0>(bp, W, gbiy, ps[5], e2);
 #pragma unroll
-                        for (int k = 0; k < 3; ++k) S.psi[1][(3 + k) * vy_ + fy] = ps[3 + k];
+                        for (int k = 0; k < 3; ++k) stg(S.psi[1], (3 + k) * vy_ + fy, ps[3 + k]);
                     }
-                    if (bx) {
+                    if (gbx) {
                         R e0 = 0, e1 = 0, e2 = 0;
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
                             if (j >= S.nc) break;
                             const R cj = S.c[j];
-                            if (ix >= j && ix < nx - j - 1) e0 = fma_r(cj, p1[j + 1] - p1[-j], e0);
-                            if (ix >= j + 1 && ix < nx - j) {
+                            if (gix >= j && gix < nx - j - 1) e0 = fma_r(cj, p1[j + 1] - p1[-j], e0);
+                            if (gix >= j + 1 && gix < nx - j) {
                                 e1 = fma_r(cj, p6[j] - p6[-(j + 1)], e1);
                                 e2 = fma_r(cj, p5[j] - p5[-(j + 1)], e2);
                             }
                         }
-                        g0 = g0 + pml_corr_lds<2, 1>(bp, W, bix, ps[6], e0);
-                        g1 = g1 + pml_corr_lds<2, 0>(bp, W, bix, ps[7], e1);
-                        g2 = g2 + pml_corr_lds<2, 0>(bp, W, bix, ps[8], e2);
+                        g0 = g0 + pml_corr_lds<2, 1>(bp, W, gbix, ps[6], e0);
+                        g1 = g1 + pml_corr_lds<2, 0>(bp, W, gbix, ps[7], e1);
+                        g2 = g2 + pml_corr_lds<2, 0>(bp, W, gbix, ps[8], e2);
 #pragma unroll
-                        for (int k = 0; k < 3; ++k) S.psi[2][(3 + k) * vx_ + fx] = ps[6 + k];
+                        for (int k = 0; k < 3; ++k) stg(S.psi[2], (3 + k) * vx_ + fx, ps[6 + k]);
                     }
                     ax = ax + g0;
                     ay = ay + g1;

@@ -136,7 +136,8 @@ __device__ __forceinline__ void load6(const __attribute__((address_space(4))) R 
 }
 
 template <int I, bool FULL, typename R, typename Row>
-__device__ __forceinline__ void stress_out(const Shot<R> &S, Row cm, const R (&q)[6], R dmp, R &sv)
+__device__ __forceinline__ void stress_out(const Shot<R> &S, Row cm, const R (&q)[6], R dmp, R &sv,
+                                           uint64_t act)
 {
     R row[6];
     load6(cm + 6 * I, row);
@@ -144,7 +145,7 @@ __device__ __forceinline__ void stress_out(const Shot<R> &S, Row cm, const R (&q
     bool any = FULL;
 #pragma unroll
     for (int J = 0; J < 6; ++J) {
-        if (FULL || ((S.act >> (6 * I + J)) & 1ull)) {
+        if (FULL || ((act >> (6 * I + J)) & 1ull)) {
             acc = fma_r(row[J], q[J], acc);
             any = true;
         }
@@ -235,23 +236,25 @@ __device__ __forceinline__ R pml_corr_lds(const R *bp, int W, int bi, R &psi, R 
 template <typename R>
 using ConstRow = const __attribute__((address_space(4))) R *;
 
+// ``act`` is the shot's active-entry mask (S.act), passed in so that a caller
+// can hand over a copy the compiler does not hoist (see stress_tiled_body).
 template <bool G, bool FULL, typename R, int GX, int NG, typename Row, int NS = 3>
 __device__ __forceinline__ void stress_cell(const Shot<R> &S, const R *__restrict__ gt,
                                             const int (&sl)[3], int c, int iz, int iy, int ix,
-                                            Row cm, R dmp, R (&sv)[6])
+                                            Row cm, R dmp, R (&sv)[6], uint64_t act)
 {
     const int nz = S.nz, ny = S.ny, nx = S.nx;
     R q[6];
     gather_lds<G, 0, R, GX, NG, NS>(gt, sl, c, iz, iy, ix, nz, ny, nx, q);
-    stress_out<0, FULL>(S, cm, q, dmp, sv[0]);
-    stress_out<1, FULL>(S, cm, q, dmp, sv[1]);
-    stress_out<2, FULL>(S, cm, q, dmp, sv[2]);
+    stress_out<0, FULL>(S, cm, q, dmp, sv[0], act);
+    stress_out<1, FULL>(S, cm, q, dmp, sv[1], act);
+    stress_out<2, FULL>(S, cm, q, dmp, sv[2], act);
     gather_lds<G, 1, R, GX, NG, NS>(gt, sl, c, iz, iy, ix, nz, ny, nx, q);
-    stress_out<3, FULL>(S, cm, q, dmp, sv[3]);
+    stress_out<3, FULL>(S, cm, q, dmp, sv[3], act);
     gather_lds<G, 2, R, GX, NG, NS>(gt, sl, c, iz, iy, ix, nz, ny, nx, q);
-    stress_out<4, FULL>(S, cm, q, dmp, sv[4]);
+    stress_out<4, FULL>(S, cm, q, dmp, sv[4], act);
     gather_lds<G, 3, R, GX, NG, NS>(gt, sl, c, iz, iy, ix, nz, ny, nx, q);
-    stress_out<5, FULL>(S, cm, q, dmp, sv[5]);
+    stress_out<5, FULL>(S, cm, q, dmp, sv[5], act);
 }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
