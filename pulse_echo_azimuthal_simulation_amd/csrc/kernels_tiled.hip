@@ -47,6 +47,7 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
     R *vt = reinterpret_cast<R *>(lds_raw);       // [3][VY][VX]
     R *gt = vt + 3 * NV;                           // [6][3][NT]: strains by thread
     R *fr = gt + 6 * 3 * NT;                       // FR: [4][NF][3][NT] frame ring
+    R *cb = fr + T::NRING;                         // CP, CORR_LDS: [2][6][NT] corrections
 
     const int nz = S.nz, ny = S.ny, nx = S.nx;
     const int sy = nx, sz = nx * ny;
@@ -169,8 +170,33 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
             }
     }
 
+    // CPML (float): the strain corrections of plane zp, which the band
+    // pre-pass wrote into S.corr, fetched one step ahead straight into LDS
+    // (slot [zp & 1][k][tid]; LDS-DMA loads take no registers) for the
+    // components any lane of the wave needs.  Each lane reads back only its
+    // own slots, after the wait at the top of the next step.
+    constexpr bool CPD = CP && T::CORR_LDS;
+    auto corr_dma = [&](int zp) {
+        if constexpr (CPD) {
+            if (L <= 0 || zp < 0 || zp >= nz) return;
+            const bool zb = pml_in(zp, nz, L);
+            const int lm = col_in && (band_xy || zb) ? mask_xy | (zb ? kPmlMaskZ : 0) : 0;
+            const size_t n = (size_t)nz * sz;
+            const R *src = S.corr + (size_t)zp * sz + col;
+            R *dst = cb + (zp & 1) * 6 * NT + (tid & ~63);
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                if (__any(lm >> k & 1))
+                    __builtin_amdgcn_global_load_lds(
+                        src + k * n, (__attribute__((address_space(3))) R *)(dst + k * NT), 4, 0,
+                        0);
+        }
+    };
+    corr_dma(z0 - 1);
+
     for (int zc = z0 - 1; zc <= z1; ++zc) {
         const bool plane_in = zc >= 0 && zc < nz;
+        if constexpr (CPD) wait_vmem();     // this plane's corrections are in LDS
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             rx[k] = rx[k + 1];
@@ -245,6 +271,7 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
             py = ldg(vy, i);
             pz = ldg(vz, i);
         }
+        if (zc < z1) corr_dma(zc + 1);
         lds_barrier();
 
         // The guarded paths' column and table inputs, re-read every step:
@@ -270,11 +297,18 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
             // its bands correct)
             if (CP && L > 0 && col_in && plane_in && (band_xy || pml_in(zc, nz, L))) {
                 const int mask = mask_xy | (pml_in(zc, nz, L) ? kPmlMaskZ : 0);
-                const size_t n = (size_t)nz * sz;
-                const int ig = zc * sz + col;
+                if constexpr (CPD) {
+                    const R *cs = cb + (zc & 1) * 6 * NT + tid;
 #pragma unroll
-                for (int k = 0; k < 6; ++k)
-                    if (mask >> k & 1) a[k] = a[k] + ldg(S.corr + k * n, ig);
+                    for (int k = 0; k < 6; ++k)
+                        if (mask >> k & 1) a[k] = a[k] + cs[k * NT];
+                } else {
+                    const size_t n = (size_t)nz * sz;
+                    const int ig = zc * sz + col;
+#pragma unroll
+                    for (int k = 0; k < 6; ++k)
+                        if (mask >> k & 1) a[k] = a[k] + ldg(S.corr + k * n, ig);
+                }
             }
             R *gs = gt + slot * NT + tid;
 #pragma unroll
@@ -1069,12 +1103,13 @@ hipError_t stress_launch(const Shot<R> &s, int zchunk, hipStream_t st, bool stor
 {
     using T = StressTile<R, TX, TY, PAD, FR>;
     const int tiles_x = (s.nx + TX - 1) / TX, tiles_y = (s.ny + TY - 1) / TY;
+    const size_t lds = s.pml > 0 ? T::LDS_CP : T::LDS;
     auto run = [&](auto kern) -> hipError_t {
         const int zl = chunk_len(s.nz, tiles_x * tiles_y, zchunk);
         const int nb = tiles_x * tiles_y * ((s.nz + zl - 1) / zl);
-        hipError_t e = allow_lds(kern, T::LDS);
+        hipError_t e = allow_lds(kern, lds);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(kern, dim3(nb), dim3(T::NT), T::LDS, st, s, tiles_x, tiles_y, zl, 0,
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(T::NT), lds, st, s, tiles_x, tiles_y, zl, 0,
                            s.nz);
         return hipGetLastError();
     };
@@ -1138,10 +1173,11 @@ hipError_t stress_batched_launch(const BatchEntry<R> *tab, const int32_t *blk, i
                                  const int32_t *step, hipStream_t st, bool cp)
 {
     using T = StressTile<R, TX, TY, PAD, FR>;
+    const size_t lds = cp ? T::LDS_CP : T::LDS;
     auto go = [&](auto kern) -> hipError_t {
-        hipError_t e = allow_lds(kern, T::LDS);
+        hipError_t e = allow_lds(kern, lds);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(kern, dim3(nb), dim3(T::NT), T::LDS, st, tab, blk, nb, step);
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(T::NT), lds, st, tab, blk, nb, step);
         return hipGetLastError();
     };
     return cp ? go(k_stress_batched<R, TX, TY, PAD, FR, true>)

@@ -56,7 +56,17 @@ struct StressTile {
     static constexpr int NF = (NFRAME + NT - 1) / NT;              // frame entries per thread
     static constexpr int NRING = FR ? 4 * NF * 3 * NT : 0;         // frame ring words
     static constexpr size_t LDS = sizeof(R) * (3 * NV + 6 * 3 * NT + NRING);
+    // with a CPML (float): the strain corrections of two planes, [2][6][NT],
+    // for the tiles where that stays within 96 KiB (a 102 KiB request of this
+    // kernel was refused at launch); the others load them directly
+    static constexpr bool CORR_LDS =
+        sizeof(R) == 4 && LDS + sizeof(R) * 12 * NT <= 96 * 1024;
+    static constexpr size_t LDS_CP = LDS + (CORR_LDS ? sizeof(R) * 12 * NT : 0);
 };
+
+// Vector-memory wait for every load of this wave, the LDS-DMA loads
+// included (s_waitcnt vmcnt(0); expcnt and lgkmcnt left at their maxima).
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 // Moved strain q_{P,J} read from the LDS strain ring (see moved_direct in
 // kernels_direct.hip for the rule), laid out [6][NS][NG].  sl[d+1] is the
