@@ -174,7 +174,9 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
     // pre-pass wrote into S.corr, fetched one step ahead straight into LDS
     // (slot [zp & 1][k][tid]; LDS-DMA loads take no registers) for the
     // components any lane of the wave needs.  Each lane reads back only its
-    // own slots, after the wait at the top of the next step.
+    // own slots, after the wait at the top of the next step (and the wait
+    // for in-flight LDS-DMA loads that the compiler puts before a barrier's
+    // release fence).
     constexpr bool CPD = CP && T::CORR_LDS;
     auto corr_dma = [&](int zp) {
         if constexpr (CPD) {
@@ -271,7 +273,6 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
             py = ldg(vy, i);
             pz = ldg(vz, i);
         }
-        if (zc < z1) corr_dma(zc + 1);
         lds_barrier();
 
         // The guarded paths' column and table inputs, re-read every step:
@@ -325,6 +326,13 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
             }
         }
         lds_barrier();
+        // the next plane's corrections: issued here, after this step's
+        // reads of the other slot, so that the release fence of the next
+        // barrier -- which waits for every LDS-DMA load still in flight --
+        // finds them issued a stress stage and a loop latch earlier (issued
+        // before the first barrier, they made it wait for the step's
+        // prefetches too)
+        if (zc < z1) corr_dma(zc + 1);
 
         // (3) stress of plane zc - 1 from the strain planes zc-2, zc-1, zc
         if (out) {
