@@ -27,9 +27,60 @@
  * Reverse step n with adjoints (lv, ls) of (v, s):
  *   E^T (velocity record) ; D^T ; C^T (pressure record) ; B^T (wavelet
  *   gradient) ; A^T with the stiffness gradient taken before ls is damped.
+ * With a CPML the state also holds the memory variables psi, and the
+ * reverse step carries their adjoints (S->cpml_psi points at them): D^T and
+ * A^T each add the transposes of their band cells' corrections (pml_adj,
+ * pml_scatter), and the stiffness gradient reads the strains of step n from
+ * S->g, since they depend on the forward memory variables.
  */
 #define CAT_(a, b) a##b
 #define CAT(a, b) CAT_(a, b)
+
+/* CPML (no upstream counterpart).  A band cell's forward correction of one
+ * derivative dd (fdtd_oracle_body.h, pml_corr) is
+ *   psi' = b psi + a dd,   corr = (1/kappa - 1) dd + psi',
+ * with corr added to the strain (acceleration) the derivative belongs to.
+ * Given the adjoint ebar of that strain (acceleration) and the adjoint pbar
+ * of psi' (carried back from the later step), with P = ebar + pbar:
+ *   adjoint of dd  = (1/kappa - 1) ebar + a P   (returned),
+ *   adjoint of psi = b P                        (stored over pbar).
+ * S->cpml_psi holds the ADJOINT memory variables in the reverse step. */
+static REAL CAT(pml_adj, SUF)(const orc_shot *S, int a, int k, int st, int iz, int iy, int ix,
+                              REAL ebar)
+{
+    const int n = a == 0 ? S->nz : (a == 1 ? S->ny : S->nx);
+    const int ia = a == 0 ? iz : (a == 1 ? iy : ix);
+    const REAL *pr = (const REAL *)S->cpml_prof[a] + (int64_t)st * 3 * n;
+    const int64_t f = (int64_t)k * pml_vol(S, a) + pml_flat(S, a, iz, iy, ix);
+    REAL *pb = (REAL *)S->cpml_psi[a];
+    const REAL P = ebar + pb[f];
+    pb[f] = pr[ia] * P;
+    if (orc_adj_plant == 4) return (pr[2 * n + ia] - (REAL)1) * ebar;
+    return (pr[2 * n + ia] - (REAL)1) * ebar + pr[n + ia] * P;
+}
+
+/* Transpose of one band derivative, pushed into field f: the forward
+ * derivative at cell i along an axis of stride st and length n (ia the
+ * cell's index on it) is sum_j c_j (f(i+j) - f(i-j-1)) for a backward
+ * difference (fwd = 0, kept iff j+1 <= ia < n-j) or sum_j c_j (f(i+j+1) -
+ * f(i-j)) for a forward one (fwd = 1, kept iff j <= ia < n-j-1), times
+ * scale. */
+static void CAT(pml_scatter, SUF)(REAL *f, int64_t i, int ia, int n, int64_t st, int fwd,
+                                  REAL w, const REAL *c, int nc, REAL scale)
+{
+    for (int j = 0; j < nc; ++j) {
+        const REAL cw = c[j] * scale * w;
+        if (!fwd) {
+            if (ia >= j + 1 && ia < n - j) {
+                f[i + j * st] += cw;
+                f[i - (j + 1) * st] -= cw;
+            }
+        } else if (ia >= j && ia < n - j - 1) {
+            f[i + (j + 1) * st] += cw;
+            f[i - j * st] -= cw;
+        }
+    }
+}
 
 /* D^T: t = d * lv; e_a = dt_h * t_a / rho_face_a; ls += D^T e; lv <- t. */
 static void CAT(adj_velocity, SUF)(const orc_shot *S, REAL *lv[3], REAL *ls[6])
@@ -93,6 +144,38 @@ static void CAT(adj_velocity, SUF)(const orc_shot *S, REAL *lv[3], REAL *ls[6])
                         s4[i - (j + 1) * sz] -= cj * ey;
                     }
                 }
+                /* CPML band cell: the transposes of its acceleration
+                 * corrections (pml_accel_corr: z band D_z s5 -> ax, D_z s4 ->
+                 * ay, D_z s3 -> az; y band D_y s6, D_y s2, D_y s4; x band D_x
+                 * s1, D_x s6, D_x s5; memory variables 3..5) */
+                if (pml_cell(S, iz, iy, ix)) {
+                    const int L = S->cpml_width;
+                    const REAL one = (REAL)1;
+                    if (pml_in(iz, nz, L)) {
+                        CAT(pml_scatter, SUF)(s5, i, iz, nz, sz, 0,
+                                              CAT(pml_adj, SUF)(S, 0, 3, 0, iz, iy, ix, ex), c, nc, one);
+                        CAT(pml_scatter, SUF)(s4, i, iz, nz, sz, 0,
+                                              CAT(pml_adj, SUF)(S, 0, 4, 0, iz, iy, ix, ey), c, nc, one);
+                        CAT(pml_scatter, SUF)(s3, i, iz, nz, sz, 1,
+                                              CAT(pml_adj, SUF)(S, 0, 5, 1, iz, iy, ix, ez), c, nc, one);
+                    }
+                    if (pml_in(iy, ny, L)) {
+                        CAT(pml_scatter, SUF)(s6, i, iy, ny, sy, 0,
+                                              CAT(pml_adj, SUF)(S, 1, 3, 0, iz, iy, ix, ex), c, nc, one);
+                        CAT(pml_scatter, SUF)(s2, i, iy, ny, sy, 1,
+                                              CAT(pml_adj, SUF)(S, 1, 4, 1, iz, iy, ix, ey), c, nc, one);
+                        CAT(pml_scatter, SUF)(s4, i, iy, ny, sy, 0,
+                                              CAT(pml_adj, SUF)(S, 1, 5, 0, iz, iy, ix, ez), c, nc, one);
+                    }
+                    if (pml_in(ix, nx, L)) {
+                        CAT(pml_scatter, SUF)(s1, i, ix, nx, 1, 1,
+                                              CAT(pml_adj, SUF)(S, 2, 3, 1, iz, iy, ix, ex), c, nc, one);
+                        CAT(pml_scatter, SUF)(s6, i, ix, nx, 1, 0,
+                                              CAT(pml_adj, SUF)(S, 2, 4, 0, iz, iy, ix, ey), c, nc, one);
+                        CAT(pml_scatter, SUF)(s5, i, ix, nx, 1, 0,
+                                              CAT(pml_adj, SUF)(S, 2, 5, 0, iz, iy, ix, ez), c, nc, one);
+                    }
+                }
             }
     FP_EXIT
 }
@@ -137,7 +220,9 @@ static void CAT(adj_stress, SUF)(const orc_shot *S, REAL *lv[3], REAL *ls[6], do
     const int dims[3] = {nz, ny, nx};
     move_t mv[4][6];
     move_table(mv, nz, ny, nx);
-    CAT(strains, SUF)(S);
+    /* strains of v^n for the gradient; with a CPML they depend on the
+     * forward memory variables too, and the caller passes them in S->g */
+    if (gC && S->cpml_width == 0) CAT(strains, SUF)(S);
     memset(gbar, 0, sizeof(REAL) * 6 * ncell);
     REAL c[4];
     for (int j = 0; j < 4; ++j) c[j] = (REAL)S->coeffs[j];
@@ -183,6 +268,44 @@ static void CAT(adj_stress, SUF)(const orc_shot *S, REAL *lv[3], REAL *ls[6], do
     REAL *vx = lv[0], *vy = lv[1], *vz = lv[2];
     const REAL *b1 = gbar, *b2 = gbar + ncell, *b3 = gbar + 2 * ncell, *b4 = gbar + 3 * ncell,
                *b5 = gbar + 4 * ncell, *b6 = gbar + 5 * ncell;
+    /* CPML band cells: the transposes of their strain corrections
+     * (pml_strain_corr: z band Db_z vz -> g3, Df_z vy -> g4, Df_z vx -> g5;
+     * y band Db_y vy -> g2, Df_y vz -> g4, Df_y vx -> g6; x band Db_x vx ->
+     * g1, Df_x vz -> g5, Df_x vy -> g6; memory variables 0..2, staggers 0, 1,
+     * 1), scaled by 1/h like the strains */
+    if (S->cpml_width > 0) {
+        const int L = S->cpml_width;
+        for (int iz = 0; iz < nz; ++iz)
+            for (int iy = 0; iy < ny; ++iy)
+                for (int ix = 0; ix < nx; ++ix) {
+                    if (!pml_cell(S, iz, iy, ix)) continue;
+                    const int64_t i = ((int64_t)iz * ny + iy) * nx + ix;
+                    if (pml_in(iz, nz, L)) {
+                        CAT(pml_scatter, SUF)(vz, i, iz, nz, sz, 0,
+                                              CAT(pml_adj, SUF)(S, 0, 0, 0, iz, iy, ix, b3[i]), c, nc, inv_h);
+                        CAT(pml_scatter, SUF)(vy, i, iz, nz, sz, 1,
+                                              CAT(pml_adj, SUF)(S, 0, 1, 1, iz, iy, ix, b4[i]), c, nc, inv_h);
+                        CAT(pml_scatter, SUF)(vx, i, iz, nz, sz, 1,
+                                              CAT(pml_adj, SUF)(S, 0, 2, 1, iz, iy, ix, b5[i]), c, nc, inv_h);
+                    }
+                    if (pml_in(iy, ny, L)) {
+                        CAT(pml_scatter, SUF)(vy, i, iy, ny, sy, 0,
+                                              CAT(pml_adj, SUF)(S, 1, 0, 0, iz, iy, ix, b2[i]), c, nc, inv_h);
+                        CAT(pml_scatter, SUF)(vz, i, iy, ny, sy, 1,
+                                              CAT(pml_adj, SUF)(S, 1, 1, 1, iz, iy, ix, b4[i]), c, nc, inv_h);
+                        CAT(pml_scatter, SUF)(vx, i, iy, ny, sy, 1,
+                                              CAT(pml_adj, SUF)(S, 1, 2, 1, iz, iy, ix, b6[i]), c, nc, inv_h);
+                    }
+                    if (pml_in(ix, nx, L)) {
+                        CAT(pml_scatter, SUF)(vx, i, ix, nx, 1, 0,
+                                              CAT(pml_adj, SUF)(S, 2, 0, 0, iz, iy, ix, b1[i]), c, nc, inv_h);
+                        CAT(pml_scatter, SUF)(vz, i, ix, nx, 1, 1,
+                                              CAT(pml_adj, SUF)(S, 2, 1, 1, iz, iy, ix, b5[i]), c, nc, inv_h);
+                        CAT(pml_scatter, SUF)(vy, i, ix, nx, 1, 1,
+                                              CAT(pml_adj, SUF)(S, 2, 2, 1, iz, iy, ix, b6[i]), c, nc, inv_h);
+                    }
+                }
+    }
     for (int iz = 0; iz < nz; ++iz)
         for (int iy = 0; iy < ny; ++iy)
             for (int ix = 0; ix < nx; ++ix) {

@@ -133,7 +133,8 @@ static void move_table(move_t mv[4][6], int nz, int ny, int nx)
 
 /* Planted-mistake switch for the adjoint tests (0 = exact adjoint):
  * 1 drops the face-density average, 2 weights 4-tap moves by 1/2,
- * 3 forgets the damping of the stress adjoint. */
+ * 3 forgets the damping of the stress adjoint, 4 drops the memory variable's
+ * share (a P) of a CPML derivative's adjoint. */
 int orc_adj_plant = 0;
 
 #define REAL float

@@ -200,7 +200,11 @@ class OracleAdjoint:
     ``step(vn, res)`` performs reverse step n given v^n (the forward state at
     the start of step n, three (nz, ny, nx) arrays) and the trace adjoint row
     dJ/dy_n (or None); it accumulates dJ/dCt into ``gC`` (n_mat, 36) float64
-    and returns dJ/dwavelet[n].
+    and returns dJ/dwavelet[n].  With a CPML the adjoint state also holds
+    ``psib``, the adjoints of the memory variables (laid out like
+    ``shot.psi``), and the gradient needs ``gn``, the strains of step n
+    (6, nz, ny, nx) as the forward computed them (they depend on the forward
+    memory variables).
     """
 
     def __init__(self, shot):
@@ -211,14 +215,24 @@ class OracleAdjoint:
         self.gC = np.zeros((len(shot.rho_t), 36))
         self.gbar = np.zeros((6,) + shot.shape, r)
         self._vn = [np.zeros(shot.shape, r) for _ in range(3)]
+        self._gn = np.zeros((6,) + shot.shape, r)
+        self.psib = [np.zeros_like(p) for p in shot.psi]
 
-    def step(self, vn, res=None, grad=True):
+    def step(self, vn, res=None, grad=True, gn=None):
         sh = self.shot
         for k in range(3):
             self._vn[k][...] = vn[k]
         st = _Shot.from_buffer_copy(sh._st)
         for k in range(3):
             st.v[k] = self._vn[k].ctypes.data
+        if self.psib:
+            for a in range(3):
+                st.cpml_psi[a] = self.psib[a].ctypes.data
+            if grad:
+                if gn is None:
+                    raise ValueError("a CPML shot's gradient needs the strains of step n (gn)")
+                self._gn[...] = gn
+                st.g = self._gn.ctypes.data
         gw = C.c_double(0.0)
         resp = None
         if res is not None:
@@ -231,29 +245,38 @@ class OracleAdjoint:
         return gw.value
 
 
-def forward_history(shot, wavelet, nt):
+def forward_history(shot, wavelet, nt, strains=False):
     """Run nt steps from zero fields; returns (traces, v history (nt, 3, ...))
-    where history[n] is v at the start of step n."""
+    where history[n] is v at the start of step n, and with ``strains`` also
+    the strains each step computed, (nt, 6, ...)."""
     hist = np.zeros((nt, 3) + shot.shape, shot.real)
+    ghist = np.zeros((nt, 6) + shot.shape, shot.real) if strains else None
     traces = np.zeros((nt, shot._st.n_rec), shot.real)
     for n in range(nt):
         for k in range(3):
             hist[n, k] = shot.v[k]
         shot.run(wavelet, n, n + 1, traces)
+        if strains:
+            ghist[n] = shot.g
+    if strains:
+        return traces, hist, ghist
     return traces, hist
 
 
 def adjoint_gradient(shot, wavelet, nt, residual_fn):
-    """Misfit gradient by forward (storing v) then the reverse loop.
+    """Misfit gradient by forward (storing v, and with a CPML the strains)
+    then the reverse loop.
 
     ``residual_fn(traces)`` returns (J, dJ/dtraces).  Returns (J, gC (n_mat,
     36), gw (nt,), traces)."""
-    traces, hist = forward_history(shot, wavelet, nt)
+    cp = bool(shot.psi)
+    out = forward_history(shot, wavelet, nt, strains=cp)
+    traces, hist = out[0], out[1]
     J, res = residual_fn(traces)
     adj = OracleAdjoint(shot)
     gw = np.zeros(nt)
     for n in range(nt - 1, -1, -1):
-        gw[n] = adj.step(hist[n], res[n])
+        gw[n] = adj.step(hist[n], res[n], gn=out[2][n] if cp else None)
     return J, adj.gC, gw, traces
 
 
