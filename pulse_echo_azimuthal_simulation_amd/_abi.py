@@ -16,7 +16,7 @@ PE_KERNEL_DIRECT, PE_KERNEL_TILED = 0, 1
 PE_HIST_VELOCITY, PE_HIST_STRAIN = 0, 1
 RECORD_MODES = {"pressure": 0, "vx": 1, "vy": 2, "vz": 3}
 KERNELS = {"direct": PE_KERNEL_DIRECT, "tiled": PE_KERNEL_TILED}
-ABI_VERSION = 9
+ABI_VERSION = 10
 
 EXPORTS = ("pe_abi_version", "pe_last_error", "pe_check_shot", "pe_stress", "pe_source",
            "pe_velocity", "pe_record", "pe_run", "pe_run_timed", "pe_tile_info", "pe_run_store",
@@ -67,6 +67,7 @@ class PeAdjoint(C.Structure):
         ("zbreak", C.c_int32), ("n_runs", C.c_int32),
         ("run_ptr", C.c_void_p), ("mat_ptr", C.c_void_p), ("mat_runs", C.c_void_p),
         ("run_acc", C.c_void_p),
+        ("cpml_psib", C.c_void_p * 3), ("cpml_work", C.c_void_p * 3),
     ]
 
 

@@ -8,6 +8,9 @@ by vendor/ringfwi/axisfield.py:37-161).  Here the label-indexed GPU step of
 
 * the gradient reduced per material (grain) in float64, deterministically
   (per-column material runs, see ``run_tables``);
+* a CPML boundary (direct kernels): the adjoints of the memory variables are
+  carried through the reverse pass, checkpoints hold the memory variables,
+  and the gradient reads the strain history;
 * wavefield checkpointing: the forward pass keeps the state every
   ``checkpoint_every`` steps; each segment is recomputed from its checkpoint
   while a per-step history is stored -- the strains the stress half-step
@@ -98,6 +101,16 @@ class DeviceAdjoint:
         self.zbreak = int(zbreak)
         if hist not in ("velocity", "strain"):
             raise ValueError("hist must be 'velocity' or 'strain'")
+        self.cpml = shot.cpml is not None
+        if self.cpml:
+            # the memory variables make the strains of a step depend on more
+            # than v, so the gradient reads the strains the forward stored
+            # (written by the direct kernels)
+            if kernel != "direct" or shot.desc.kernel != _abi.PE_KERNEL_DIRECT:
+                raise ValueError("the adjoint of a CPML shot runs the direct kernels: build the "
+                                 "DeviceShot with kernel='direct'")
+            if gradient and hist != "strain":
+                raise ValueError("the gradient of a CPML shot needs hist='strain'")
         self.hist = hist
         self.hist_fields = 6 if hist == "strain" else 3
         self.lv = [torch.zeros(shot.shape, dtype=real, device=dev) for _ in range(3)]
@@ -121,7 +134,14 @@ class DeviceAdjoint:
         self.mat_runs = torch.as_tensor(mat_runs, device=dev)
         self.run_acc = (torch.zeros((self.n_runs, self.K), dtype=torch.float64, device=dev)
                         if gradient else None)
+        # CPML: adjoints of the memory variables (laid out like shot.psi) and
+        # the band cells' derivative adjoints
+        self.psib = [torch.zeros_like(p) for p in shot.psi]
+        self.cpml_work = [torch.zeros((3, p.shape[1]), dtype=real, device=dev) for p in shot.psi]
         d = _abi.PeAdjoint()
+        for a in range(len(self.psib)):
+            d.cpml_psib[a] = self.psib[a].data_ptr()
+            d.cpml_work[a] = self.cpml_work[a].data_ptr()
         for k in range(3):
             d.lv[k] = self.lv[k].data_ptr()
         for k in range(6):
@@ -190,17 +210,18 @@ def _segments(nt, every):
 def forward_checkpointed(shot, wav, nt, traces, checkpoint_every=200, history=None,
                          hist_fields=3):
     """Forward run from zero fields.  Returns (checkpoints, stored) where
-    checkpoints[a] is the state (v, s) at the start of segment [a, b) and
-    stored[a] the (b-a, hist_fields, nz, ny, nx) history of a segment kept
-    during this pass (segments listed in ``history``): v (3 fields) or the
-    strains of v (6 fields)."""
+    checkpoints[a] is the state at the start of segment [a, b) -- (v, s), and
+    with a CPML the memory variables too -- and stored[a] the (b-a,
+    hist_fields, nz, ny, nx) history of a segment kept during this pass
+    (segments listed in ``history``): v (3 fields) or the strains of v (6
+    fields)."""
     torch = shot.torch
     shot.reset()
     ckpt, stored = {}, {}
     history = history or ()
     for a, b in _segments(nt, checkpoint_every):
         if a not in history:
-            ckpt[a] = torch.stack(shot.v + shot.s)
+            ckpt[a] = (torch.stack(shot.v + shot.s), [p.clone() for p in shot.psi])
             shot.run(wav, a, b, traces)
         else:
             h = torch.empty((b - a, hist_fields) + shot.shape, dtype=shot.real,
@@ -222,10 +243,13 @@ def _run_storing(shot, wav, a, b, traces, hist):
 
 
 def _restore(shot, state):
+    fields, psi = state
     for k in range(3):
-        shot.v[k].copy_(state[k])
+        shot.v[k].copy_(fields[k])
     for k in range(6):
-        shot.s[k].copy_(state[3 + k])
+        shot.s[k].copy_(fields[3 + k])
+    for p, q in zip(shot.psi, psi):
+        p.copy_(q)
 
 
 def reverse_pass(shot, adj, wav, nt, residual, ckpt, stored, checkpoint_every=200,
@@ -263,8 +287,11 @@ def history_segments(nt, checkpoint_every, shape, real_bytes, budget_bytes, fiel
 def choose_history(shot, checkpoint_every, budget_fraction=0.3):
     """'strain' (6 fields per step, no strain recompute in the reverse pass)
     when a segment of it fits in budget_fraction of the free device memory,
-    else 'velocity' (3 fields)."""
+    else 'velocity' (3 fields).  A CPML shot always takes 'strain' (its
+    strains depend on the memory variables as well as on v)."""
     torch = shot.torch
+    if shot.cpml is not None:
+        return "strain"
     real_bytes = 4 if shot.real == torch.float32 else 8
     need = int(checkpoint_every) * 6 * int(np.prod(shot.shape)) * real_bytes
     free = torch.cuda.mem_get_info(shot.device)[0]
@@ -310,9 +337,9 @@ def _autograd_function():
 
         @staticmethod
         def forward(ctx, Ct, shot, wav, nt, checkpoint_every, history_bytes):
-            if shot.cpml is not None:
-                raise ValueError("the adjoint runs the sponge boundary only: this shot has a "
-                                 "CPML")
+            if shot.cpml is not None and shot.desc.kernel != _abi.PE_KERNEL_DIRECT:
+                raise ValueError("the adjoint of a CPML shot runs the direct kernels: build the "
+                                 "DeviceShot with kernel='direct'")
             table = Ct.detach().reshape(shot.Ct.shape).to(device=shot.device,
                                                           dtype=shot.real).clone()
             shot.Ct.copy_(table)

@@ -451,6 +451,10 @@ struct Adj {
     int zbreak;
     const int32_t *run_ptr;
     double *run_acc;
+    // CPML: adjoint memory variables (6 x band volume per axis) and the band
+    // cells' derivative adjoints (3 x band volume per axis)
+    R *psib[3];
+    R *pw[3];
 };
 
 template <typename R>
@@ -467,6 +471,10 @@ inline Adj<R> make_adj(const pe_adjoint *a)
     A.zbreak = a->zbreak;
     A.run_ptr = a->run_ptr;
     A.run_acc = a->run_acc;
+    for (int k = 0; k < 3; ++k) {
+        A.psib[k] = (R *)a->cpml_psib[k];
+        A.pw[k] = (R *)a->cpml_work[k];
+    }
     return A;
 }
 

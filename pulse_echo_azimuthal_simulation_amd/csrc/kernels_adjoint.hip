@@ -20,6 +20,11 @@
 //   A^T: grad += dt ls q(v^n)  k_adj_grad (strains of v^n in the scratch)
 //        gbar = M^T (dt C^T ls)  k_adj_gbar
 //        lv = d * lv + G^T gbar, ls <- d * ls   k_adj_strain_t
+// With a CPML (no upstream counterpart) the state also holds the memory
+// variables; k_adj_pml_pre runs before D^T and before k_adj_strain_t, one
+// thread per band cell, and the two gather kernels add the transposed band
+// derivatives near the bands.  The gradient then reads the strain history
+// (the strains depend on the forward memory variables).
 //
 // Transposed stencils.  With the forward differences
 //   F_j f(i) = f(i+j+1) - f(i-j)  (kept iff j <= i <= n-j-2)
@@ -110,7 +115,143 @@ __device__ __forceinline__ R diff_t(const Shot<R> &S, Fn f, int k, int kz, int k
     return acc;
 }
 
-// D^T: ls_I(k) += transposed stress-divergence stencils of e.
+// ------------------------------------------------------------------ CPML
+// A band cell's correction of one derivative dd (pml_corr in common.hpp) is
+// psi' = b psi + a dd, corr = (1/kappa - 1) dd + psi'.  Given the adjoint
+// ebar of the strain (acceleration) it corrects and the adjoint pbar of psi'
+// (A.psib, carried from the later step), with P = ebar + pbar: the adjoint of
+// dd is (1/kappa - 1) ebar + a P, and that of psi is b P (stored over pbar).
+// No upstream counterpart.
+template <int AX, typename R>
+__device__ __forceinline__ int band_vol(const Shot<R> &S)
+{
+    const int W = 2 * S.pml + 2;
+    return AX == 0 ? W * S.ny * S.nx : (AX == 1 ? S.nz * W * S.nx : S.nz * S.ny * W);
+}
+
+template <int AX, typename R>
+__device__ __forceinline__ int band_flat(const Shot<R> &S, int iz, int iy, int ix)
+{
+    const int L = S.pml, W = 2 * L + 2;
+    if constexpr (AX == 0) return ((iz <= L ? iz : iz - (S.nz - W)) * S.ny + iy) * S.nx + ix;
+    else if constexpr (AX == 1) return (iz * W + (iy <= L ? iy : iy - (S.ny - W))) * S.nx + ix;
+    else return (iz * S.ny + iy) * W + (ix <= L ? ix : ix - (S.nx - W));
+}
+
+template <int AX, int K, int ST, typename R>
+__device__ __forceinline__ R pml_adj_d(const Shot<R> &S, const Adj<R> &A, int iz, int iy, int ix,
+                                       R ebar)
+{
+    const int n = extent<AX>(S), ia = coord<AX>(iz, iy, ix);
+    const R *pr = S.prof[AX] + (size_t)ST * 3 * n;
+    R *pb = A.psib[AX] + (size_t)K * band_vol<AX>(S) + band_flat<AX>(S, iz, iy, ix);
+    const R P = ebar + *pb;
+    *pb = pr[ia] * P;
+    return (pr[2 * n + ia] - (R)1) * ebar + pr[n + ia] * P;
+}
+
+// One thread per band cell (pml_band_cell order): the adjoints of the
+// cell's band derivatives into A.pw[axis][k][band cell], k the derivative of
+// that axis, and its adjoint memory variables advanced.  HALF 1: the
+// velocity half-step's (memory variables 3..5), from the adjoints e of the
+// accelerations; HALF 0: the strain half-step's (0..2), from gbar (A.work).
+template <typename R, int HALF>
+__global__ __launch_bounds__(kBlock) void k_adj_pml_pre(Shot<R> S, Adj<R> A)
+{
+    const long long b = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (b >= pml_band_cells(S.nz, S.ny, S.nx, S.pml)) return;
+    int iz, iy, ix;
+    pml_band_cell((int)b, S.nz, S.ny, S.nx, S.pml, iz, iy, ix);
+    const int L = S.pml;
+    const int i = (iz * S.ny + iy) * S.nx + ix;
+    R t0, t1, t2, u0, u1, u2, w0, w1, w2;    // the corrected quantities' adjoints per axis
+    if constexpr (HALF == 1) {
+        // z band: D_z s5 -> ax, D_z s4 -> ay, D_z s3 -> az; y band: D_y s6,
+        // D_y s2, D_y s4; x band: D_x s1, D_x s6, D_x s5
+        const R ex = e_val<0>(S, A, i, iz, iy, ix), ey = e_val<1>(S, A, i, iz, iy, ix),
+                ez = e_val<2>(S, A, i, iz, iy, ix);
+        t0 = u0 = w0 = ex;
+        t1 = u1 = w1 = ey;
+        t2 = u2 = w2 = ez;
+    } else {
+        // z band: Db_z vz -> g3, Df_z vy -> g4, Df_z vx -> g5; y band: Db_y
+        // vy -> g2, Df_y vz -> g4, Df_y vx -> g6; x band: Db_x vx -> g1, Df_x
+        // vz -> g5, Df_x vy -> g6
+        const size_t n = (size_t)S.nz * S.ny * S.nx;
+        const R *g = A.work;
+        t0 = g[2 * n + i];
+        t1 = g[3 * n + i];
+        t2 = g[4 * n + i];
+        u0 = g[1 * n + i];
+        u1 = g[3 * n + i];
+        u2 = g[5 * n + i];
+        w0 = g[0 * n + i];
+        w1 = g[4 * n + i];
+        w2 = g[5 * n + i];
+    }
+    if (pml_in(iz, S.nz, L)) {
+        R *d = A.pw[0] + band_flat<0>(S, iz, iy, ix);
+        const size_t v = (size_t)band_vol<0>(S);
+        if constexpr (HALF == 1) {
+            d[0] = pml_adj_d<0, 3, 0>(S, A, iz, iy, ix, t0);
+            d[v] = pml_adj_d<0, 4, 0>(S, A, iz, iy, ix, t1);
+            d[2 * v] = pml_adj_d<0, 5, 1>(S, A, iz, iy, ix, t2);
+        } else {
+            d[0] = pml_adj_d<0, 0, 0>(S, A, iz, iy, ix, t0);
+            d[v] = pml_adj_d<0, 1, 1>(S, A, iz, iy, ix, t1);
+            d[2 * v] = pml_adj_d<0, 2, 1>(S, A, iz, iy, ix, t2);
+        }
+    }
+    if (pml_in(iy, S.ny, L)) {
+        R *d = A.pw[1] + band_flat<1>(S, iz, iy, ix);
+        const size_t v = (size_t)band_vol<1>(S);
+        if constexpr (HALF == 1) {
+            d[0] = pml_adj_d<1, 3, 0>(S, A, iz, iy, ix, u0);
+            d[v] = pml_adj_d<1, 4, 1>(S, A, iz, iy, ix, u1);
+            d[2 * v] = pml_adj_d<1, 5, 0>(S, A, iz, iy, ix, u2);
+        } else {
+            d[0] = pml_adj_d<1, 0, 0>(S, A, iz, iy, ix, u0);
+            d[v] = pml_adj_d<1, 1, 1>(S, A, iz, iy, ix, u1);
+            d[2 * v] = pml_adj_d<1, 2, 1>(S, A, iz, iy, ix, u2);
+        }
+    }
+    if (pml_in(ix, S.nx, L)) {
+        R *d = A.pw[2] + band_flat<2>(S, iz, iy, ix);
+        const size_t v = (size_t)band_vol<2>(S);
+        if constexpr (HALF == 1) {
+            d[0] = pml_adj_d<2, 3, 1>(S, A, iz, iy, ix, w0);
+            d[v] = pml_adj_d<2, 4, 0>(S, A, iz, iy, ix, w1);
+            d[2 * v] = pml_adj_d<2, 5, 0>(S, A, iz, iy, ix, w2);
+        } else {
+            d[0] = pml_adj_d<2, 0, 0>(S, A, iz, iy, ix, w0);
+            d[v] = pml_adj_d<2, 1, 1>(S, A, iz, iy, ix, w1);
+            d[2 * v] = pml_adj_d<2, 2, 1>(S, A, iz, iy, ix, w2);
+        }
+    }
+}
+
+// The derivative adjoints of axis AX, component K, as a field over the
+// cells (0 outside the band), for the transposed stencils of diff_t.
+template <int AX, int K, typename R>
+__device__ __forceinline__ auto band_field(const Shot<R> &S, const Adj<R> &A)
+{
+    return [&S, &A](int, int iz, int iy, int ix) -> R {
+        const int n = extent<AX>(S), c = coord<AX>(iz, iy, ix);
+        if (!pml_in(c, n, S.pml)) return (R)0;
+        return A.pw[AX][(size_t)K * band_vol<AX>(S) + band_flat<AX>(S, iz, iy, ix)];
+    };
+}
+
+// Is cell (kz, ky, kx) within the taps' reach (4 cells) of the band of axis AX?
+template <int AX, typename R>
+__device__ __forceinline__ bool near_band(const Shot<R> &S, int kz, int ky, int kx)
+{
+    const int n = extent<AX>(S), c = coord<AX>(kz, ky, kx), L = S.pml;
+    return L > 0 && (c <= L + 4 || c >= n - 1 - L - 4);
+}
+
+// D^T: ls_I(k) += transposed stress-divergence stencils of e (and, near a
+// CPML band, those of the band cells' derivative adjoints).
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_adj_velocity_direct(Shot<R> S, Adj<R> A)
 {
@@ -122,12 +263,29 @@ __global__ __launch_bounds__(kBlock) void k_adj_velocity_direct(Shot<R> S, Adj<R
     auto ex = [&](int i, int iz, int iy, int ix) { return e_val<0>(S, A, i, iz, iy, ix); };
     auto ey = [&](int i, int iz, int iy, int ix) { return e_val<1>(S, A, i, iz, iy, ix); };
     auto ez = [&](int i, int iz, int iy, int ix) { return e_val<2>(S, A, i, iz, iy, ix); };
-    const R d1 = diff_t<true, 2>(S, ex, k, kz, ky, kx);
-    const R d2 = diff_t<true, 1>(S, ey, k, kz, ky, kx);
-    const R d3 = diff_t<true, 0>(S, ez, k, kz, ky, kx);
-    const R d4 = diff_t<false, 0>(S, ey, k, kz, ky, kx) + diff_t<false, 1>(S, ez, k, kz, ky, kx);
-    const R d5 = diff_t<false, 0>(S, ex, k, kz, ky, kx) + diff_t<false, 2>(S, ez, k, kz, ky, kx);
-    const R d6 = diff_t<false, 1>(S, ex, k, kz, ky, kx) + diff_t<false, 2>(S, ey, k, kz, ky, kx);
+    R d1 = diff_t<true, 2>(S, ex, k, kz, ky, kx);
+    R d2 = diff_t<true, 1>(S, ey, k, kz, ky, kx);
+    R d3 = diff_t<true, 0>(S, ez, k, kz, ky, kx);
+    R d4 = diff_t<false, 0>(S, ey, k, kz, ky, kx) + diff_t<false, 1>(S, ez, k, kz, ky, kx);
+    R d5 = diff_t<false, 0>(S, ex, k, kz, ky, kx) + diff_t<false, 2>(S, ez, k, kz, ky, kx);
+    R d6 = diff_t<false, 1>(S, ex, k, kz, ky, kx) + diff_t<false, 2>(S, ey, k, kz, ky, kx);
+    // CPML: z band D_z s5, D_z s4 (backward), D_z s3 (forward); y band D_y
+    // s6 (b), D_y s2 (f), D_y s4 (b); x band D_x s1 (f), D_x s6, D_x s5 (b)
+    if (near_band<0>(S, kz, ky, kx)) {
+        d5 = d5 + diff_t<false, 0>(S, band_field<0, 0>(S, A), k, kz, ky, kx);
+        d4 = d4 + diff_t<false, 0>(S, band_field<0, 1>(S, A), k, kz, ky, kx);
+        d3 = d3 + diff_t<true, 0>(S, band_field<0, 2>(S, A), k, kz, ky, kx);
+    }
+    if (near_band<1>(S, kz, ky, kx)) {
+        d6 = d6 + diff_t<false, 1>(S, band_field<1, 0>(S, A), k, kz, ky, kx);
+        d2 = d2 + diff_t<true, 1>(S, band_field<1, 1>(S, A), k, kz, ky, kx);
+        d4 = d4 + diff_t<false, 1>(S, band_field<1, 2>(S, A), k, kz, ky, kx);
+    }
+    if (near_band<2>(S, kz, ky, kx)) {
+        d1 = d1 + diff_t<true, 2>(S, band_field<2, 0>(S, A), k, kz, ky, kx);
+        d6 = d6 + diff_t<false, 2>(S, band_field<2, 1>(S, A), k, kz, ky, kx);
+        d5 = d5 + diff_t<false, 2>(S, band_field<2, 2>(S, A), k, kz, ky, kx);
+    }
     A.ls[0][k] += d1;
     A.ls[1][k] += d2;
     A.ls[2][k] += d3;
@@ -234,12 +392,29 @@ __global__ __launch_bounds__(kBlock) void k_adj_strain_t_direct(Shot<R> S, Adj<R
     auto b4 = [&](int i, int, int, int) { return w[3 * n + i]; };
     auto b5 = [&](int i, int, int, int) { return w[4 * n + i]; };
     auto b6 = [&](int i, int, int, int) { return w[5 * n + i]; };
-    const R gx = diff_t<false, 2>(S, b1, k, kz, ky, kx) + diff_t<true, 0>(S, b5, k, kz, ky, kx) +
-                 diff_t<true, 1>(S, b6, k, kz, ky, kx);
-    const R gy = diff_t<false, 1>(S, b2, k, kz, ky, kx) + diff_t<true, 0>(S, b4, k, kz, ky, kx) +
-                 diff_t<true, 2>(S, b6, k, kz, ky, kx);
-    const R gz = diff_t<false, 0>(S, b3, k, kz, ky, kx) + diff_t<true, 1>(S, b4, k, kz, ky, kx) +
-                 diff_t<true, 2>(S, b5, k, kz, ky, kx);
+    R gx = diff_t<false, 2>(S, b1, k, kz, ky, kx) + diff_t<true, 0>(S, b5, k, kz, ky, kx) +
+           diff_t<true, 1>(S, b6, k, kz, ky, kx);
+    R gy = diff_t<false, 1>(S, b2, k, kz, ky, kx) + diff_t<true, 0>(S, b4, k, kz, ky, kx) +
+           diff_t<true, 2>(S, b6, k, kz, ky, kx);
+    R gz = diff_t<false, 0>(S, b3, k, kz, ky, kx) + diff_t<true, 1>(S, b4, k, kz, ky, kx) +
+           diff_t<true, 2>(S, b5, k, kz, ky, kx);
+    // CPML: z band Db_z vz, Df_z vy, Df_z vx; y band Db_y vy, Df_y vz, Df_y
+    // vx; x band Db_x vx, Df_x vz, Df_x vy
+    if (near_band<0>(S, kz, ky, kx)) {
+        gz = gz + diff_t<false, 0>(S, band_field<0, 0>(S, A), k, kz, ky, kx);
+        gy = gy + diff_t<true, 0>(S, band_field<0, 1>(S, A), k, kz, ky, kx);
+        gx = gx + diff_t<true, 0>(S, band_field<0, 2>(S, A), k, kz, ky, kx);
+    }
+    if (near_band<1>(S, kz, ky, kx)) {
+        gy = gy + diff_t<false, 1>(S, band_field<1, 0>(S, A), k, kz, ky, kx);
+        gz = gz + diff_t<true, 1>(S, band_field<1, 1>(S, A), k, kz, ky, kx);
+        gx = gx + diff_t<true, 1>(S, band_field<1, 2>(S, A), k, kz, ky, kx);
+    }
+    if (near_band<2>(S, kz, ky, kx)) {
+        gx = gx + diff_t<false, 2>(S, band_field<2, 0>(S, A), k, kz, ky, kx);
+        gz = gz + diff_t<true, 2>(S, band_field<2, 1>(S, A), k, kz, ky, kx);
+        gy = gy + diff_t<true, 2>(S, band_field<2, 2>(S, A), k, kz, ky, kx);
+    }
     const R dmp = S.damp ? S.damp[k] : (R)1;
     A.lv[0][k] = A.lv[0][k] * dmp + S.inv_h * gx;
     A.lv[1][k] = A.lv[1][k] * dmp + S.inv_h * gy;
@@ -437,6 +612,13 @@ hipError_t adjoint_reset(const pe_shot *p, const pe_adjoint *a, hipStream_t st)
     for (int k = 0; k < 6 && e == hipSuccess; ++k) e = fill_zero((R *)a->ls[k], ncell, st);
     const int K = a->n_proj > 0 ? a->n_proj : 36;
     if (e == hipSuccess && a->run_acc) e = fill_zero(a->run_acc, (size_t)a->n_runs * K, st);
+    if (p->cpml_width > 0) {
+        const size_t W = 2 * (size_t)p->cpml_width + 2;
+        const size_t vols[3] = {W * p->ny * p->nx, (size_t)p->nz * W * p->nx,
+                                (size_t)p->nz * p->ny * W};
+        for (int k = 0; k < 3 && e == hipSuccess; ++k)
+            e = fill_zero((R *)a->cpml_psib[k], 6 * vols[k], st);
+    }
     if (e == hipSuccess && a->kernel == PE_KERNEL_TILED) e = adjoint_face_weights<R>(p, a, st);
     return e;
 }
@@ -449,6 +631,7 @@ hipError_t adjoint_run(const pe_shot *p, const pe_adjoint *a, int32_t n_hi, int3
     const Adj<R> A = make_adj<R>(a);
     const int ncell = p->nz * p->ny * p->nx;
     const int ncol = p->ny * p->nx;
+    const size_t nband = S.pml > 0 ? (size_t)pml_band_cells(S.nz, S.ny, S.nx, S.pml) : 0;
     const R *res = (const R *)residual;
     const bool vel_mode = p->record_mode != PE_RECORD_PRESSURE;
     const bool rec = res && p->n_rec > 0;
@@ -468,6 +651,9 @@ hipError_t adjoint_run(const pe_shot *p, const pe_adjoint *a, int32_t n_hi, int3
             e = adjoint_step_tiled<R>(S, Sv, A, 0, n != n_hi - 1, st);
             if (e != hipSuccess) break;
         } else {
+            if (S.pml > 0)
+                hipLaunchKernelGGL((k_adj_pml_pre<R, 1>), dim3(blocks_of(nband, kBlock)),
+                                   dim3(kBlock), 0, st, S, A);
             hipLaunchKernelGGL(k_adj_velocity_direct<R>, dim3(blocks_of(ncell, kBlock)),
                                dim3(kBlock), 0, st, S, A);
         }
@@ -510,6 +696,9 @@ hipError_t adjoint_run(const pe_shot *p, const pe_adjoint *a, int32_t n_hi, int3
         } else {
             hipLaunchKernelGGL(k_adj_gbar_direct<R>, dim3(blocks_of(ncell, kBlock)), dim3(kBlock),
                                0, st, S, A);
+            if (S.pml > 0)
+                hipLaunchKernelGGL((k_adj_pml_pre<R, 0>), dim3(blocks_of(nband, kBlock)),
+                                   dim3(kBlock), 0, st, S, A);
             hipLaunchKernelGGL(k_adj_strain_t_direct<R>, dim3(blocks_of(ncell, kBlock)),
                                dim3(kBlock), 0, st, S, A);
         }
