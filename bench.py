@@ -636,15 +636,11 @@ def bench_adjoint(args):
     wav = target.wavelet(cfg["wavelet"])
     d = target.new_traces(nt)
     target.run(wav, 0, nt, d)
-    # the CPML adjoint runs the direct kernels; the forward it is measured
-    # against stays the tiled one
-    fwd_shot = target
-    shot = mk(Ct, "direct" if cp else "tiled")
+    shot = mk(Ct)
     for h in (0, 1):
         shot.desc.tile[h], shot.desc.zchunk[h] = best[h][0], best[h][1]
-    if not cp:
-        del target
-        fwd_shot = shot
+    del target
+    fwd_shot = shot
     proj = fabric.angle_projections(theta, phi)
     hist = args.hist if args.hist != "auto" else adjoint.choose_history(shot,
                                                                         args.checkpoint_every)
@@ -664,8 +660,6 @@ def bench_adjoint(args):
     torch.cuda.synchronize()
     tr = fwd_shot.new_traces(nt)
     fwd_shot.reset()
-    if cp:
-        fwd_shot.Ct.copy_(shot.Ct)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     fwd_shot.run(wav, 0, nt, tr)
@@ -695,7 +689,7 @@ def bench_adjoint(args):
                                "grid, 1 MHz Hann, target traces from axes tilted N(0, 5 deg), "
                                "20-cell %s, checkpoints every %d steps, %s history, first-pass "
                                "history budget %.1f GiB"
-                               % ("CPML (direct adjoint kernels)" if cp else "sponge",
+                               % ("CPML" if cp else "sponge",
                                   args.checkpoint_every, hist, hist_bytes / 2 ** 30),
                    "grid_zyx": [nz, ny, nx], "cells": ncell, "mode": "fwd+adj", "history": hist},
         "forward_s": round(t_fwd, 4), "gradient_s": round(t_grad, 4),

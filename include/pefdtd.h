@@ -140,8 +140,8 @@ int pe_run_store(const pe_shot *shot, int32_t n0, int32_t n1, const double *wave
 /* pe_run that also stores the strains g1..g6 of v at the start of every
  * step n (the values its stress half-step computes, 6 x ncell per step) into
  * ghist + (n-n0)*6*ncell: the history of PE_HIST_STRAIN adjoints, which then
- * need not recompute them; with a CPML (direct kernels only) the strains
- * with their band corrections.  No upstream counterpart (the upstream keeps
+ * need not recompute them; with a CPML the strains with their band
+ * corrections.  No upstream counterpart (the upstream keeps
  * velocities, vendor/ringfwi/elastic3d.py:301-305). */
 int pe_run_store_strains(const pe_shot *shot, int32_t n0, int32_t n1, const double *wavelet,
                          void *traces, void *ghist, void *stream);
@@ -216,9 +216,9 @@ typedef struct pe_adjoint {
     const int32_t *mat_ptr;      /* n_mat + 1 offsets into mat_runs */
     const int32_t *mat_runs;     /* n_runs run indices grouped by material */
     double *run_acc;             /* n_runs x K accumulators, or NULL: no gradient */
-    /* CPML shots (shot->cpml_width > 0; PE_KERNEL_DIRECT, and with a
-     * gradient PE_HIST_STRAIN, since a step's strains depend on the forward
-     * memory variables): cpml_psib[a] the adjoints of the memory variables,
+    /* CPML shots (shot->cpml_width > 0; with a gradient PE_HIST_STRAIN,
+     * since a step's strains depend on the forward memory variables):
+     * cpml_psib[a] the adjoints of the memory variables,
      * laid out like shot->cpml_psi[a] (6 x band volume; zeroed by
      * pe_adjoint_reset, carried from step to step), cpml_work[a] a scratch
      * of 3 x band volume of axis a.  No upstream counterpart. */

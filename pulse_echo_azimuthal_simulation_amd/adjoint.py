@@ -8,9 +8,9 @@ by vendor/ringfwi/axisfield.py:37-161).  Here the label-indexed GPU step of
 
 * the gradient reduced per material (grain) in float64, deterministically
   (per-column material runs, see ``run_tables``);
-* a CPML boundary (direct kernels): the adjoints of the memory variables are
-  carried through the reverse pass, checkpoints hold the memory variables,
-  and the gradient reads the strain history;
+* a CPML boundary: the adjoints of the memory variables are carried through
+  the reverse pass, checkpoints hold the memory variables, and the gradient
+  reads the strain history;
 * wavefield checkpointing: the forward pass keeps the state every
   ``checkpoint_every`` steps; each segment is recomputed from its checkpoint
   while a per-step history is stored -- the strains the stress half-step
@@ -105,10 +105,6 @@ class DeviceAdjoint:
         if self.cpml:
             # the memory variables make the strains of a step depend on more
             # than v, so the gradient reads the strains the forward stored
-            # (written by the direct kernels)
-            if kernel != "direct" or shot.desc.kernel != _abi.PE_KERNEL_DIRECT:
-                raise ValueError("the adjoint of a CPML shot runs the direct kernels: build the "
-                                 "DeviceShot with kernel='direct'")
             if gradient and hist != "strain":
                 raise ValueError("the gradient of a CPML shot needs hist='strain'")
         self.hist = hist
@@ -337,9 +333,6 @@ def _autograd_function():
 
         @staticmethod
         def forward(ctx, Ct, shot, wav, nt, checkpoint_every, history_bytes):
-            if shot.cpml is not None and shot.desc.kernel != _abi.PE_KERNEL_DIRECT:
-                raise ValueError("the adjoint of a CPML shot runs the direct kernels: build the "
-                                 "DeviceShot with kernel='direct'")
             table = Ct.detach().reshape(shot.Ct.shape).to(device=shot.device,
                                                           dtype=shot.real).clone()
             shot.Ct.copy_(table)

@@ -478,9 +478,6 @@ int pe_run_store_strains(const pe_shot *p, int32_t n0, int32_t n1, const double 
     if (n0 < 0 || n1 < n0) return fail(PE_EINVAL, "bad step range [%d, %d)", n0, n1);
     if (!wavelet && p->n_src > 0) return fail(PE_EINVAL, "wavelet is NULL");
     if (!ghist) return fail(PE_EINVAL, "ghist is NULL");
-    if (p->cpml_width > 0 && p->kernel != PE_KERNEL_DIRECT)
-        return fail(PE_EINVAL, "the strain history of a CPML shot is written by the direct "
-                               "kernels (PE_KERNEL_DIRECT)");
     return p->precision == PE_F32
                ? run_impl<float>(p, n0, n1, wavelet, traces, (hipStream_t)stream, ghist, 1)
                : run_impl<double>(p, n0, n1, wavelet, traces, (hipStream_t)stream, ghist, 1);
@@ -521,9 +518,6 @@ int pe_check_adjoint(const pe_shot *p, const pe_adjoint *a)
     if (rc) return rc;
     if (!a) return fail(PE_EINVAL, "adjoint descriptor is NULL");
     if (p->cpml_width > 0) {
-        if (a->kernel != PE_KERNEL_DIRECT)
-            return fail(PE_EINVAL, "the adjoint of a CPML shot runs the direct kernels "
-                                   "(PE_KERNEL_DIRECT)");
         if (a->run_acc && a->hist_kind != PE_HIST_STRAIN)
             return fail(PE_EINVAL, "the gradient of a CPML shot needs the strain history "
                                    "(PE_HIST_STRAIN): its strains depend on the memory variables");

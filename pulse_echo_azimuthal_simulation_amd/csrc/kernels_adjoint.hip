@@ -250,6 +250,85 @@ __device__ __forceinline__ bool near_band(const Shot<R> &S, int kz, int ky, int 
     return L > 0 && (c <= L + 4 || c >= n - 1 - L - 4);
 }
 
+// Transposed band derivatives of the velocity half-step at cell k, added to
+// the stress adjoint sums d1..d6: z band D_z s5, D_z s4 (backward
+// differences), D_z s3 (forward); y band D_y s6 (b), D_y s2 (f), D_y s4 (b);
+// x band D_x s1 (f), D_x s6, D_x s5 (b).
+template <typename R>
+__device__ __forceinline__ void vel_band_t(const Shot<R> &S, const Adj<R> &A, int k, int kz,
+                                           int ky, int kx, R &d1, R &d2, R &d3, R &d4, R &d5,
+                                           R &d6)
+{
+    if (near_band<0>(S, kz, ky, kx)) {
+        d5 = d5 + diff_t<false, 0>(S, band_field<0, 0>(S, A), k, kz, ky, kx);
+        d4 = d4 + diff_t<false, 0>(S, band_field<0, 1>(S, A), k, kz, ky, kx);
+        d3 = d3 + diff_t<true, 0>(S, band_field<0, 2>(S, A), k, kz, ky, kx);
+    }
+    if (near_band<1>(S, kz, ky, kx)) {
+        d6 = d6 + diff_t<false, 1>(S, band_field<1, 0>(S, A), k, kz, ky, kx);
+        d2 = d2 + diff_t<true, 1>(S, band_field<1, 1>(S, A), k, kz, ky, kx);
+        d4 = d4 + diff_t<false, 1>(S, band_field<1, 2>(S, A), k, kz, ky, kx);
+    }
+    if (near_band<2>(S, kz, ky, kx)) {
+        d1 = d1 + diff_t<true, 2>(S, band_field<2, 0>(S, A), k, kz, ky, kx);
+        d6 = d6 + diff_t<false, 2>(S, band_field<2, 1>(S, A), k, kz, ky, kx);
+        d5 = d5 + diff_t<false, 2>(S, band_field<2, 2>(S, A), k, kz, ky, kx);
+    }
+}
+
+// Transposed band derivatives of the strain half-step at cell k, added to
+// the velocity adjoint sums (before the 1/h scale): z band Db_z vz, Df_z vy,
+// Df_z vx; y band Db_y vy, Df_y vz, Df_y vx; x band Db_x vx, Df_x vz, Df_x
+// vy.
+template <typename R>
+__device__ __forceinline__ void strain_band_t(const Shot<R> &S, const Adj<R> &A, int k, int kz,
+                                              int ky, int kx, R &gx, R &gy, R &gz)
+{
+    if (near_band<0>(S, kz, ky, kx)) {
+        gz = gz + diff_t<false, 0>(S, band_field<0, 0>(S, A), k, kz, ky, kx);
+        gy = gy + diff_t<true, 0>(S, band_field<0, 1>(S, A), k, kz, ky, kx);
+        gx = gx + diff_t<true, 0>(S, band_field<0, 2>(S, A), k, kz, ky, kx);
+    }
+    if (near_band<1>(S, kz, ky, kx)) {
+        gy = gy + diff_t<false, 1>(S, band_field<1, 0>(S, A), k, kz, ky, kx);
+        gz = gz + diff_t<true, 1>(S, band_field<1, 1>(S, A), k, kz, ky, kx);
+        gx = gx + diff_t<true, 1>(S, band_field<1, 2>(S, A), k, kz, ky, kx);
+    }
+    if (near_band<2>(S, kz, ky, kx)) {
+        gx = gx + diff_t<false, 2>(S, band_field<2, 0>(S, A), k, kz, ky, kx);
+        gz = gz + diff_t<true, 2>(S, band_field<2, 1>(S, A), k, kz, ky, kx);
+        gy = gy + diff_t<true, 2>(S, band_field<2, 2>(S, A), k, kz, ky, kx);
+    }
+}
+
+// The tiled family's CPML band terms, as separate gathers after its own
+// kernels (whose updates they add to): HALF 1 into ls after D^T, HALF 0
+// into lv (times 1/h) after G^T.  One thread per cell; cells beyond the
+// taps' reach of every band exit at once.
+template <typename R, int HALF>
+__global__ __launch_bounds__(kBlock) void k_adj_pml_gather(Shot<R> S, Adj<R> A)
+{
+    const int nz = S.nz, ny = S.ny, nx = S.nx, sz = nx * ny;
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= sz * nz) return;
+    const int kx = k % nx, ky = (k / nx) % ny, kz = k / sz;
+    if (!near_band<0>(S, kz, ky, kx) && !near_band<1>(S, kz, ky, kx) &&
+        !near_band<2>(S, kz, ky, kx))
+        return;
+    if constexpr (HALF == 1) {
+        R d[6] = {0, 0, 0, 0, 0, 0};
+        vel_band_t(S, A, k, kz, ky, kx, d[0], d[1], d[2], d[3], d[4], d[5]);
+#pragma unroll
+        for (int I = 0; I < 6; ++I) A.ls[I][k] += d[I];
+    } else {
+        R gx = 0, gy = 0, gz = 0;
+        strain_band_t(S, A, k, kz, ky, kx, gx, gy, gz);
+        A.lv[0][k] += S.inv_h * gx;
+        A.lv[1][k] += S.inv_h * gy;
+        A.lv[2][k] += S.inv_h * gz;
+    }
+}
+
 // D^T: ls_I(k) += transposed stress-divergence stencils of e (and, near a
 // CPML band, those of the band cells' derivative adjoints).
 template <typename R>
@@ -269,23 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_velocity_direct(Shot<R> S, Adj<R
     R d4 = diff_t<false, 0>(S, ey, k, kz, ky, kx) + diff_t<false, 1>(S, ez, k, kz, ky, kx);
     R d5 = diff_t<false, 0>(S, ex, k, kz, ky, kx) + diff_t<false, 2>(S, ez, k, kz, ky, kx);
     R d6 = diff_t<false, 1>(S, ex, k, kz, ky, kx) + diff_t<false, 2>(S, ey, k, kz, ky, kx);
-    // CPML: z band D_z s5, D_z s4 (backward), D_z s3 (forward); y band D_y
-    // s6 (b), D_y s2 (f), D_y s4 (b); x band D_x s1 (f), D_x s6, D_x s5 (b)
-    if (near_band<0>(S, kz, ky, kx)) {
-        d5 = d5 + diff_t<false, 0>(S, band_field<0, 0>(S, A), k, kz, ky, kx);
-        d4 = d4 + diff_t<false, 0>(S, band_field<0, 1>(S, A), k, kz, ky, kx);
-        d3 = d3 + diff_t<true, 0>(S, band_field<0, 2>(S, A), k, kz, ky, kx);
-    }
-    if (near_band<1>(S, kz, ky, kx)) {
-        d6 = d6 + diff_t<false, 1>(S, band_field<1, 0>(S, A), k, kz, ky, kx);
-        d2 = d2 + diff_t<true, 1>(S, band_field<1, 1>(S, A), k, kz, ky, kx);
-        d4 = d4 + diff_t<false, 1>(S, band_field<1, 2>(S, A), k, kz, ky, kx);
-    }
-    if (near_band<2>(S, kz, ky, kx)) {
-        d1 = d1 + diff_t<true, 2>(S, band_field<2, 0>(S, A), k, kz, ky, kx);
-        d6 = d6 + diff_t<false, 2>(S, band_field<2, 1>(S, A), k, kz, ky, kx);
-        d5 = d5 + diff_t<false, 2>(S, band_field<2, 2>(S, A), k, kz, ky, kx);
-    }
+    vel_band_t(S, A, k, kz, ky, kx, d1, d2, d3, d4, d5, d6);
     A.ls[0][k] += d1;
     A.ls[1][k] += d2;
     A.ls[2][k] += d3;
@@ -398,23 +461,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_strain_t_direct(Shot<R> S, Adj<R
            diff_t<true, 2>(S, b6, k, kz, ky, kx);
     R gz = diff_t<false, 0>(S, b3, k, kz, ky, kx) + diff_t<true, 1>(S, b4, k, kz, ky, kx) +
            diff_t<true, 2>(S, b5, k, kz, ky, kx);
-    // CPML: z band Db_z vz, Df_z vy, Df_z vx; y band Db_y vy, Df_y vz, Df_y
-    // vx; x band Db_x vx, Df_x vz, Df_x vy
-    if (near_band<0>(S, kz, ky, kx)) {
-        gz = gz + diff_t<false, 0>(S, band_field<0, 0>(S, A), k, kz, ky, kx);
-        gy = gy + diff_t<true, 0>(S, band_field<0, 1>(S, A), k, kz, ky, kx);
-        gx = gx + diff_t<true, 0>(S, band_field<0, 2>(S, A), k, kz, ky, kx);
-    }
-    if (near_band<1>(S, kz, ky, kx)) {
-        gy = gy + diff_t<false, 1>(S, band_field<1, 0>(S, A), k, kz, ky, kx);
-        gz = gz + diff_t<true, 1>(S, band_field<1, 1>(S, A), k, kz, ky, kx);
-        gx = gx + diff_t<true, 1>(S, band_field<1, 2>(S, A), k, kz, ky, kx);
-    }
-    if (near_band<2>(S, kz, ky, kx)) {
-        gx = gx + diff_t<false, 2>(S, band_field<2, 0>(S, A), k, kz, ky, kx);
-        gz = gz + diff_t<true, 2>(S, band_field<2, 1>(S, A), k, kz, ky, kx);
-        gy = gy + diff_t<true, 2>(S, band_field<2, 2>(S, A), k, kz, ky, kx);
-    }
+    strain_band_t(S, A, k, kz, ky, kx, gx, gy, gz);
     const R dmp = S.damp ? S.damp[k] : (R)1;
     A.lv[0][k] = A.lv[0][k] * dmp + S.inv_h * gx;
     A.lv[1][k] = A.lv[1][k] * dmp + S.inv_h * gy;
@@ -646,10 +693,16 @@ hipError_t adjoint_run(const pe_shot *p, const pe_adjoint *a, int32_t n_hi, int3
                                0, (const int64_t *)nullptr, (const R *)nullptr, (double *)nullptr);
         }
         if (tiled) {
+            if (S.pml > 0)
+                hipLaunchKernelGGL((k_adj_pml_pre<R, 1>), dim3(blocks_of(nband, kBlock)),
+                                   dim3(kBlock), 0, st, S, A);
             // the first reverse step of a call starts from consistent adjoints;
             // later ones apply the previous step's deferred damping of ls
             e = adjoint_step_tiled<R>(S, Sv, A, 0, n != n_hi - 1, st);
             if (e != hipSuccess) break;
+            if (S.pml > 0)
+                hipLaunchKernelGGL((k_adj_pml_gather<R, 1>), dim3(blocks_of(ncell, kBlock)),
+                                   dim3(kBlock), 0, st, S, A);
         } else {
             if (S.pml > 0)
                 hipLaunchKernelGGL((k_adj_pml_pre<R, 1>), dim3(blocks_of(nband, kBlock)),
@@ -690,7 +743,17 @@ hipError_t adjoint_run(const pe_shot *p, const pe_adjoint *a, int32_t n_hi, int3
             default: hipLaunchKernelGGL((k_adj_grad_direct<R, 4>), gb, tb, 0, st, S, A, g); break;
             }
         }
-        if (tiled) {
+        if (tiled && S.pml > 0) {
+            // M^T, the band cells' strain adjoints, G^T, the band gathers
+            e = adjoint_step_tiled<R>(S, Sv, A, 3, 0, st);
+            if (e != hipSuccess) break;
+            hipLaunchKernelGGL((k_adj_pml_pre<R, 0>), dim3(blocks_of(nband, kBlock)),
+                               dim3(kBlock), 0, st, S, A);
+            e = adjoint_step_tiled<R>(S, Sv, A, 4, 0, st);
+            if (e != hipSuccess) break;
+            hipLaunchKernelGGL((k_adj_pml_gather<R, 0>), dim3(blocks_of(ncell, kBlock)),
+                               dim3(kBlock), 0, st, S, A);
+        } else if (tiled) {
             e = adjoint_step_tiled<R>(S, Sv, A, 2, 0, st);
             if (e != hipSuccess) break;
         } else {

@@ -1151,8 +1151,11 @@ hipError_t adjoint_step_tiled(const Shot<R> &S, const Shot<R> &Sv, const Adj<R> 
         default: return grad_tiled<R, 4>(Sv, A, st, gh);
         }
     }
+    // stage 2: M^T then G^T; stage 3: M^T alone; stage 4: G^T alone (a CPML
+    // shot's band cells read gbar in between)
+    if (stage == 4) return adj_strain_t_launch<R, 32, 8>(S, A, st);
     hipError_t e = f32 ? adj_gbar_launch<R, 32, 8>(S, A, st) : adj_gbar_launch<R, 32, 4>(S, A, st);
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess || stage == 3) return e;
     return adj_strain_t_launch<R, 32, 8>(S, A, st);
 }
 

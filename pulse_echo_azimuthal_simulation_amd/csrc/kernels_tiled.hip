@@ -1122,9 +1122,11 @@ hipError_t stress_launch(const Shot<R> &s, int zchunk, hipStream_t st, bool stor
         return hipGetLastError();
     };
     if (s.pml > 0) {
-        if (store_g) return hipErrorInvalidValue;   // the strain history serves the adjoint only
         hipError_t e = pml_corr_launch(s, 0, st);
         if (e != hipSuccess) return e;
+        // with the strain history of the adjoint: the strains with their
+        // band corrections
+        if (store_g) return run(k_stress_tiled<R, TX, TY, PAD, true, FR, true>);
         return run(k_stress_tiled<R, TX, TY, PAD, false, FR, true>);
     }
     if (store_g) return run(k_stress_tiled<R, TX, TY, PAD, true, FR>);

@@ -212,8 +212,10 @@ def test_autograd_two_forwards_then_backward_of_first(torch_gpu):
     assert H.rel_l2(t1.grad.numpy(), g_ref) <= 1e-9
 
 
-def test_autograd_refuses_cpml_shot(torch_gpu):
-    _case("adjoint autograd cpml refusal")
+def test_cpml_gradient_refuses_velocity_history(torch_gpu):
+    """A CPML step's strains depend on the memory variables as well as on v:
+    its gradient needs the strain history."""
+    _case("adjoint cpml velocity-history refusal")
     torch = torch_gpu
     from pulse_echo_azimuthal_simulation_amd import adjoint, cpml
     from pulse_echo_azimuthal_simulation_amd.solver import DeviceShot
@@ -223,9 +225,8 @@ def test_autograd_refuses_cpml_shot(torch_gpu):
     sh = DeviceShot(inp["mats"], inp["Ct"], inp["rho_t"], inp["damp"], inp["coeffs"], inp["h"],
                     inp["dt"], inp["src"][0], inp["src"][1], *inp["rec"], precision="f64",
                     kernel="tiled", cpml=cp)
-    t = torch.tensor(np.asarray(inp["Ct"], float).reshape(-1, 36), requires_grad=True)
-    with pytest.raises(ValueError, match="CPML"):
-        adjoint.traces_from_table(t, sh, inp["wavelet"], inp["nt"])
+    with pytest.raises(ValueError, match="strain"):
+        adjoint.DeviceAdjoint(sh, hist="velocity")
 
 
 @pytest.mark.parametrize("precision,shape", [("f32", (40, 44, 150)), ("f64", (36, 40, 90)),

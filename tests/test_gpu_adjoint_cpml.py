@@ -1,4 +1,4 @@
-"""GPU adjoint of CPML shots (direct kernels, through the C ABI) against the
+"""GPU adjoint of CPML shots (both kernel families, through the C ABI) against the
 CPU oracle's reverse step with its CPML transposes (pinned by dot-product
 identities and central differences in tests/test_oracle_adjoint.py) and
 against the identities it must satisfy on the device itself.
@@ -63,17 +63,18 @@ def _oracle_grad(inp, record="pressure", precision="f64"):
     return J, gC, gw, d
 
 
-def _device_shot(inp, precision, record="pressure", Ct=None):
+def _device_shot(inp, precision, record="pressure", Ct=None, kernel="direct"):
     from pulse_echo_azimuthal_simulation_amd.solver import DeviceShot
     return DeviceShot(inp["mats"], inp["Ct"] if Ct is None else Ct, inp["rho_t"], inp["damp"],
                       inp["coeffs"], inp["h"], inp["dt"], inp["src"][0], inp["src"][1],
-                      *inp["rec"], record=record, precision=precision, kernel="direct",
+                      *inp["rec"], record=record, precision=precision, kernel=kernel,
                       cpml=inp["cpml"])
 
 
-def _gpu_grad(torch, inp, d, precision="f64", record="pressure", every=7, history_bytes=0):
+def _gpu_grad(torch, inp, d, precision="f64", record="pressure", every=7, history_bytes=0,
+              kernel="direct"):
     from pulse_echo_azimuthal_simulation_amd import adjoint
-    sh = _device_shot(inp, precision, record)
+    sh = _device_shot(inp, precision, record, kernel=kernel)
     dd = torch.as_tensor(d, device=sh.device).to(sh.real)
 
     def resid(tr):
@@ -88,12 +89,13 @@ def _gpu_grad(torch, inp, d, precision="f64", record="pressure", every=7, histor
     return J, g.cpu().numpy(), gw.cpu().numpy()
 
 
+@pytest.mark.parametrize("kernel", ["direct", "tiled"])
 @pytest.mark.parametrize("record", ["pressure", "vx", "vz"])
-def test_cpml_f64_gradient_matches_oracle(torch_gpu, record):
-    _case("cpml adjoint f64 %s" % record)
+def test_cpml_f64_gradient_matches_oracle(torch_gpu, record, kernel):
+    _case("cpml adjoint f64 %s %s" % (kernel, record))
     inp = _medium()
     J0, g0, gw0, d = _oracle_grad(inp, record)
-    J, g, gw = _gpu_grad(torch_gpu, inp, d, "f64", record)
+    J, g, gw = _gpu_grad(torch_gpu, inp, d, "f64", record, kernel=kernel)
     assert abs(J - J0) <= 1e-12 * J0
     assert np.abs(g0).max() > 0
     assert H.rel_l2(g, g0) <= 1e-9, H.rel_l2(g, g0)
@@ -112,11 +114,13 @@ def test_cpml_history_kept_and_recomputed_agree(torch_gpu):
     assert np.array_equal(gw_a, gw_b)
 
 
-@pytest.mark.parametrize("shape,L,seed", [((14, 15, 16), 3, 0), ((12, 13, 17), 4, 1)])
-def test_cpml_one_step_dot_product_on_gpu(torch_gpu, shape, L, seed):
+@pytest.mark.parametrize("kernel", ["direct", "tiled"])
+@pytest.mark.parametrize("shape,L,seed", [((14, 15, 16), 3, 0), ((12, 13, 17), 4, 1),
+                                          ((30, 36, 70), 5, 2)])
+def test_cpml_one_step_dot_product_on_gpu(torch_gpu, shape, L, seed, kernel):
     """<L x, y> = <x, L^T y> for one device step from a random state (fields
     and memory variables) and random adjoints, f64, with random profiles."""
-    _case("cpml adjoint dot product %s L %d" % (shape, L))
+    _case("cpml adjoint dot product %s %s L %d" % (kernel, shape, L))
     torch = torch_gpu
     from pulse_echo_azimuthal_simulation_amd import adjoint
     inp = _medium(shape, seed, nt=1, L=L)
@@ -125,7 +129,7 @@ def test_cpml_one_step_dot_product_on_gpu(torch_gpu, shape, L, seed):
     for a in range(3):
         cp.prof[a] = rng.uniform(0.2, 1.2, size=np.shape(cp.prof[a]))
     inp["damp"] = rng.uniform(0.6, 1.0, inp["shape"])
-    sh = _device_shot(inp, "f64")
+    sh = _device_shot(inp, "f64", kernel=kernel)
     x_v = [rng.normal(size=inp["shape"]) for _ in range(3)]
     x_s = [rng.normal(size=inp["shape"]) for _ in range(6)]
     x_p = [rng.normal(size=tuple(p.shape)) for p in sh.psi]
@@ -159,11 +163,12 @@ def test_cpml_one_step_dot_product_on_gpu(torch_gpu, shape, L, seed):
     assert abs(lhs - rhs) <= 1e-11 * abs(lhs), (lhs, rhs)
 
 
-def test_cpml_f32_gradient_close_to_f64_oracle(torch_gpu):
-    _case("cpml adjoint f32")
+@pytest.mark.parametrize("kernel", ["direct", "tiled"])
+def test_cpml_f32_gradient_close_to_f64_oracle(torch_gpu, kernel):
+    _case("cpml adjoint f32 %s" % kernel)
     inp = _medium((16, 18, 20), seed=2)
     _, g0, gw0, d = _oracle_grad(inp)
-    _, g, gw = _gpu_grad(torch_gpu, inp, d, "f32", every=11)
+    _, g, gw = _gpu_grad(torch_gpu, inp, d, "f32", every=11, kernel=kernel)
     assert H.rel_l2(g, g0) <= 2e-4
     assert H.rel_l2(gw, gw0) <= 2e-4
 
@@ -196,12 +201,12 @@ def test_cpml_gradient_matches_central_differences(torch_gpu):
 
 
 def test_cpml_autograd_matches_misfit_gradient(torch_gpu):
-    _case("cpml adjoint autograd")
+    _case("cpml adjoint autograd (tiled)")
     torch = torch_gpu
     from pulse_echo_azimuthal_simulation_amd import adjoint
     inp = _medium(seed=5)
     _, g0, _, d = _oracle_grad(inp)
-    sh = _device_shot(inp, "f64")
+    sh = _device_shot(inp, "f64", kernel="tiled")
     dd = torch.as_tensor(d, dtype=torch.float64)
     t = torch.tensor(np.asarray(inp["Ct"], float).reshape(-1, 36), requires_grad=True)
     tr = adjoint.traces_from_table(t, sh, inp["wavelet"], inp["nt"], checkpoint_every=8)

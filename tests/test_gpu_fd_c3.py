@@ -33,8 +33,7 @@ sys.path.insert(0, ROOT)
 def fd_check(steps=4000, grains=20, h=1e-3, checkpoint_every=200, boundary="sponge"):
     """Adjoint angle gradient against central differences; returns a dict.
     ``boundary``: the workload's 20-cell sponge, or a 20-cell CPML (the
-    adjoint then runs the direct kernels; the forward runs of the
-    differences stay tiled, which are bit-identical)."""
+    adjoint then carries the memory variables' adjoints)."""
     import numpy as np
     import torch
 
@@ -74,7 +73,7 @@ def fd_check(steps=4000, grains=20, h=1e-3, checkpoint_every=200, boundary="spon
     P = fabric.angle_projections(theta, phi)
     grads = {}
     for prec in ("f64", "f32"):
-        sh = shot(Ct0, prec, "direct" if cp else "tiled")
+        sh = shot(Ct0, prec)
         d = d64.to(sh.real)
 
         def resid(tr, d=d):
