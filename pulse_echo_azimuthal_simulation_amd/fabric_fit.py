@@ -159,8 +159,9 @@ class FabricInversion:
             out.append(tr)
         return out
 
-    def misfit_gradient(self, p, data, h_alpha=1e-6, h_kappa=1e-6):
-        """(J, dJ/d(alpha, kappa)) in float64 for the fabric p."""
+    def misfit_gradient(self, p, data, h_alpha=1e-6, h_kappa=1e-6, log=None):
+        """(J, dJ/d(alpha, kappa)) in float64 for the fabric p; ``log``, if
+        given, receives a line per azimuth."""
         from . import adjoint
         torch = self.torch
         J = 0.0
@@ -185,6 +186,9 @@ class FabricInversion:
                 dC = (self.tables64(hp, k) - self.tables64(hm, k)) / (2.0 * dp[j])
                 g[j] += float(np.sum(gC * dC))
             J += Jk
+            if log:
+                log("azimuth %d/%d (%.1f deg): J_k %.4e" % (k + 1, len(self.shots),
+                                                            self.azimuths[k], Jk))
         torch.cuda.synchronize()
         return J, g
 
