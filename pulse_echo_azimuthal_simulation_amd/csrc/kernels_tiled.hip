@@ -176,29 +176,41 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
     // components any lane of the wave needs.  Each lane reads back only its
     // own slots, after the wait at the top of the next step (and the wait
     // for in-flight LDS-DMA loads that the compiler puts before a barrier's
-    // release fence).
+    // release fence).  Buffer loads rather than global_load_lds: a FLAT-
+    // encoded load that writes LDS counts as both a vector-memory and an
+    // LDS access, and the compiler then waits for both counters to drain
+    // (vmcnt(0) lgkmcnt(0)) at the first dependency that follows it.
     constexpr bool CPD = CP && T::CORR_LDS;
     auto corr_dma = [&](int zp) {
         if constexpr (CPD) {
             if (L <= 0 || zp < 0 || zp >= nz) return;
             const bool zb = pml_in(zp, nz, L);
             const int lm = col_in && (band_xy || zb) ? mask_xy | (zb ? kPmlMaskZ : 0) : 0;
-            const size_t n = (size_t)nz * sz;
-            const R *src = S.corr + (size_t)zp * sz + col;
+            const uint32_t n = (uint32_t)nz * (uint32_t)sz;
+            const int off = (int)(((uint32_t)zp * (uint32_t)sz + (uint32_t)col) * 4u);
             R *dst = cb + (zp & 1) * 6 * NT + (tid & ~63);
 #pragma unroll
             for (int k = 0; k < 6; ++k)
-                if (__any(lm >> k & 1))
-                    __builtin_amdgcn_global_load_lds(
-                        src + k * n, (__attribute__((address_space(3))) R *)(dst + k * NT), 4, 0,
+                if (__any(lm >> k & 1)) {
+                    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+                        S.corr + (size_t)k * n, (short)0, (int)(n * 4u), kBufferFlags);
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                        r, (__attribute__((address_space(3))) void *)(dst + k * NT), 4, off, 0, 0,
                         0);
+                }
         }
     };
     corr_dma(z0 - 1);
+    // the prologue's loads all complete before the loop: at the loop head the
+    // compiler then knows that only the last step's stores can be in flight
+    wait_vmem();
 
     for (int zc = z0 - 1; zc <= z1; ++zc) {
         const bool plane_in = zc >= 0 && zc < nz;
-        if constexpr (CPD) wait_vmem();     // this plane's corrections are in LDS
+        // this plane's corrections are in LDS: everything but the previous
+        // step's six output stores (issued after the corrections' DMA on
+        // every path) has completed
+        if constexpr (CPD) wait_vmem_le<6>();
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             rx[k] = rx[k + 1];
@@ -351,6 +363,18 @@ __device__ __forceinline__ void stress_tiled_body(const Shot<R> &S, int tiles_x,
                 stress_cell<true, false, R, GX, NT>(S, gt, sl, tid, zo, giy, gix,
                                                     S.Ct + (size_t)m * 36, dmp, sv, gact);
             }
+        }
+        // With the CPML's correction DMA every lane issues the six stores
+        // (stb drops those without an output): the loop body then issues a
+        // fixed number of vector-memory operations after the DMA, and the
+        // wait at the top of the next step leaves the stores in flight.
+        // Without it the stores stay under the branch (measured: the buffer
+        // resources cost the plain kernel more than the wait they save).
+        if constexpr (CPD) {
+            const uint32_t n = (uint32_t)nz * (uint32_t)sz;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) stb(S.s[k], n, io, out, sv[k]);
+        } else if (out) {
 #pragma unroll
             for (int k = 0; k < 6; ++k) stg(S.s[k], io, sv[k]);
         }

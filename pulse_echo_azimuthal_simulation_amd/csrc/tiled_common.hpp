@@ -27,6 +27,33 @@ __device__ __forceinline__ void stg(T *base, int i, T v)
     *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (uint32_t)i * (uint32_t)sizeof(T)) = v;
 }
 
+// Store of element i of an n-element field through a raw buffer resource,
+// issued by every lane: a lane with ok == false aims at byte n * sizeof(T),
+// past the resource's range, and the hardware drops its write.  Unlike a
+// store under a branch, the instruction is issued on every path, so the
+// vector-memory operations a loop body issues after a given load are a fixed
+// count and a wait for that load at the next step can leave the stores in
+// flight.  The empty asm makes the base look modified, so the resource is
+// built at each store instead of holding four scalar registers per field
+// across the caller's loop.  Needs n * sizeof(T) < 4 GiB, like ldg / stg.
+constexpr int kBufferFlags = 0x00020000;   // gfx9 raw buffer: 32-bit data format
+
+template <typename T>
+__device__ __forceinline__ void stb(T *base, uint32_t n, int i, bool ok, T v)
+{
+    __asm__ volatile("" : "+s"(base));
+    const __amdgpu_buffer_rsrc_t r =
+        __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)(n * (uint32_t)sizeof(T)),
+                                          kBufferFlags);
+    const int off = (int)((ok ? (uint32_t)i : n) * (uint32_t)sizeof(T));
+    if constexpr (sizeof(T) == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, off, 0, 0);
+    } else {
+        typedef int int2v __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(int2v, v), r, off, 0, 0);
+    }
+}
+
 // Remap a workgroup id so consecutive logical tiles land on the same XCD.
 __device__ __forceinline__ int xcd_remap(int bid, int nb)
 {
@@ -67,6 +94,16 @@ struct StressTile {
 // Vector-memory wait for every load of this wave, the LDS-DMA loads
 // included (s_waitcnt vmcnt(0); expcnt and lgkmcnt left at their maxima).
 __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// The same wait with the last N (<= 15) vector-memory operations of this
+// wave left in flight: the counter retires in issue order, so everything
+// issued before them has completed (s_waitcnt vmcnt(N)).
+template <int N>
+__device__ __forceinline__ void wait_vmem_le()
+{
+    static_assert(N >= 0 && N <= 15, "vmcnt field");
+    __builtin_amdgcn_s_waitcnt(0x0F70 | N);
+}
 
 // Moved strain q_{P,J} read from the LDS strain ring (see moved_direct in
 // kernels_direct.hip for the rule), laid out [6][NS][NG].  sl[d+1] is the
