@@ -41,7 +41,13 @@ constexpr int kBufferFlags = 0x00020000;   // gfx9 raw buffer: 32-bit data forma
 template <typename T>
 __device__ __forceinline__ void stb(T *base, uint32_t n, int i, bool ok, T v)
 {
-    __asm__ volatile("" : "+s"(base));
+    // (the base is wave-uniform; readfirstlane states it where the compiler
+    // cannot see it, e.g. a base read from a batch table)
+    const uint64_t a = reinterpret_cast<uint64_t>(base);
+    uint64_t u = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(a >> 32)) << 32) |
+                 (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a);
+    __asm__ volatile("" : "+s"(u));
+    base = reinterpret_cast<T *>(u);
     const __amdgpu_buffer_rsrc_t r =
         __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)(n * (uint32_t)sizeof(T)),
                                           kBufferFlags);
