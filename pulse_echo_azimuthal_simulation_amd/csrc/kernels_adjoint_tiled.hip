@@ -231,6 +231,7 @@ __global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_adj_veloc
             pz = e_at(i5, 2);
         }
         lds_barrier();
+        R nl[6] = {0, 0, 0, 0, 0, 0};
         if (col_in) {
             R o[6];
             const bool in = inner_xy && z >= 7 && z <= nz - 8;
@@ -248,8 +249,13 @@ __global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_adj_veloc
             }
 #undef PE_DT
 #pragma unroll
-            for (int k = 0; k < 6; ++k) A.ls[k][i] = sv[k] * d + o[k];
+            for (int k = 0; k < 6; ++k) nl[k] = sv[k] * d + o[k];
         }
+        // every lane issues the stores (stb drops the lanes outside the
+        // grid), so the next step's wait for its prefetches leaves them in
+        // flight
+#pragma unroll
+        for (int k = 0; k < 6; ++k) stb(A.ls[k], (uint32_t)ncell, i, col_in, nl[k]);
         lds_barrier();
     }
 }
@@ -395,6 +401,7 @@ __global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_adj_strai
             p5 = g5[i5];
         }
         lds_barrier();
+        R nl[3] = {0, 0, 0};
         if (col_in) {
             R gx, gy, gz;
             const bool in = inner_xy && z >= 7 && z <= nz - 8;
@@ -411,10 +418,13 @@ __global__ __launch_bounds__(TX *TY, (MinWaves<R, TX * TY>::v)) void k_adj_strai
                 PE_GT(true)
             }
 #undef PE_GT
-            A.lv[0][i] = lx0 * d + S.inv_h * gx;
-            A.lv[1][i] = ly0 * d + S.inv_h * gy;
-            A.lv[2][i] = lz0 * d + S.inv_h * gz;
+            nl[0] = lx0 * d + S.inv_h * gx;
+            nl[1] = ly0 * d + S.inv_h * gy;
+            nl[2] = lz0 * d + S.inv_h * gz;
         }
+        // (stores issued by every lane, as in k_adj_velocity_tiled)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) stb(A.lv[k], (uint32_t)ncell, i, col_in, nl[k]);
         lds_barrier();
     }
 }
@@ -591,11 +601,11 @@ __global__ __launch_bounds__((GbarTile<R, TX, TY>::NT)) void k_adj_gbar_tiled(
         load_ls(z + 2, nls);
         nm2 = S.mats[at(z + 3)];
         lds_barrier();
+        const int i = z * sz + col;
+        R o[6] = {0, 0, 0, 0, 0, 0};
         if (core) {
             const int sl[3] = {(z + 2) % 3, (z + 3) % 3, (z + 4) % 3};   // planes z-1, z, z+1
             const bool in = inner_xy && z >= 1 && z <= nz - 2;
-            const int i = z * sz + col;
-            R o[6];
 #define PE_GB(G)                                                                                  \
     o[0] = gbar_j<G, 0, R, GX, NG>(S, ut, sl, c, own_cur, z, iy, ix);                            \
     o[1] = gbar_j<G, 1, R, GX, NG>(S, ut, sl, c, own_cur, z, iy, ix);                            \
@@ -609,9 +619,10 @@ __global__ __launch_bounds__((GbarTile<R, TX, TY>::NT)) void k_adj_gbar_tiled(
                 PE_GB(true)
             }
 #undef PE_GB
-#pragma unroll
-            for (int k = 0; k < 6; ++k) A.work[(size_t)k * ncell + i] = o[k];
         }
+        // (stores issued by every lane, as in k_adj_velocity_tiled)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) stb(A.work + (size_t)k * ncell, (uint32_t)ncell, i, core, o[k]);
         lds_barrier();
     }
 }
