@@ -446,10 +446,6 @@ def bench_sweep(args):
     t_prep = time.perf_counter() - t0
     t1 = time.perf_counter()
     best, _ = autotune.tune(shots[0], reps=3)
-    if cw:
-        # a batch with CPML shots runs tile variant 0 (its z-chunk as tuned
-        # if the autotuner chose that variant)
-        best = {hh: best[hh] if best[hh][0] == 0 else (0, 0, best[hh][2]) for hh in (0, 1)}
     for sh in shots:
         for hh in (0, 1):
             sh.desc.tile[hh] = best[hh][0]

@@ -254,21 +254,16 @@ int batch_step_impl(const pe_batch *b, const BatchLayout &L, int32_t steps, hipS
     const auto *blk_s = reinterpret_cast<const int32_t *>((const char *)b->table + L.blk_s_off);
     const auto *blk_v = reinterpret_cast<const int32_t *>((const char *)b->table + L.blk_v_off);
     const auto *blk_p = reinterpret_cast<const int32_t *>((const char *)b->table + L.blk_p_off);
-    int vs = b->shots[0].tile[0];
+    const int vs = b->shots[0].tile[0];
     const int vv = b->shots[0].tile[1];
     int max_mat = 1, max_pml = 0;
     for (int32_t i = 0; i < b->n_shot; ++i) {
         max_mat = std::max(max_mat, (int)b->shots[i].n_mat);
         max_pml = std::max(max_pml, (int)b->shots[i].cpml_width);
     }
+    // (the one-barrier pipelined stress body has no CPML corrections: with
+    // CPML shots launch_stress_batched runs the plain kernel of its tile)
     const int np = (int)L.nb[2];
-    if (np > 0) {
-        // the one-barrier pipelined stress body has no CPML corrections: a
-        // batch with CPML shots runs the default tile instead
-        int tx, ty, pad, fr;
-        pe::tile_variants(b->shots[0].precision, 0, vs, &tx, &ty, &pad, &fr);
-        if (fr == 2) vs = 0;
-    }
     for (int32_t k = 0; k < steps; ++k) {
         hipError_t e = pe::launch_pml_corr_batched<R>(0, tab, blk_p, np, b->step, st);
         if (e == hipSuccess)

@@ -930,16 +930,13 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
     };
     // two planes per iteration: the loads one plane issues feed the next
     // without the register moves of a loop latch, each of which waits for
-    // its load (measured 0.481 -> 0.457 ms at 400x400x200, 64x8 tiles; the
-    // kernel stays within the registers of its occupancy)
+    // its load (measured 0.481 -> 0.457 ms at 400x400x200, 64x8 tiles; every
+    // variant stays within the registers of its occupancy, the CPML ones at
+    // 107-115, where the 64x8 tile went from 0.750 to 0.700 ms)
     int z = z0;
-    // (with a CPML only the 4-wave tiles, whose register bound is twice the
-    // 8-wave tiles', have room for the second step of the unrolled loop)
-    if constexpr (!CP || TX * TY <= 256) {
-        for (; z + 1 < z1; z += 2) {
-            step(z);
-            step(z + 1);
-        }
+    for (; z + 1 < z1; z += 2) {
+        step(z);
+        step(z + 1);
     }
     for (; z < z1; ++z) step(z);
 }
@@ -1332,8 +1329,14 @@ hipError_t launch_stress_batched<float>(int v, const BatchEntry<float> *tab, con
     case 9: return stress_batched_launch<float, 30, 14, 0, true>(tab, blk, nb, step, st, cp);
     case 10: return stress_batched_launch<float, 62, 14, 0>(tab, blk, nb, step, st, cp);
     case 11: return stress_batched_launch<float, 62, 14, 0, true>(tab, blk, nb, step, st, cp);
-    case 12: return stress_pipe_batched_launch<float, 32, 13>(tab, blk, nb, step, st);
-    case 13: return stress_pipe_batched_launch<float, 30, 14>(tab, blk, nb, step, st);
+    // the pipelined body has no CPML path: a batch with CPML shots runs the
+    // plain kernel of the same tile, as a single shot does
+    case 12:
+        return cp ? stress_batched_launch<float, 32, 13, 0>(tab, blk, nb, step, st, cp)
+                  : stress_pipe_batched_launch<float, 32, 13>(tab, blk, nb, step, st);
+    case 13:
+        return cp ? stress_batched_launch<float, 30, 14, 0>(tab, blk, nb, step, st, cp)
+                  : stress_pipe_batched_launch<float, 30, 14>(tab, blk, nb, step, st);
     default: return stress_batched_launch<float, 32, 13, 24>(tab, blk, nb, step, st, cp);
     }
 }
@@ -1346,7 +1349,9 @@ hipError_t launch_stress_batched<double>(int v, const BatchEntry<double> *tab, c
     case 1: return stress_batched_launch<double, 32, 4, 0>(tab, blk, nb, step, st, cp);
     case 2: return stress_batched_launch<double, 16, 8, 0>(tab, blk, nb, step, st, cp);
     case 3: return stress_batched_launch<double, 32, 6, 24>(tab, blk, nb, step, st, cp);
-    case 4: return stress_pipe_batched_launch<double, 32, 6>(tab, blk, nb, step, st);
+    case 4:
+        return cp ? stress_batched_launch<double, 32, 6, 0>(tab, blk, nb, step, st, cp)
+                  : stress_pipe_batched_launch<double, 32, 6>(tab, blk, nb, step, st);
     default: return stress_batched_launch<double, 32, 6, 0>(tab, blk, nb, step, st, cp);
     }
 }

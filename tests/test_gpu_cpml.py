@@ -132,14 +132,16 @@ def test_cpml_thin_grid_without_band_free_middle(torch_gpu, nz, precision):
 
 
 @pytest.mark.parametrize("precision,tile", [("f32", (0, 0)), ("f64", (0, 0)), ("f32", (8, 2)),
-                                            ("f32", (12, 3))])
+                                            ("f32", (12, 3)), ("f32", (13, 1)), ("f64", (4, 1))])
 def test_cpml_batch_equals_single_shots(torch_gpu, precision, tile):
     """Two CPML shots of different sizes, band widths and step counts and a
     sponge shot in one batch (graph replays of 7 steps): every trace, field
     and memory variable bit-identical to that shot's own run.  Tile
     variants: the defaults, the frame-ring stress tile with 32x8 velocity
-    tiles, and the one-barrier stress tile (which a batch with CPML shots
-    replaces by the default tile) with 64x4 velocity tiles."""
+    tiles, and the one-barrier stress tiles (which a batch with CPML shots
+    replaces by the plain kernel of the same tile, as a single shot does:
+    32x13 and 30x14 in f32, 32x6 in f64) with 64x4, 32x16 and 64x4 velocity
+    tiles."""
     print("\n[gpu case] cpml batch %s %s" % (precision, tile), flush=True)
     torch = torch_gpu
     from pulse_echo_azimuthal_simulation_amd import sweep
