@@ -132,8 +132,12 @@ def test_csr_layout():
     assert list(p["src_idx"]) == [3 * 56 + 3 * 8 + 4]
 
 
-def test_cpml_is_refused_by_adjoint_and_history():
-    """The adjoint and its strain history run the sponge boundary only."""
+def test_cpml_adjoint_needs_its_memory_variable_adjoints():
+    """The adjoint of a CPML shot carries the adjoints of the memory
+    variables (cpml_psib) and their band scratch (cpml_work), and its
+    stiffness gradient reads the strain history (the strains depend on the
+    memory variables as well as on v): pe_check_adjoint refuses a CPML
+    adjoint without them and accepts it with them."""
     lib = _abi.load()
     fake = 0x1000
     d = _desc(cpml_width=1)
@@ -150,10 +154,18 @@ def test_cpml_is_refused_by_adjoint_and_history():
     for k in range(6):
         adj.ls[k] = fake
     adj.work = fake
+    adj.wface = fake
     assert lib.pe_check_adjoint(C.byref(d), C.byref(adj)) == _abi.PE_EINVAL
     assert "CPML" in lib.pe_last_error().decode()
-    assert lib.pe_run_store_strains(C.byref(d), 0, 1, fake, None, fake, None) == _abi.PE_EINVAL
-    assert "CPML" in lib.pe_last_error().decode()
+    for k in range(3):
+        adj.cpml_psib[k] = fake
+        adj.cpml_work[k] = fake
+    assert lib.pe_check_adjoint(C.byref(d), C.byref(adj)) == _abi.PE_OK, lib.pe_last_error()
+    # a gradient (run accumulators) needs the strain history
+    adj.run_acc = fake
+    adj.hist_kind = _abi.PE_HIST_VELOCITY
+    assert lib.pe_check_adjoint(C.byref(d), C.byref(adj)) == _abi.PE_EINVAL
+    assert "strain history" in lib.pe_last_error().decode()
 
 
 def test_batch_with_cpml_shots_sizes_the_band_prepass():
