@@ -20,10 +20,12 @@ from . import _abi
 
 _CACHE: dict = {}
 # z planes per workgroup (0: the library's default, at least 16).  Short
-# chunks give small grids enough workgroups: on the 96x96x48 disk (+ 23-cell
-# pads) with a 20-cell CPML, 6-plane chunks run 6.3 G cell-updates/s against
-# 4.9 with the default.
-ZCHUNKS = (0, 6, 8, 12, 25, 50, 100)
+# chunks give small grids enough workgroups (on the 96x96x48 disk + 23-cell
+# pads the stress half-step picks 12 planes); between them the count of
+# workgroup rounds per launch and the halo planes of each chunk trade off
+# (400x400x200 with a CPML: the 64x4 velocity tile runs 0.667 ms in 25-plane
+# chunks, 0.646 in 34-plane ones).
+ZCHUNKS = (0, 6, 8, 12, 16, 20, 25, 34, 40, 50, 100)
 
 
 def _time_half(shot, half, reps):
