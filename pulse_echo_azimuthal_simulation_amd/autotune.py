@@ -21,11 +21,23 @@ from . import _abi
 _CACHE: dict = {}
 # z planes per workgroup (0: the library's default, at least 16).  Short
 # chunks give small grids enough workgroups (on the 96x96x48 disk + 23-cell
-# pads the stress half-step picks 12 planes); between them the count of
-# workgroup rounds per launch and the halo planes of each chunk trade off
-# (400x400x200 with a CPML: the 64x4 velocity tile runs 0.667 ms in 25-plane
-# chunks, 0.646 in 34-plane ones).
-ZCHUNKS = (0, 6, 8, 12, 16, 20, 25, 34, 40, 50, 100)
+# pads the stress half-step picks 12 planes).  Beyond that the time depends
+# on the chunk count, sharply and not monotonically (workgroup rounds per
+# launch against the halo planes each chunk re-reads), and an uneven last
+# chunk costs: at 400x400x200 the plain stress kernel runs 0.917 ms in
+# 25-plane chunks (8 even ones), 0.978 in 24 (9, the last of 8 planes),
+# 0.951 in 23 (9, the last of 16) and 0.902 in 29 (7, the last of 26).  So the
+# candidates are, besides the default and the short chunks, ceil(nz / k) for
+# k = 1 .. 16 chunks: for each count the most even split.
+SHORT_ZCHUNKS = (0, 6, 8, 12)
+
+
+def zchunk_candidates(nz, kmax=16):
+    """Z-chunk lengths the autotuner times on an nz-plane grid."""
+    c = set(SHORT_ZCHUNKS)
+    for k in range(1, kmax + 1):
+        c.add(-(-nz // k))
+    return tuple(sorted(z for z in c if z <= nz))
 
 
 def _time_half(shot, half, reps):
@@ -42,10 +54,13 @@ def _time_half(shot, half, reps):
     return a.elapsed_time(b) / reps
 
 
-def tune(shot, reps=5, zchunks=ZCHUNKS, verbose=False):
-    """Pick tile variants and z-chunk lengths for ``shot`` (a DeviceShot with
-    zero fields); sets them in its descriptor and returns
-    {half: (variant, zchunk, ms)} plus the full timing table."""
+def tune(shot, reps=5, zchunks=None, verbose=False):
+    """Pick tile variants and z-chunk lengths (``zchunks``, default
+    zchunk_candidates of the grid's nz) for ``shot`` (a DeviceShot with zero
+    fields); sets them in its descriptor and returns {half: (variant, zchunk,
+    ms)} plus the full timing table."""
+    if zchunks is None:
+        zchunks = zchunk_candidates(int(shot.shape[0]))
     key = (shot.shape, shot.desc.precision, shot.desc.cpml_width)
     if key in _CACHE:
         best = _CACHE[key]
