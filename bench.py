@@ -337,9 +337,10 @@ def main():
         del shot, traces
         torch.cuda.empty_cache()
         sp = make_shot("sponge")
+        tuned2 = None
         if args.kernel == "tiled" and not args.no_tune:
-            from pulse_echo_azimuthal_simulation_amd import autotune
-            autotune.tune(sp)
+            from pulse_echo_azimuthal_simulation_amd import _abi, autotune
+            tuned2, _ = autotune.tune(sp)
         tr2 = sp.new_traces(args.sponge_steps)
         sp.run(wav, 0, min(args.warmup, args.sponge_steps), tr2)
         torch.cuda.synchronize()
@@ -360,6 +361,10 @@ def main():
                 traffic2 = json.load(fh).get(dom2)
         out["sponge20"] = {"value": round(ncell * args.sponge_steps / w2 / 1e9, 4),
                            "steps": args.sponge_steps,
+                           "launch": None if tuned2 is None else
+                           {h: {"tile": list(_abi.tile_variants(p, i)[tuned2[i][0]]),
+                                "zchunk": tuned2[i][1]}
+                            for i, h in ((0, "stress"), (1, "velocity"))},
                            "kernels_ms_per_step": {"stress": round(ts2 * 1e3, 5),
                                                    "velocity": round(tv2 * 1e3, 5)},
                            "roofline": {"kernel": dom2, "bound": "hbm", "achieved": round(ach2, 1),
