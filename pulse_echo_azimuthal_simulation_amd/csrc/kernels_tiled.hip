@@ -1275,7 +1275,8 @@ constexpr TileInfo kStressF32[] = {{32, 13, 24, 0}, {62, 6, 0, 0},  {30, 14, 24,
                                    {32, 16, 24, 0}, {32, 8, 24, 0}, {16, 16, 0, 0},
                                    {32, 16, 0, 0},  {30, 14, 24, 1}, {32, 13, 24, 1},
                                    {30, 14, 0, 1},  {62, 14, 0, 0}, {62, 14, 0, 1},
-                                   {32, 13, 0, 2},  {30, 14, 0, 2}};
+                                   {32, 13, 0, 2},  {30, 14, 0, 2}, {40, 10, 24, 1},
+                                   {40, 10, 24, 0}};
 constexpr TileInfo kStressF64[] = {{32, 6, 0, 0}, {32, 4, 0, 0}, {16, 8, 0, 0}, {32, 6, 24, 0},
                                    {32, 6, 0, 2}};
 constexpr TileInfo kVelF32[] = {{64, 8, 0, 0},  {32, 16, 0, 0}, {32, 8, 0, 0},
@@ -1337,6 +1338,8 @@ hipError_t launch_stress_batched<float>(int v, const BatchEntry<float> *tab, con
     case 13:
         return cp ? stress_batched_launch<float, 30, 14, 0>(tab, blk, nb, step, st, cp)
                   : stress_pipe_batched_launch<float, 30, 14>(tab, blk, nb, step, st);
+    case 14: return stress_batched_launch<float, 40, 10, 24, true>(tab, blk, nb, step, st, cp);
+    case 15: return stress_batched_launch<float, 40, 10, 24>(tab, blk, nb, step, st, cp);
     default: return stress_batched_launch<float, 32, 13, 24>(tab, blk, nb, step, st, cp);
     }
 }
@@ -1422,6 +1425,8 @@ hipError_t launch_stress_tiled<float>(const Shot<float> &s, int v, int zc, hipSt
     case 11: return stress_launch<float, 62, 14, 0, true>(s, zc, st, g);
     case 12: return stress_pipe_launch<float, 32, 13>(s, zc, st, g);
     case 13: return stress_pipe_launch<float, 30, 14>(s, zc, st, g);
+    case 14: return stress_launch<float, 40, 10, 24, true>(s, zc, st, g);
+    case 15: return stress_launch<float, 40, 10, 24>(s, zc, st, g);
     default: return stress_launch<float, 32, 13, 24>(s, zc, st, g);
     }
 }

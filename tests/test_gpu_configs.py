@@ -44,8 +44,9 @@ def cfg_c1():
     return specimen.config_c1(nt=8)
 
 
-def _shots(cfg, boundary):
-    """DeviceShot (tiled f32, 25-plane chunks) and OracleShot of one config."""
+def _shots(cfg, boundary, tile=(0, 0), zchunk=(25, 25)):
+    """DeviceShot (tiled f32, by default tile variant 0 in 25-plane chunks)
+    and OracleShot of one config."""
     from oracle import oracle as O
     from pulse_echo_azimuthal_simulation_amd import cpml, solver
     sponge = 0 if boundary == "cpml" else cfg["sponge_width"]
@@ -60,7 +61,7 @@ def _shots(cfg, boundary):
     sh = solver.DeviceShot(prep["mats"], prep["Ct"], prep["rho_t"], prep["damp"], prep["coeffs"],
                            cfg["h"], cfg["dt"], prep["src_idx"], prep["src_w"], prep["rec_ptr"],
                            prep["rec_idx"], prep["rec_w"], precision="f32", kernel="tiled",
-                           zchunk=(25, 25), cpml=cp)
+                           tile=tile, zchunk=zchunk, cpml=cp)
     osh = O.OracleShot(prep["mats"], prep["Ct"], prep["rho_t"], prep["damp"], prep["coeffs"],
                        cfg["h"], cfg["dt"], (prep["src_idx"], prep["src_w"]),
                        (prep["rec_ptr"], prep["rec_idx"], prep["rec_w"]), precision="f32",
@@ -68,11 +69,18 @@ def _shots(cfg, boundary):
     return sh, osh
 
 
-@pytest.mark.parametrize("boundary,nt", [("cpml", 3), ("sponge", 2)])
-def test_c1_grid_from_random_state_matches_oracle(torch_gpu, cfg_c1, boundary, nt):
-    _case("400x400x200 %s, %d steps from a random state" % (boundary, nt))
+# the default tiles, and the 40x10 stress tiles (frame ring for the sponge,
+# correction double buffer for the CPML) in 29-plane chunks that the
+# autotuner picks on this grid
+@pytest.mark.parametrize("boundary,nt,tile,zchunk", [("cpml", 3, (0, 0), (25, 25)),
+                                                     ("sponge", 2, (0, 0), (25, 25)),
+                                                     ("cpml", 3, (15, 3), (29, 29)),
+                                                     ("sponge", 2, (14, 0), (29, 29))])
+def test_c1_grid_from_random_state_matches_oracle(torch_gpu, cfg_c1, boundary, nt, tile, zchunk):
+    _case("400x400x200 %s, %d steps from a random state, tiles %s, chunks %s"
+          % (boundary, nt, tile, zchunk))
     torch = torch_gpu
-    sh, osh = _shots(cfg_c1, boundary)
+    sh, osh = _shots(cfg_c1, boundary, tile, zchunk)
     assert sh.shape == (200, 400, 400)
     H.random_state(torch, sh, osh, seed=3)
     tr = sh.new_traces(nt)
