@@ -54,11 +54,13 @@ def _time_half(shot, half, reps):
     return a.elapsed_time(b) / reps
 
 
-def tune(shot, reps=5, zchunks=None, verbose=False):
-    """Pick tile variants and z-chunk lengths (``zchunks``, default
-    zchunk_candidates of the grid's nz) for ``shot`` (a DeviceShot with zero
-    fields); sets them in its descriptor and returns {half: (variant, zchunk,
-    ms)} plus the full timing table."""
+def tune(shot, reps=5, zchunks=None, verbose=False, keep=3):
+    """Pick tile variants and z-chunk lengths for ``shot`` (a DeviceShot with
+    zero fields); sets them in its descriptor and returns {half: (variant,
+    zchunk, ms)} plus the full timing table.  Two rounds per half-step: every
+    tile variant at the library's default chunk, then the ``keep`` fastest
+    of them at every chunk of ``zchunks`` (default zchunk_candidates of the
+    grid's nz), which costs a few times less than timing every pair."""
     if zchunks is None:
         zchunks = zchunk_candidates(int(shot.shape[0]))
     key = (shot.shape, shot.desc.precision, shot.desc.cpml_width)
@@ -70,20 +72,27 @@ def tune(shot, reps=5, zchunks=None, verbose=False):
     prec = "f32" if shot.desc.precision == _abi.PE_F32 else "f64"
     table = {}
     best = {}
+
+    def timed(h, v, zc):
+        if (h, v, zc) not in table:
+            shot.desc.tile[h] = v
+            shot.desc.zchunk[h] = zc
+            _abi.check(shot.lib.pe_check_shot(C.byref(shot.desc)), "pe_check_shot")
+            table[(h, v, zc)] = _time_half(shot, h, reps)
+            if verbose:
+                print("tune half %d tile %s zchunk %d: %.4f ms"
+                      % (h, _abi.tile_variants(prec, h)[v], zc, table[(h, v, zc)]), flush=True)
+        ms = table[(h, v, zc)]
+        if h not in best or ms < best[h][2]:
+            best[h] = (v, zc, ms)
+        return ms
+
     for h in (0, 1):
-        variants = _abi.tile_variants(prec, h)
-        for v in range(len(variants)):
+        nv = len(_abi.tile_variants(prec, h))
+        first = sorted(range(nv), key=lambda v: timed(h, v, 0))
+        for v in first[:keep]:
             for zc in zchunks:
-                shot.desc.tile[h] = v
-                shot.desc.zchunk[h] = zc
-                _abi.check(shot.lib.pe_check_shot(C.byref(shot.desc)), "pe_check_shot")
-                ms = _time_half(shot, h, reps)
-                table[(h, v, zc)] = ms
-                if verbose:
-                    print("tune half %d tile %s zchunk %d: %.4f ms" % (h, variants[v], zc, ms),
-                          flush=True)
-                if h not in best or ms < best[h][2]:
-                    best[h] = (v, zc, ms)
+                timed(h, v, zc)
         shot.desc.tile[h], shot.desc.zchunk[h] = best[h][0], best[h][1]
     _CACHE[key] = best
     return best, table
