@@ -150,12 +150,16 @@ class FabricInversion:
     """
 
     def __init__(self, build, azimuths, h, coeffs, f0, nt, lat, sponge=10,
-                 checkpoint_every=100, precision="f32", element_d_mm=6.35):
+                 checkpoint_every=100, precision="f32", element_d_mm=6.35, history_gb=None):
         from . import adjoint, sweep
         self.azimuths = [float(a) for a in azimuths]
         self.lat = lat
         self.nt = int(nt)
         self.checkpoint_every = int(checkpoint_every)
+        # first-pass history kept in HBM per azimuth (segments that do not
+        # fit are recomputed from their checkpoints); None: the free device
+        # memory less 16 GiB at each evaluation
+        self.history_gb = history_gb
         prep = sweep.GpuAzimuthPrep(build, h, coeffs, f0, sponge=sponge,
                                     element_d_mm=element_d_mm)
         G = len(lat[0])
@@ -220,9 +224,17 @@ class FabricInversion:
                 r = tr - d
                 return 0.5 * float((r.double() ** 2).sum()), r
 
+            if self.history_gb is None:
+                # (the caching allocator's free blocks, the previous
+                # azimuth's history among them, count as free)
+                free = torch.cuda.mem_get_info(sh.device)[0] + \
+                    torch.cuda.memory_reserved(sh.device) - torch.cuda.memory_allocated(sh.device)
+                hist_bytes = max(0, free - (16 << 30))
+            else:
+                hist_bytes = int(self.history_gb * (1 << 30))
             Jk, gC, _ = adjoint.misfit_gradient(sh, self.wavs[k], self.nt, resid,
                                                 checkpoint_every=self.checkpoint_every,
-                                                adjoint=self.adj[k])
+                                                adjoint=self.adj[k], history_bytes=hist_bytes)
             gC = gC.cpu().numpy()
             for j, dp in enumerate(steps):
                 hp = np.asarray(p, float) + dp
