@@ -230,16 +230,40 @@ __global__ __launch_bounds__(kBlock) void k_adj_pml_pre(Shot<R> S, Adj<R> A)
     }
 }
 
-// The derivative adjoints of axis AX, component K, as a field over the
-// cells (0 outside the band), for the transposed stencils of diff_t.
-template <int AX, int K, typename R>
-__device__ __forceinline__ auto band_field(const Shot<R> &S, const Adj<R> &A)
+// The transposed stencil of diff_t applied to the derivative adjoints of
+// axis AX, component K (A.pw, taken as a field over the cells that is 0
+// outside the band), with every tap loaded before the arithmetic: a tap
+// outside the band (or the grid) reads the cell at band index 0 of its row
+// and is then dropped, so the wave waits for its loads once instead of once
+// per guarded tap.  The terms and their guards are diff_t's.
+template <bool FWD, int AX, int K, typename R>
+__device__ __forceinline__ R band_diff_t(const Shot<R> &S, const Adj<R> &A, int kz, int ky, int kx)
 {
-    return [&S, &A](int, int iz, int iy, int ix) -> R {
-        const int n = extent<AX>(S), c = coord<AX>(iz, iy, ix);
-        if (!pml_in(c, n, S.pml)) return (R)0;
-        return A.pw[AX][(size_t)K * band_vol<AX>(S) + band_flat<AX>(S, iz, iy, ix)];
+    const int n = extent<AX>(S), kc = coord<AX>(kz, ky, kx), L = S.pml, W = 2 * L + 2;
+    const R *pw = A.pw[AX] + (size_t)K * band_vol<AX>(S);
+    auto flat = [&](int c) {
+        const int b = (c >= 0 && c < n && pml_in(c, n, L)) ? (c <= L ? c : c - (n - W)) : 0;
+        if constexpr (AX == 0) return (b * S.ny + ky) * S.nx + kx;
+        else if constexpr (AX == 1) return (kz * W + b) * S.nx + kx;
+        else return (kz * S.ny + ky) * W + b;
     };
+    R vlo[4], vhi[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        vlo[j] = pw[flat(kc - (FWD ? j + 1 : j))];
+        vhi[j] = pw[flat(kc + (FWD ? j : j + 1))];
+    }
+    R acc = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (j >= S.nc) break;
+        const int lo = FWD ? j + 1 : j, hi = FWD ? j : j + 1;
+        R t = 0;
+        if (kc >= 2 * j + 1) t = pml_in(kc - lo, n, L) ? vlo[j] : (R)0;
+        if (kc <= n - 2 * j - 2) t = t - (pml_in(kc + hi, n, L) ? vhi[j] : (R)0);
+        acc = fma_r(S.c[j], t, acc);
+    }
+    return acc;
 }
 
 // Is cell (kz, ky, kx) within the taps' reach (4 cells) of the band of axis AX?
@@ -260,19 +284,19 @@ __device__ __forceinline__ void vel_band_t(const Shot<R> &S, const Adj<R> &A, in
                                            R &d6)
 {
     if (near_band<0>(S, kz, ky, kx)) {
-        d5 = d5 + diff_t<false, 0>(S, band_field<0, 0>(S, A), k, kz, ky, kx);
-        d4 = d4 + diff_t<false, 0>(S, band_field<0, 1>(S, A), k, kz, ky, kx);
-        d3 = d3 + diff_t<true, 0>(S, band_field<0, 2>(S, A), k, kz, ky, kx);
+        d5 = d5 + band_diff_t<false, 0, 0>(S, A, kz, ky, kx);
+        d4 = d4 + band_diff_t<false, 0, 1>(S, A, kz, ky, kx);
+        d3 = d3 + band_diff_t<true, 0, 2>(S, A, kz, ky, kx);
     }
     if (near_band<1>(S, kz, ky, kx)) {
-        d6 = d6 + diff_t<false, 1>(S, band_field<1, 0>(S, A), k, kz, ky, kx);
-        d2 = d2 + diff_t<true, 1>(S, band_field<1, 1>(S, A), k, kz, ky, kx);
-        d4 = d4 + diff_t<false, 1>(S, band_field<1, 2>(S, A), k, kz, ky, kx);
+        d6 = d6 + band_diff_t<false, 1, 0>(S, A, kz, ky, kx);
+        d2 = d2 + band_diff_t<true, 1, 1>(S, A, kz, ky, kx);
+        d4 = d4 + band_diff_t<false, 1, 2>(S, A, kz, ky, kx);
     }
     if (near_band<2>(S, kz, ky, kx)) {
-        d1 = d1 + diff_t<true, 2>(S, band_field<2, 0>(S, A), k, kz, ky, kx);
-        d6 = d6 + diff_t<false, 2>(S, band_field<2, 1>(S, A), k, kz, ky, kx);
-        d5 = d5 + diff_t<false, 2>(S, band_field<2, 2>(S, A), k, kz, ky, kx);
+        d1 = d1 + band_diff_t<true, 2, 0>(S, A, kz, ky, kx);
+        d6 = d6 + band_diff_t<false, 2, 1>(S, A, kz, ky, kx);
+        d5 = d5 + band_diff_t<false, 2, 2>(S, A, kz, ky, kx);
     }
 }
 
@@ -285,19 +309,19 @@ __device__ __forceinline__ void strain_band_t(const Shot<R> &S, const Adj<R> &A,
                                               int ky, int kx, R &gx, R &gy, R &gz)
 {
     if (near_band<0>(S, kz, ky, kx)) {
-        gz = gz + diff_t<false, 0>(S, band_field<0, 0>(S, A), k, kz, ky, kx);
-        gy = gy + diff_t<true, 0>(S, band_field<0, 1>(S, A), k, kz, ky, kx);
-        gx = gx + diff_t<true, 0>(S, band_field<0, 2>(S, A), k, kz, ky, kx);
+        gz = gz + band_diff_t<false, 0, 0>(S, A, kz, ky, kx);
+        gy = gy + band_diff_t<true, 0, 1>(S, A, kz, ky, kx);
+        gx = gx + band_diff_t<true, 0, 2>(S, A, kz, ky, kx);
     }
     if (near_band<1>(S, kz, ky, kx)) {
-        gy = gy + diff_t<false, 1>(S, band_field<1, 0>(S, A), k, kz, ky, kx);
-        gz = gz + diff_t<true, 1>(S, band_field<1, 1>(S, A), k, kz, ky, kx);
-        gx = gx + diff_t<true, 1>(S, band_field<1, 2>(S, A), k, kz, ky, kx);
+        gy = gy + band_diff_t<false, 1, 0>(S, A, kz, ky, kx);
+        gz = gz + band_diff_t<true, 1, 1>(S, A, kz, ky, kx);
+        gx = gx + band_diff_t<true, 1, 2>(S, A, kz, ky, kx);
     }
     if (near_band<2>(S, kz, ky, kx)) {
-        gx = gx + diff_t<false, 2>(S, band_field<2, 0>(S, A), k, kz, ky, kx);
-        gz = gz + diff_t<true, 2>(S, band_field<2, 1>(S, A), k, kz, ky, kx);
-        gy = gy + diff_t<true, 2>(S, band_field<2, 2>(S, A), k, kz, ky, kx);
+        gx = gx + band_diff_t<false, 2, 0>(S, A, kz, ky, kx);
+        gz = gz + band_diff_t<true, 2, 1>(S, A, kz, ky, kx);
+        gy = gy + band_diff_t<true, 2, 2>(S, A, kz, ky, kx);
     }
 }
 
