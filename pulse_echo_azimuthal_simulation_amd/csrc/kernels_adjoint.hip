@@ -138,16 +138,35 @@ __device__ __forceinline__ int band_flat(const Shot<R> &S, int iz, int iy, int i
     else return (iz * S.ny + iy) * W + (ix <= L ? ix : ix - (S.nx - W));
 }
 
-template <int AX, int K, int ST, typename R>
-__device__ __forceinline__ R pml_adj_d(const Shot<R> &S, const Adj<R> &A, int iz, int iy, int ix,
-                                       R ebar)
+// The adjoint of the corrected derivatives of band axis AX at one cell
+// (memory variables K0 .. K0+2, staggers S0, S1, S2; e0..e2 their ebar):
+// P = ebar + pbar, the adjoint memory variable b P stored over pbar, the
+// adjoint of dd, (1/kappa - 1) ebar + a P, into A.pw[AX][0..2][band cell].
+// Every load of the axis (adjoint memory variables, profile values) comes
+// ahead of its arithmetic and stores, so that the three wait for memory once.
+template <int AX, int K0, int S0, int S1, int S2, typename R>
+__device__ __forceinline__ void pml_adj_axis(const Shot<R> &S, const Adj<R> &A, int iz, int iy,
+                                             int ix, R e0, R e1, R e2)
 {
     const int n = extent<AX>(S), ia = coord<AX>(iz, iy, ix);
-    const R *pr = S.prof[AX] + (size_t)ST * 3 * n;
-    R *pb = A.psib[AX] + (size_t)K * band_vol<AX>(S) + band_flat<AX>(S, iz, iy, ix);
-    const R P = ebar + *pb;
-    *pb = pr[ia] * P;
-    return (pr[2 * n + ia] - (R)1) * ebar + pr[n + ia] * P;
+    const size_t v = (size_t)band_vol<AX>(S);
+    const int f = band_flat<AX>(S, iz, iy, ix);
+    R *pb = A.psib[AX] + (size_t)K0 * v + f;
+    R *d = A.pw[AX] + f;
+    const R *p0 = S.prof[AX] + (size_t)S0 * 3 * n;
+    const R *p1 = S.prof[AX] + (size_t)S1 * 3 * n;
+    const R *p2 = S.prof[AX] + (size_t)S2 * 3 * n;
+    const R q0 = pb[0], q1 = pb[v], q2 = pb[2 * v];
+    const R b0 = p0[ia], a0 = p0[n + ia], k0 = p0[2 * n + ia];
+    const R b1 = p1[ia], a1 = p1[n + ia], k1 = p1[2 * n + ia];
+    const R b2 = p2[ia], a2 = p2[n + ia], k2 = p2[2 * n + ia];
+    const R P0 = e0 + q0, P1 = e1 + q1, P2 = e2 + q2;
+    pb[0] = b0 * P0;
+    pb[v] = b1 * P1;
+    pb[2 * v] = b2 * P2;
+    d[0] = (k0 - (R)1) * e0 + a0 * P0;
+    d[v] = (k1 - (R)1) * e1 + a1 * P1;
+    d[2 * v] = (k2 - (R)1) * e2 + a2 * P2;
 }
 
 // One thread per band cell (pml_band_cell order): the adjoints of the
@@ -190,43 +209,16 @@ __global__ __launch_bounds__(kBlock) void k_adj_pml_pre(Shot<R> S, Adj<R> A)
         w2 = g[5 * n + i];
     }
     if (pml_in(iz, S.nz, L)) {
-        R *d = A.pw[0] + band_flat<0>(S, iz, iy, ix);
-        const size_t v = (size_t)band_vol<0>(S);
-        if constexpr (HALF == 1) {
-            d[0] = pml_adj_d<0, 3, 0>(S, A, iz, iy, ix, t0);
-            d[v] = pml_adj_d<0, 4, 0>(S, A, iz, iy, ix, t1);
-            d[2 * v] = pml_adj_d<0, 5, 1>(S, A, iz, iy, ix, t2);
-        } else {
-            d[0] = pml_adj_d<0, 0, 0>(S, A, iz, iy, ix, t0);
-            d[v] = pml_adj_d<0, 1, 1>(S, A, iz, iy, ix, t1);
-            d[2 * v] = pml_adj_d<0, 2, 1>(S, A, iz, iy, ix, t2);
-        }
+        if constexpr (HALF == 1) pml_adj_axis<0, 3, 0, 0, 1>(S, A, iz, iy, ix, t0, t1, t2);
+        else pml_adj_axis<0, 0, 0, 1, 1>(S, A, iz, iy, ix, t0, t1, t2);
     }
     if (pml_in(iy, S.ny, L)) {
-        R *d = A.pw[1] + band_flat<1>(S, iz, iy, ix);
-        const size_t v = (size_t)band_vol<1>(S);
-        if constexpr (HALF == 1) {
-            d[0] = pml_adj_d<1, 3, 0>(S, A, iz, iy, ix, u0);
-            d[v] = pml_adj_d<1, 4, 1>(S, A, iz, iy, ix, u1);
-            d[2 * v] = pml_adj_d<1, 5, 0>(S, A, iz, iy, ix, u2);
-        } else {
-            d[0] = pml_adj_d<1, 0, 0>(S, A, iz, iy, ix, u0);
-            d[v] = pml_adj_d<1, 1, 1>(S, A, iz, iy, ix, u1);
-            d[2 * v] = pml_adj_d<1, 2, 1>(S, A, iz, iy, ix, u2);
-        }
+        if constexpr (HALF == 1) pml_adj_axis<1, 3, 0, 1, 0>(S, A, iz, iy, ix, u0, u1, u2);
+        else pml_adj_axis<1, 0, 0, 1, 1>(S, A, iz, iy, ix, u0, u1, u2);
     }
     if (pml_in(ix, S.nx, L)) {
-        R *d = A.pw[2] + band_flat<2>(S, iz, iy, ix);
-        const size_t v = (size_t)band_vol<2>(S);
-        if constexpr (HALF == 1) {
-            d[0] = pml_adj_d<2, 3, 1>(S, A, iz, iy, ix, w0);
-            d[v] = pml_adj_d<2, 4, 0>(S, A, iz, iy, ix, w1);
-            d[2 * v] = pml_adj_d<2, 5, 0>(S, A, iz, iy, ix, w2);
-        } else {
-            d[0] = pml_adj_d<2, 0, 0>(S, A, iz, iy, ix, w0);
-            d[v] = pml_adj_d<2, 1, 1>(S, A, iz, iy, ix, w1);
-            d[2 * v] = pml_adj_d<2, 2, 1>(S, A, iz, iy, ix, w2);
-        }
+        if constexpr (HALF == 1) pml_adj_axis<2, 3, 1, 0, 0>(S, A, iz, iy, ix, w0, w1, w2);
+        else pml_adj_axis<2, 0, 0, 1, 1>(S, A, iz, iy, ix, w0, w1, w2);
     }
 }
 
