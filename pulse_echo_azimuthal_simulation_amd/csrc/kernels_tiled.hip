@@ -1009,6 +1009,27 @@ __device__ __forceinline__ void pml_strain_axis(const Shot<R> &S, int i, int iz,
         t2[2 * j] = f2[i + (pf - ia) * st];
         t2[2 * j + 1] = f2[i + (mf - ia) * st];
     }
+    // pml_corr<A, 0, 0>, <A, 1, 1>, <A, 2, 1> written out with their memory
+    // variables and profile values loaded together with the taps (a memory
+    // variable's store would otherwise keep the next one's load behind it)
+    const int L = S.pml, W = 2 * L + 2;
+    const int bi = ia <= L ? ia : ia - (n - W);
+    size_t vol, f;
+    if constexpr (A == 0) {
+        vol = (size_t)W * S.ny * S.nx;
+        f = ((size_t)bi * S.ny + iy) * S.nx + ix;
+    } else if constexpr (A == 1) {
+        vol = (size_t)S.nz * W * S.nx;
+        f = ((size_t)iz * W + bi) * S.nx + ix;
+    } else {
+        vol = (size_t)S.nz * S.ny * W;
+        f = ((size_t)iz * S.ny + iy) * W + bi;
+    }
+    R *psi = S.psi[A] + f;
+    const R *pr0 = S.prof[A], *pr1 = S.prof[A] + (size_t)3 * n;
+    const R q0 = psi[0], q1 = psi[vol], q2 = psi[2 * vol];
+    const R b0 = pr0[ia], a0 = pr0[n + ia], k0 = pr0[2 * n + ia];
+    const R b1 = pr1[ia], a1 = pr1[n + ia], k1 = pr1[2 * n + ia];
     R d0 = 0, d1 = 0, d2 = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -1020,9 +1041,15 @@ __device__ __forceinline__ void pml_strain_axis(const Shot<R> &S, int i, int iz,
             d2 = fma_r(cj, t2[2 * j] - t2[2 * j + 1], d2);
         }
     }
-    ga[K0] = ga[K0] + pml_corr<A, 0, 0>(S, iz, iy, ix, d0 * S.inv_h);
-    ga[K1] = ga[K1] + pml_corr<A, 1, 1>(S, iz, iy, ix, d1 * S.inv_h);
-    ga[K2] = ga[K2] + pml_corr<A, 2, 1>(S, iz, iy, ix, d2 * S.inv_h);
+    const R dd0 = d0 * S.inv_h, dd1 = d1 * S.inv_h, dd2 = d2 * S.inv_h;
+    const R p0 = fma_r(b0, q0, a0 * dd0), p1 = fma_r(b1, q1, a1 * dd1),
+            p2 = fma_r(b1, q2, a1 * dd2);
+    psi[0] = p0;
+    psi[vol] = p1;
+    psi[2 * vol] = p2;
+    ga[K0] = ga[K0] + fma_r(dd0, k0 - (R)1, p0);
+    ga[K1] = ga[K1] + fma_r(dd1, k1 - (R)1, p1);
+    ga[K2] = ga[K2] + fma_r(dd2, k1 - (R)1, p2);
 }
 
 template <typename R>
