@@ -781,27 +781,37 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
         const bool bz = CP && L > 0 && gcol && pml_in(z, nz, L);
         // (memory-variable offsets in 32 bits: the host caps each axis's
         // 6 x band-volume array under 4 GiB)
+        // oz, oy, ox: offset of memory variable 3 of each band axis, formed
+        // once here so that the stores below take adds only (re-derived
+        // there, the compiler folded it into a 64-bit multiply-add whose
+        // unused high half sat in a prefetch's destination register, and
+        // waited for every prefetch of the next plane at that point; other
+        // waves hid that wait: 0.644 ms per step before and after at
+        // 400x400x200)
         R ps[9];
-        int fz = 0, fy = 0, fx = 0, vz_ = 0, vy_ = 0, vx_ = 0;
+        int oz = 0, oy = 0, ox = 0, vz_ = 0, vy_ = 0, vx_ = 0;
         if constexpr (CP) {
             if (bz) {
                 const int biz = z <= L ? z : z - (nz - W);
                 vz_ = W * ny * nx;
-                fz = (biz * ny + giy) * nx + gix;
+                oz = 3 * vz_ + (biz * ny + giy) * nx + gix;
+                __asm__ volatile("" : "+v"(oz));
 #pragma unroll
-                for (int k = 0; k < 3; ++k) ps[k] = ldg(S.psi[0], (3 + k) * vz_ + fz);
+                for (int k = 0; k < 3; ++k) ps[k] = ldg(S.psi[0], oz + k * vz_);
             }
             if (gby) {
                 vy_ = nz * W * nx;
-                fy = (z * W + gbiy) * nx + gix;
+                oy = 3 * vy_ + (z * W + gbiy) * nx + gix;
+                __asm__ volatile("" : "+v"(oy));
 #pragma unroll
-                for (int k = 0; k < 3; ++k) ps[3 + k] = ldg(S.psi[1], (3 + k) * vy_ + fy);
+                for (int k = 0; k < 3; ++k) ps[3 + k] = ldg(S.psi[1], oy + k * vy_);
             }
             if (gbx) {
                 vx_ = nz * ny * W;
-                fx = (z * ny + giy) * W + gbix;
+                ox = 3 * vx_ + (z * ny + giy) * W + gbix;
+                __asm__ volatile("" : "+v"(ox));
 #pragma unroll
-                for (int k = 0; k < 3; ++k) ps[6 + k] = ldg(S.psi[2], (3 + k) * vx_ + fx);
+                for (int k = 0; k < 3; ++k) ps[6 + k] = ldg(S.psi[2], ox + k * vx_);
             }
         }
         // (1) plane z of s1, s2, s4, s5, s6 into LDS
@@ -861,7 +871,7 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
                         g1 = g1 + pml_corr_lds<0, 0>(bp, W, biz, ps[1], e1);
                         g2 = g2 + pml_corr_lds<0, 1>(bp, W, biz, ps[2], e2);
 #pragma unroll
-                        for (int k = 0; k < 3; ++k) stg(S.psi[0], (3 + k) * vz_ + fz, ps[k]);
+                        for (int k = 0; k < 3; ++k) stg(S.psi[0], oz + k * vz_, ps[k]);
                     }
                     if (gby) {
                         R e0 = 0, e1 = 0, e2 = 0;
@@ -880,7 +890,7 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
                         g1 = g1 + pml_corr_lds<1, 1>(bp, W, gbiy, ps[4], e1);
                         g2 = g2 + pml_corr_lds<1, 0>(bp, W, gbiy, ps[5], e2);
 #pragma unroll
-                        for (int k = 0; k < 3; ++k) stg(S.psi[1], (3 + k) * vy_ + fy, ps[3 + k]);
+                        for (int k = 0; k < 3; ++k) stg(S.psi[1], oy + k * vy_, ps[3 + k]);
                     }
                     if (gbx) {
                         R e0 = 0, e1 = 0, e2 = 0;
@@ -898,7 +908,7 @@ __device__ __forceinline__ void velocity_tiled_body(const Shot<R> &S, int tiles_
                         g1 = g1 + pml_corr_lds<2, 0>(bp, W, gbix, ps[7], e1);
                         g2 = g2 + pml_corr_lds<2, 0>(bp, W, gbix, ps[8], e2);
 #pragma unroll
-                        for (int k = 0; k < 3; ++k) stg(S.psi[2], (3 + k) * vx_ + fx, ps[6 + k]);
+                        for (int k = 0; k < 3; ++k) stg(S.psi[2], ox + k * vx_, ps[6 + k]);
                     }
                     ax = ax + g0;
                     ay = ay + g1;
