@@ -1006,40 +1006,52 @@ __device__ __forceinline__ void pml_strain_axis(const Shot<R> &S, int i, int iz,
     const R *__restrict__ fb = S.v[A == 0 ? 2 : (A == 1 ? 1 : 0)];
     const R *__restrict__ f1 = S.v[A == 2 ? 2 : (A == 1 ? 2 : 1)];
     const R *__restrict__ f2 = S.v[A == 2 ? 1 : 0];
+    // pml_corr<A, 0, 0>, <A, 1, 1>, <A, 2, 1> written out with their memory
+    // variables and profile values loaded together with the taps (a memory
+    // variable's store would otherwise keep the next one's load behind it).
+    // Their addresses are formed, and the loads issued, ahead of the taps:
+    // formed after them, the compiler folded the offsets into 64-bit
+    // multiply-adds whose unused high halves sat in tap destination
+    // registers, so these loads waited for most taps to return (eight waves
+    // per SIMD hid it: 215 us per launch at 400x400x200 before and after).
+    // Offsets in 32 bits (the host caps each axis's 6 x band-volume array
+    // under 4 GiB).
+    const int L = S.pml, W = 2 * L + 2;
+    const int bi = ia <= L ? ia : ia - (n - W);
+    int vol, f;
+    if constexpr (A == 0) {
+        vol = W * S.ny * S.nx;
+        f = (bi * S.ny + iy) * S.nx + ix;
+    } else if constexpr (A == 1) {
+        vol = S.nz * W * S.nx;
+        f = (iz * W + bi) * S.nx + ix;
+    } else {
+        vol = S.nz * S.ny * W;
+        f = (iz * S.ny + iy) * W + bi;
+    }
+    R *__restrict__ psi = S.psi[A];
+    const R *pr0 = S.prof[A], *pr1 = S.prof[A] + 3 * n;
+    const R q0 = psi[f], q1 = psi[f + vol], q2 = psi[f + 2 * vol];
+    const R b0 = pr0[ia], a0 = pr0[n + ia], k0 = pr0[2 * n + ia];
+    const R b1 = pr1[ia], a1 = pr1[n + ia], k1 = pr1[2 * n + ia];
+    // taps from the cell's own address (an index i + d * st folds into the
+    // same kind of 64-bit multiply-add as above)
+    const R *__restrict__ cb = fb + i;
+    const R *__restrict__ c1 = f1 + i;
+    const R *__restrict__ c2 = f2 + i;
     R tb[8], t1[8], t2[8];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         // Db taps ia+j, ia-(j+1); Df taps ia+j+1, ia-j
         const int pb = min(max(ia + j, 0), n - 1), mb = min(max(ia - (j + 1), 0), n - 1);
         const int pf = min(max(ia + j + 1, 0), n - 1), mf = min(max(ia - j, 0), n - 1);
-        tb[2 * j] = fb[i + (pb - ia) * st];
-        tb[2 * j + 1] = fb[i + (mb - ia) * st];
-        t1[2 * j] = f1[i + (pf - ia) * st];
-        t1[2 * j + 1] = f1[i + (mf - ia) * st];
-        t2[2 * j] = f2[i + (pf - ia) * st];
-        t2[2 * j + 1] = f2[i + (mf - ia) * st];
+        tb[2 * j] = cb[(pb - ia) * st];
+        tb[2 * j + 1] = cb[(mb - ia) * st];
+        t1[2 * j] = c1[(pf - ia) * st];
+        t1[2 * j + 1] = c1[(mf - ia) * st];
+        t2[2 * j] = c2[(pf - ia) * st];
+        t2[2 * j + 1] = c2[(mf - ia) * st];
     }
-    // pml_corr<A, 0, 0>, <A, 1, 1>, <A, 2, 1> written out with their memory
-    // variables and profile values loaded together with the taps (a memory
-    // variable's store would otherwise keep the next one's load behind it)
-    const int L = S.pml, W = 2 * L + 2;
-    const int bi = ia <= L ? ia : ia - (n - W);
-    size_t vol, f;
-    if constexpr (A == 0) {
-        vol = (size_t)W * S.ny * S.nx;
-        f = ((size_t)bi * S.ny + iy) * S.nx + ix;
-    } else if constexpr (A == 1) {
-        vol = (size_t)S.nz * W * S.nx;
-        f = ((size_t)iz * W + bi) * S.nx + ix;
-    } else {
-        vol = (size_t)S.nz * S.ny * W;
-        f = ((size_t)iz * S.ny + iy) * W + bi;
-    }
-    R *psi = S.psi[A] + f;
-    const R *pr0 = S.prof[A], *pr1 = S.prof[A] + (size_t)3 * n;
-    const R q0 = psi[0], q1 = psi[vol], q2 = psi[2 * vol];
-    const R b0 = pr0[ia], a0 = pr0[n + ia], k0 = pr0[2 * n + ia];
-    const R b1 = pr1[ia], a1 = pr1[n + ia], k1 = pr1[2 * n + ia];
     R d0 = 0, d1 = 0, d2 = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -1054,9 +1066,9 @@ __device__ __forceinline__ void pml_strain_axis(const Shot<R> &S, int i, int iz,
     const R dd0 = d0 * S.inv_h, dd1 = d1 * S.inv_h, dd2 = d2 * S.inv_h;
     const R p0 = fma_r(b0, q0, a0 * dd0), p1 = fma_r(b1, q1, a1 * dd1),
             p2 = fma_r(b1, q2, a1 * dd2);
-    psi[0] = p0;
-    psi[vol] = p1;
-    psi[2 * vol] = p2;
+    psi[f] = p0;
+    psi[f + vol] = p1;
+    psi[f + 2 * vol] = p2;
     ga[K0] = ga[K0] + fma_r(dd0, k0 - (R)1, p0);
     ga[K1] = ga[K1] + fma_r(dd1, k1 - (R)1, p1);
     ga[K2] = ga[K2] + fma_r(dd2, k1 - (R)1, p2);
