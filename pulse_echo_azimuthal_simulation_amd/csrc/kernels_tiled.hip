@@ -1116,6 +1116,107 @@ __global__ __launch_bounds__(kPmlBlock) void k_pml_corr(Shot<R> S)
     pml_corr_body<R, HALF>(S, blockIdx.x * kPmlBlock + threadIdx.x);
 }
 
+// Stress half-step band pre-pass over the band cells that k_pml_corr_zslab
+// leaves: those of the z slabs also in the y or x band, and all the others.
+template <typename R>
+__global__ __launch_bounds__(kPmlBlock) void k_pml_corr_rest(Shot<R> S)
+{
+    const int b = blockIdx.x * kPmlBlock + threadIdx.x;
+    const int plane = S.ny * S.nx;
+    if (b < (2 * S.pml + 2) * plane) {
+        const int r = b % plane, iy = r / S.nx, ix = r - iy * S.nx;
+        if (!pml_in(iy, S.ny, S.pml) && !pml_in(ix, S.nx, S.pml)) return;
+    }
+    pml_corr_body<R, 0>(S, b);
+}
+
+// The z slabs of the stress half-step pre-pass (band cells 0 .. W*ny*nx of
+// pml_band_cell), columns outside the y and x bands: one thread per (y, x)
+// column, slab and run of kZslabRun planes, marching over them with register
+// rings of the z taps, so that a cell loads one plane of vz, vy, vx instead
+// of its 24 z taps.  The arithmetic is pml_strain_axis<0>'s (the same taps,
+// guards and operation order).  Needs nz >= 2 (L + 1).
+constexpr int kZslabRun = 8;
+
+template <typename R>
+__global__ __launch_bounds__(kPmlBlock) void k_pml_corr_zslab(Shot<R> S)
+{
+    const int nz = S.nz, ny = S.ny, nx = S.nx, L = S.pml, W = 2 * L + 2;
+    const int plane = ny * nx;
+    const int runs = (L + 1 + kZslabRun - 1) / kZslabRun;   // runs per slab
+    const int t = blockIdx.x * kPmlBlock + threadIdx.x;
+    if (t >= 2 * runs * plane) return;
+    const int u = t / plane, r = t - u * plane;
+    const int slab = u & 1, q0r = (u >> 1) * kZslabRun, q1r = min(L + 1, q0r + kZslabRun);
+    const int iy = r / nx, ix = r - iy * nx;
+    if (pml_in(iy, ny, L) || pml_in(ix, nx, L)) return;   // k_pml_corr_rest's
+    const int zb = (slab ? nz - 1 - L : 0) + q0r;  // first plane of the run
+    const int vol = W * plane;                     // one memory variable
+    const size_t n = (size_t)nz * plane;
+    const R *__restrict__ vz = S.v[2] + r;
+    const R *__restrict__ vy = S.v[1] + r;
+    const R *__restrict__ vx = S.v[0] + r;
+    const R *pr0 = S.prof[0], *pr1 = S.prof[0] + 3 * nz;
+    auto pz = [&](int z) { return min(max(z, 0), nz - 1) * plane; };
+    // rz[k] = vz at plane iz - 4 + k (Db taps iz+j, iz-(j+1)); ry, rx at
+    // plane iz - 3 + k (Df taps iz+j+1, iz-j); planes clamped to the grid
+    R rz[8], ry[8], rx[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        rz[k] = vz[pz(zb - 4 + k)];
+        ry[k] = vy[pz(zb - 3 + k)];
+        rx[k] = vx[pz(zb - 3 + k)];
+    }
+#pragma unroll 1
+    for (int q = q0r; q < q1r; ++q) {
+        const int iz = zb + (q - q0r);
+        const int i = iz * plane + r;
+        const int f = (slab * (L + 1) + q) * plane + r;
+        R *__restrict__ psi = S.psi[0];
+        const R q0 = psi[f], q1 = psi[f + vol], q2 = psi[f + 2 * vol];
+        const R b0 = pr0[iz], a0 = pr0[nz + iz], k0 = pr0[2 * nz + iz];
+        const R b1 = pr1[iz], a1 = pr1[nz + iz], k1 = pr1[2 * nz + iz];
+        R d0 = 0, d1 = 0, d2 = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j >= S.nc) break;
+            const R cj = S.c[j];
+            if (iz >= j + 1 && iz < nz - j) d0 = fma_r(cj, rz[4 + j] - rz[3 - j], d0);
+            if (iz >= j && iz < nz - j - 1) {
+                d1 = fma_r(cj, ry[4 + j] - ry[3 - j], d1);
+                d2 = fma_r(cj, rx[4 + j] - rx[3 - j], d2);
+            }
+        }
+        const R dd0 = d0 * S.inv_h, dd1 = d1 * S.inv_h, dd2 = d2 * S.inv_h;
+        const R p0 = fma_r(b0, q0, a0 * dd0), p1 = fma_r(b1, q1, a1 * dd1),
+                p2 = fma_r(b1, q2, a1 * dd2);
+        psi[f] = p0;
+        psi[f + vol] = p1;
+        psi[f + 2 * vol] = p2;
+        R ga[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) ga[k] = 0;
+        ga[2] = ga[2] + fma_r(dd0, k0 - (R)1, p0);
+        ga[3] = ga[3] + fma_r(dd1, k1 - (R)1, p1);
+        ga[4] = ga[4] + fma_r(dd2, k1 - (R)1, p2);
+        // (pml_strain_mask: the z band's components)
+        S.corr[2 * n + i] = ga[2];
+        S.corr[3 * n + i] = ga[3];
+        S.corr[4 * n + i] = ga[4];
+        // next plane: shift the rings, load plane iz + 4 of vz and iz + 5
+        // of vy, vx
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            rz[k] = rz[k + 1];
+            ry[k] = ry[k + 1];
+            rx[k] = rx[k + 1];
+        }
+        rz[7] = vz[pz(iz + 4)];
+        ry[7] = vy[pz(iz + 5)];
+        rx[7] = vx[pz(iz + 5)];
+    }
+}
+
 // Multi-shot kernels: the workgroups of every shot of a batch in one launch.
 // blk_shot maps a (remapped) workgroup id to its shot; a shot whose step
 // counter has reached its step count exits at once.
@@ -1181,7 +1282,16 @@ hipError_t pml_corr_launch(const Shot<R> &s, int half, hipStream_t st)
 {
     const long long cells = pml_band_cells(s.nz, s.ny, s.nx, s.pml);
     const int nb = (int)((cells + kPmlBlock - 1) / kPmlBlock);
-    if (half == 0)
+    const int W = 2 * s.pml + 2;
+    if (half == 0 && s.nz >= W) {
+        // the z slabs' columns outside the y and x bands marched, then the
+        // rest of the band cell by cell
+        const long long thr =
+            2LL * ((s.pml + 1 + kZslabRun - 1) / kZslabRun) * s.ny * s.nx;
+        hipLaunchKernelGGL((k_pml_corr_zslab<R>), dim3((int)((thr + kPmlBlock - 1) / kPmlBlock)),
+                           dim3(kPmlBlock), 0, st, s);
+        hipLaunchKernelGGL((k_pml_corr_rest<R>), dim3(nb), dim3(kPmlBlock), 0, st, s);
+    } else if (half == 0)
         hipLaunchKernelGGL((k_pml_corr<R, 0>), dim3(nb), dim3(kPmlBlock), 0, st, s);
     else
         hipLaunchKernelGGL((k_pml_corr<R, 1>), dim3(nb), dim3(kPmlBlock), 0, st, s);
